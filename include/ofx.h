@@ -1,0 +1,151 @@
+/* ofx.h -- C ABI of the MI355X dense flow engine (libofx.so).
+ *
+ * The drop-in boundary of the reference's fine-stage plugin: the `FineEngine` trait
+ * (rust/crates/optflow-core/src/engine.rs:31-125) with its error type `EngineError(String)`
+ * (engine.rs:22-24), extended by whole-pair and sequence entry points that run the coarse stage,
+ * confidence, guard and strain on the GPU as well (the reference runs those on the CPU,
+ * rust/README.md:137-138).
+ *
+ * Conventions
+ *  - Images are row-major, width w (columns, x) by height h (rows, y), index y * w + x; the
+ *    reference's Image<f32> (rust/crates/optflow-core/src/image.rs:1-26).
+ *  - Every pointer argument is caller-owned. Functions named *_device take device (HBM) pointers
+ *    valid on the engine's device; all others take host pointers. No pointer is kept after a call
+ *    returns.
+ *  - Every function returns OFX_OK (0) or an error status; the message of the last failure on
+ *    the calling thread is ofx_last_error(). There is no CPU fallback: a failure is reported,
+ *    never hidden (rust/README.md:118-122).
+ *  - One engine runs one call at a time per lane; a call that finds every lane busy fails with
+ *    OFX_EBUSY instead of waiting (rust/crates/optflow-gpu/src/engine.rs:664-681).
+ */
+#ifndef OFX_H
+#define OFX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OFX_OK 0
+#define OFX_EINVAL 1    /* bad argument or configuration (the reference's validate messages) */
+#define OFX_ETOOLARGE 2 /* window exceeds device limits (GpuError::TooLarge, gpu/src/lib.rs:14-31) */
+#define OFX_EDEVICE 3   /* HIP runtime error (GpuError::Command) */
+#define OFX_ENODEVICE 4 /* no usable HIP device (GpuError::NoDevice) */
+#define OFX_EBUSY 5     /* every lane of the engine is in use */
+#define OFX_EUNSUPPORTED 6
+
+#define OFX_MAX_SCALES 16
+
+/* Settings of the path: the fields and defaults of python/optflow_ref/config.py:24-90 and
+ * rust/crates/optflow-core/src/config.rs:487-557 that the engine consumes. */
+typedef struct ofx_config {
+    int grid_step;            /* 16 */
+    int poc_win;              /* 64 */
+    double max_disp;          /* 32 */
+    double poc_peak_min;      /* 0.1 */
+    double median_thresh;     /* 2.0 */
+    double lam;               /* 8 */
+    int n_scales;             /* 3 */
+    double scales[OFX_MAX_SCALES]; /* 0.6, 1.0, 1.6 */
+    int n_orient;             /* 4 */
+    double bandwidth;         /* 0.5 */
+    double dc_cut;            /* 2.0 */
+    double sigma_w;           /* 4 */
+    int shape_order;          /* 1 (2 is refused with OFX_EUNSUPPORTED) */
+    double sigma_s;           /* 4 */
+    int outer_iters;          /* 8 */
+    int inner_iters;          /* 2 */
+    double conf_min;          /* 0.2 */
+    double edge_guard;        /* -1: automatic */
+    double edge_guard_min_area; /* -1: automatic */
+    double phase_tol;         /* 0.4 */
+    int strain_win;           /* 15 */
+    double ref_update_conf;   /* 0.5 */
+    int seq_hybrid;           /* 0: seq_mode "chain", 1: "hybrid" */
+    int fb_max;               /* 10 */
+    double fb_radius;         /* -1 */
+    int fb_min_support;       /* 20 */
+    double fb_tol;            /* 0.5 */
+    int prewarp_freeze_at;    /* 0: off */
+} ofx_config;
+
+typedef struct ofx_engine ofx_engine;
+
+/* Fills *cfg with the reference defaults. */
+void ofx_config_default(ofx_config *cfg);
+
+/* Engine on HIP device `device` with `lanes` concurrent calls (one HIP stream each; the
+ * reference's OPTFLOW_GPU_LANES, gpu/src/engine.rs:72-95). Replaces GpuEngine::new. */
+int ofx_create(int device, int lanes, ofx_engine **out);
+void ofx_destroy(ofx_engine *e);
+
+/* Message of the last failed call on this thread (EngineError's text). */
+const char *ofx_last_error(void);
+
+/* FineEngine::lanes (engine.rs:68-75). */
+int ofx_lanes(const ofx_engine *e);
+
+/* Per-stage GPU time, the GPU counterpart of the reference's OPTFLOW_GPU_PROFILE=1 stage timer
+ * (rust/crates/optflow-gpu/src/engine.rs:111; stage names of rust/crates/optflow-core/src/prof.rs:17-44).
+ * ofx_profile(e, 1) starts recording (and zeroes the totals) with one HIP event per stage boundary
+ * on each lane's stream; ofx_profile_read sums the milliseconds and the number of times each
+ * stage ran over all calls since; ofx_profile_stage(i) names stage i (NULL past the last). */
+int ofx_profile(ofx_engine *e, int enable);
+int ofx_profile_read(ofx_engine *e, double *ms, int64_t *calls, int n);
+const char *ofx_profile_stage(int i);
+
+/* FineEngine::returns_slope (engine.rs:82-86): 0, the slope stays on the device. */
+int ofx_returns_slope(const ofx_engine *e);
+
+/* FineEngine::check_window (engine.rs:42-46; gpu/src/engine.rs:1014-1071): OFX_ETOOLARGE before
+ * any work if a w x h window does not fit the device, OFX_EINVAL for a bad configuration. */
+int ofx_check_window(ofx_engine *e, int w, int h, const ofx_config *cfg);
+
+/* FineEngine::release_caches (engine.rs:114-117): frees the per-size device workspaces. */
+int ofx_release_caches(ofx_engine *e);
+
+/* FineEngine::refine_raw (engine.rs:31-125, output RawFlow of phaseflow.rs:1408-1414): the fine
+ * stage of image pair (i1, i2) from the start field (ux, uy). i2 is w2 x h2 and is sampled at
+ * X + shift + s(X); this engine refines whole frames, so it requires w2 == w, h2 == h and a zero
+ * shift (OFX_EUNSUPPORTED otherwise). Outputs are f64, w x h each. */
+int ofx_refine_raw(ofx_engine *e, const float *i1, int w, int h, const float *i2, int w2, int h2,
+                   int64_t shift_x, int64_t shift_y, const float *ux, const float *uy,
+                   const ofx_config *cfg, double *out_ux, double *out_uy, double *out_energy,
+                   double *out_resid);
+
+/* Outputs of one pair or sequence frame: the reference's TrackResult fields and .npy outputs
+ * (rust/crates/optflow-core/src/track.rs:157-209, cli/io.rs:558-600). Any pointer may be NULL
+ * (not written). ux, uy: NaN where invalid; valid: 1 where ux is finite; raw_*: unmasked
+ * field; fb: hybrid sequences only. */
+typedef struct ofx_result {
+    float *ux, *uy, *conf, *raw_ux, *raw_uy;
+    float *exx, *eyy, *exy, *gmax, *vol;
+    uint8_t *valid;
+    int32_t *fb;
+    double t_coarse, t_fine, t_strain; /* seconds (Timing, track.rs:148-161) */
+    int reference;                     /* sequences: reference frame index */
+    double mean_conf;                  /* sequences: mean confidence of the leg */
+} ofx_result;
+
+/* track_pair (rust/crates/optflow-core/src/track.rs:688-861; python/optflow_ref/track.py:115-146):
+ * coarse POC, fine stage, confidence, guard and strain for I1 -> I2 (both w x h). mask: ROI
+ * (nonzero = inside) or NULL; init_ux/init_uy: dense start guess or NULL. Host pointers. */
+int ofx_track_pair(ofx_engine *e, const float *i1, const float *i2, int w, int h,
+                   const uint8_t *mask, const float *init_ux, const float *init_uy,
+                   const ofx_config *cfg, ofx_result *out);
+
+/* The same with every image, mask, guess and output array in device memory (HBM). */
+int ofx_track_pair_device(ofx_engine *e, const float *d_i1, const float *d_i2, int w, int h,
+                          const uint8_t *d_mask, const float *d_init_ux, const float *d_init_uy,
+                          const ofx_config *cfg, ofx_result *d_out);
+
+/* track_sequence in seq_mode "chain" (track.rs:1112-1225; track.py:260-315): frames is n x h x w
+ * (host), results has n - 1 entries for frames 1 .. n - 1. */
+int ofx_track_sequence(ofx_engine *e, const float *frames, int n, int w, int h,
+                       const uint8_t *mask, const ofx_config *cfg, ofx_result *results);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,258 @@
+"""MI355X dense flow engine: Python binding of libofx.so (include/ofx.h).
+
+The functions mirror the reference's Python API (python/optflow_ref/track.py:115 `track_pair`,
+:260 `track_sequence`, phaseflow.py:427 `refine`) with the same argument meaning and the same
+error behaviour (ValueError for bad input or settings, RuntimeError for device failures; no CPU
+fallback). Every computation runs on the GPU through the C ABI; this module only moves arrays.
+"""
+import ctypes
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .config import Config
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libofx.so")
+
+OFX_OK, OFX_EINVAL, OFX_ETOOLARGE, OFX_EDEVICE, OFX_ENODEVICE, OFX_EBUSY, OFX_EUNSUPPORTED = range(7)
+
+# every symbol include/ofx.h declares
+SYMBOLS = ("ofx_config_default", "ofx_create", "ofx_destroy", "ofx_last_error", "ofx_lanes", "ofx_returns_slope",
+           "ofx_check_window", "ofx_release_caches", "ofx_refine_raw", "ofx_track_pair", "ofx_track_pair_device",
+           "ofx_track_sequence", "ofx_profile", "ofx_profile_read", "ofx_profile_stage")
+
+
+class OfxConfig(ctypes.Structure):
+    _fields_ = [("grid_step", ctypes.c_int), ("poc_win", ctypes.c_int), ("max_disp", ctypes.c_double),
+                ("poc_peak_min", ctypes.c_double), ("median_thresh", ctypes.c_double), ("lam", ctypes.c_double),
+                ("n_scales", ctypes.c_int), ("scales", ctypes.c_double * 16), ("n_orient", ctypes.c_int),
+                ("bandwidth", ctypes.c_double), ("dc_cut", ctypes.c_double), ("sigma_w", ctypes.c_double),
+                ("shape_order", ctypes.c_int), ("sigma_s", ctypes.c_double), ("outer_iters", ctypes.c_int),
+                ("inner_iters", ctypes.c_int), ("conf_min", ctypes.c_double), ("edge_guard", ctypes.c_double),
+                ("edge_guard_min_area", ctypes.c_double), ("phase_tol", ctypes.c_double),
+                ("strain_win", ctypes.c_int), ("ref_update_conf", ctypes.c_double), ("seq_hybrid", ctypes.c_int),
+                ("fb_max", ctypes.c_int), ("fb_radius", ctypes.c_double), ("fb_min_support", ctypes.c_int),
+                ("fb_tol", ctypes.c_double), ("prewarp_freeze_at", ctypes.c_int)]
+
+
+_FP = ctypes.POINTER(ctypes.c_float)
+_RES_F32 = ("ux", "uy", "conf", "raw_ux", "raw_uy", "exx", "eyy", "exy", "gmax", "vol")
+
+
+class OfxResult(ctypes.Structure):
+    _fields_ = [(k, _FP) for k in _RES_F32] + [
+        ("valid", ctypes.POINTER(ctypes.c_uint8)), ("fb", ctypes.POINTER(ctypes.c_int32)),
+        ("t_coarse", ctypes.c_double), ("t_fine", ctypes.c_double), ("t_strain", ctypes.c_double),
+        ("reference", ctypes.c_int), ("mean_conf", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libofx.so; raises if it has not been built (there is no fallback)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m optflow_soil_amd.build` "
+                               "(or __graft_entry__.build()); this engine has no CPU fallback")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.ofx_last_error.restype = ctypes.c_char_p
+        _lib.ofx_profile_stage.restype = ctypes.c_char_p
+    return _lib
+
+
+def to_c(cfg: Config) -> OfxConfig:
+    cfg.validate()
+    c = OfxConfig()
+    for name, _ in OfxConfig._fields_:
+        if name in ("n_scales", "scales", "seq_hybrid"):
+            continue
+        setattr(c, name, getattr(cfg, name))
+    c.n_scales = len(cfg.scales)
+    for i, s in enumerate(cfg.scales):
+        c.scales[i] = s
+    c.seq_hybrid = 1 if cfg.seq_mode == "hybrid" else 0
+    return c
+
+
+def _check(rc):
+    if rc == OFX_OK:
+        return
+    msg = lib().ofx_last_error().decode()
+    if rc in (OFX_EINVAL, OFX_ETOOLARGE):
+        raise ValueError(msg)
+    if rc == OFX_EUNSUPPORTED:
+        raise NotImplementedError(msg)
+    raise RuntimeError(msg)
+
+
+def _f32(a):
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+@dataclass
+class Result:
+    """One pair or sequence frame (track.py:15-42): ux, uy NaN where invalid, conf before masking,
+    strain dict (Exx, Eyy, Exy, gmax, vol), raw (unmasked) field, valid mask, timing."""
+    ux: np.ndarray
+    uy: np.ndarray
+    conf: np.ndarray
+    strain: dict
+    raw_ux: np.ndarray
+    raw_uy: np.ndarray
+    valid: np.ndarray
+    timing: dict = field(default_factory=dict)
+
+
+def _new_result(h, w):
+    arrs = {k: np.empty((h, w), np.float32) for k in _RES_F32}
+    valid = np.empty((h, w), np.uint8)
+    r = OfxResult()
+    for k in _RES_F32:
+        setattr(r, k, arrs[k].ctypes.data_as(_FP))
+    r.valid = valid.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
+    r.fb = None
+    return r, arrs, valid
+
+
+def _to_result(r, arrs, valid):
+    strain = {"Exx": arrs["exx"], "Eyy": arrs["eyy"], "Exy": arrs["exy"], "gmax": arrs["gmax"], "vol": arrs["vol"]}
+    timing = {"coarse": r.t_coarse, "fine": r.t_fine, "strain": r.t_strain, "reference": r.reference,
+              "mean_conf": r.mean_conf, "device": "gpu"}
+    return Result(arrs["ux"], arrs["uy"], arrs["conf"], strain, arrs["raw_ux"], arrs["raw_uy"], valid.astype(bool),
+                  timing)
+
+
+class Engine:
+    """A GPU engine (one HIP stream per lane). Mirrors FineEngine (core/engine.rs:31-125)."""
+
+    def __init__(self, device=0, lanes=1):
+        self._h = ctypes.c_void_p()
+        _check(lib().ofx_create(int(device), int(lanes), ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            lib().ofx_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def lanes(self):
+        return lib().ofx_lanes(self._h)
+
+    def profile(self, enable=True):
+        """Start (and zero) or stop the per-stage GPU timer."""
+        _check(lib().ofx_profile(self._h, int(enable)))
+
+    def profile_read(self):
+        """{stage: (milliseconds, calls)} since profile(True)."""
+        names = []
+        while True:
+            nm = lib().ofx_profile_stage(len(names))
+            if nm is None:
+                break
+            names.append(nm.decode())
+        ms = (ctypes.c_double * len(names))()
+        calls = (ctypes.c_int64 * len(names))()
+        _check(lib().ofx_profile_read(self._h, ms, calls, len(names)))
+        return {n: (ms[i], calls[i]) for i, n in enumerate(names)}
+
+    def check_window(self, w, h, cfg):
+        _check(lib().ofx_check_window(self._h, int(w), int(h), ctypes.byref(to_c(cfg))))
+
+    def release_caches(self):
+        _check(lib().ofx_release_caches(self._h))
+
+    def refine_raw(self, i1, i2, ux, uy, cfg):
+        """The fine stage from the start field (ux, uy): returns (ux, uy, energy, resid), float64."""
+        i1, i2, ux, uy = _f32(i1), _f32(i2), _f32(ux), _f32(uy)
+        h, w = i1.shape
+        o = [np.empty((h, w)) for _ in range(4)]
+        c = to_c(cfg)
+        _check(lib().ofx_refine_raw(self._h, _p(i1), w, h, _p(i2), i2.shape[1], i2.shape[0], ctypes.c_int64(0),
+                                    ctypes.c_int64(0), _p(ux), _p(uy), ctypes.byref(c), *[_p(a) for a in o]))
+        return tuple(o)
+
+    def track_pair(self, I1, I2, cfg, mask=None, init=None) -> Result:
+        """track.py:115-146: displacement of I1's pixels into I2."""
+        I1, I2 = _f32(I1), _f32(I2)
+        if I1.shape != I2.shape:
+            raise ValueError(f"image sizes differ: {I1.shape} vs {I2.shape}")
+        if mask is not None and np.shape(mask) != I1.shape:
+            raise ValueError(f"ROI mask shape {np.shape(mask)} differs from the image shape {I1.shape}")
+        h, w = I1.shape
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        ix = iy = None
+        if init is not None:
+            ix, iy = _f32(init[0]), _f32(init[1])
+        r, arrs, valid = _new_result(h, w)
+        c = to_c(cfg)
+        _check(lib().ofx_track_pair(self._h, _p(I1), _p(I2), w, h, _p(m), _p(ix), _p(iy), ctypes.byref(c),
+                                    ctypes.byref(r)))
+        return _to_result(r, arrs, valid)
+
+    def track_pair_device(self, d_i1, d_i2, w, h, cfg, d_out, d_mask=0, d_init=(0, 0)):
+        """Device-pointer form (integers from e.g. torch's data_ptr()); d_out maps result names to
+        device pointers. Returns the stage times."""
+        r = OfxResult()
+        for k in _RES_F32:
+            setattr(r, k, ctypes.cast(ctypes.c_void_p(d_out.get(k, 0) or None), _FP))
+        r.valid = ctypes.cast(ctypes.c_void_p(d_out.get("valid", 0) or None), ctypes.POINTER(ctypes.c_uint8))
+        c = to_c(cfg)
+        _check(lib().ofx_track_pair_device(self._h, ctypes.c_void_p(d_i1), ctypes.c_void_p(d_i2), int(w), int(h),
+                                           ctypes.c_void_p(d_mask or None), ctypes.c_void_p(d_init[0] or None),
+                                           ctypes.c_void_p(d_init[1] or None), ctypes.byref(c), ctypes.byref(r)))
+        return {"coarse": r.t_coarse, "fine": r.t_fine, "strain": r.t_strain}
+
+    def track_sequence(self, frames, cfg, mask=None):
+        """track.py:260-315 (chain): list of (k, Result) for k = 1 .. n - 1."""
+        fr = _f32(np.stack(frames))
+        n, h, w = fr.shape
+        if mask is not None and np.shape(mask) != (h, w):
+            raise ValueError(f"ROI mask shape {np.shape(mask)} differs from the frame shape {(h, w)}")
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        if m is not None and not m.any():
+            raise ValueError("ROI mask is empty")
+        rs = (OfxResult * max(n - 1, 1))()
+        keep = []
+        for k in range(n - 1):
+            r, arrs, valid = _new_result(h, w)
+            rs[k] = r
+            keep.append((arrs, valid))
+        c = to_c(cfg)
+        _check(lib().ofx_track_sequence(self._h, _p(fr), n, w, h, _p(m), ctypes.byref(c), rs))
+        return [(k + 1, _to_result(rs[k], *keep[k])) for k in range(n - 1)]
+
+
+_default = None
+
+
+def default_engine() -> Engine:
+    global _default
+    if _default is None:
+        _default = Engine()
+    return _default
+
+
+def track_pair(I1, I2, cfg=None, mask=None, init=None) -> Result:
+    return default_engine().track_pair(I1, I2, cfg or Config(), mask=mask, init=init)
+
+
+def track_sequence(frames, cfg=None, mask=None):
+    return default_engine().track_sequence(frames, cfg or Config(), mask=mask)
+
+
+def refine(I1, I2, ux, uy, cfg=None):
+    return default_engine().refine_raw(I1, I2, ux, uy, cfg or Config())

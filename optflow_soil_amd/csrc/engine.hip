@@ -1,0 +1,848 @@
+// The engine: per-size device workspaces, the pair pipeline (coarse stage, fine stage,
+// confidence, guard, strain) on one HIP stream per lane, the chained sequence, and the C ABI of
+// include/ofx.h.
+//
+// Reference call structure: rust/crates/optflow-core/src/track.rs:567-861 (pair),
+// rust/crates/optflow-core/src/phaseflow.rs:1514-1613 (fine stage loop),
+// rust/crates/optflow-core/src/track.rs:1112-1225 (chained sequence),
+// rust/crates/optflow-gpu/src/engine.rs:633-1071 (lanes, check_window, errors).
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+
+#include "fine.hpp"
+
+namespace ofx {
+
+namespace {
+
+thread_local std::string g_last_error;
+
+std::vector<double> gauss_taps(double sigma) {
+    const int r = (int)(4.0 * sigma + 0.5);
+    std::vector<double> k(2 * r + 1);
+    double s = 0;
+    for (int i = -r; i <= r; i++) {
+        const double t = i / sigma;
+        k[i + r] = std::exp(-0.5 * t * t);
+        s += k[i + r];
+    }
+    for (auto &v : k) v /= s;
+    return k;
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    ~DevBuf() {
+        if (p) (void)hipFree(p);
+    }
+    template <class T>
+    T *get(size_t n) {
+        const size_t b = n * sizeof(T);
+        if (bytes < b) {
+            if (p) (void)hipFree(p);
+            p = nullptr;
+            bytes = 0;
+            HIPCHECK(hipMalloc(&p, b));
+            bytes = b;
+        }
+        return (T *)p;
+    }
+};
+
+}  // namespace
+
+// Per-stage GPU time (the stages of rust/crates/optflow-core/src/prof.rs:17-44): an event marks
+// the start of every stage on the lane's stream; at the end of a call the time between
+// consecutive marks is added to the earlier mark's stage.
+enum Stage {
+    ST_COARSE, ST_REFERENCE, ST_PREWARP, ST_SMOOTH, ST_WARP, ST_RESPONSES, ST_ACCUMULATE, ST_MOMENTS, ST_SOLVE,
+    ST_CONFIDENCE, ST_GUARD, ST_STRAIN, ST_END, ST_COUNT
+};
+const char *const STAGE_NAMES[ST_COUNT] = {"coarse", "reference", "prewarp", "smooth", "warp", "responses",
+                                           "accumulate", "moments", "solve", "confidence", "guard", "strain", "end"};
+
+struct Profiler {
+    bool on = false;
+    std::vector<hipEvent_t> pool;
+    std::vector<int> stage;
+    size_t used = 0;
+    double ms[ST_COUNT] = {0};
+    long long calls[ST_COUNT] = {0};
+    void mark(int st, hipStream_t s) {
+        if (!on) return;
+        if (used == pool.size()) {
+            hipEvent_t e;
+            HIPCHECK(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        HIPCHECK(hipEventRecord(pool[used], s));
+        if (stage.size() < pool.size()) stage.resize(pool.size());
+        stage[used++] = st;
+    }
+    void flush() {
+        if (!on || used == 0) return;
+        HIPCHECK(hipEventSynchronize(pool[used - 1]));
+        for (size_t i = 0; i + 1 < used; i++) {
+            float t = 0;
+            HIPCHECK(hipEventElapsedTime(&t, pool[i], pool[i + 1]));
+            ms[stage[i]] += t;
+            calls[stage[i]] += 1;
+        }
+        used = 0;
+    }
+    void reset() {
+        used = 0;
+        for (int i = 0; i < ST_COUNT; i++) {
+            ms[i] = 0;
+            calls[i] = 0;
+        }
+    }
+    ~Profiler() {
+        for (auto e : pool) (void)hipEventDestroy(e);
+    }
+};
+
+// Device buffers of one lane, grown to the largest window seen (release_caches frees them).
+struct Workspace {
+    Profiler prof;
+    int w = 0, h = 0, F = 0;
+    DevBuf bi1, bi2, bmask, binx, biny;                 // staging of host inputs
+    DevBuf bc2, bspec, btmp, bresp, brefp;
+    DevBuf bux, buy, bnux, bnuy, bsx, bsy, bcsx, bcsy, bwarped, bfx, bfy, bbt;
+    DevBuf bsums, bym, benergy, bresid, bslope, bmed, bmedscr;
+    DevBuf bconf, bconf64, blow, bkept, bwithin, bok, bcold2, bcolrow, bcnt;
+    DevBuf bparent, bsizes, blarge, bsrc, bnear, bmx, bmy, bytmp;
+    DevBuf bstart_x, bstart_y;
+    DevBuf bout;                                       // staging of host outputs (f32 x 10 + u8)
+    DevBuf btaps_s, btaps_w;
+    double taps_s_sigma = -1, taps_w_sigma = -1;
+    int taps_s_r = 0, taps_w_r = 0;
+    // sequence state
+    DevBuf bu0x, bu0y, bc0r, bcx, bcy, bconf0, bca, bprev_x, bprev_y, bprev_c, bleg_c, bmean, bframe_prev, bframe_ref;
+    CoarseWs *coarse = nullptr;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    ~Workspace() {
+        coarse_release(coarse);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+
+struct Lane {
+    hipStream_t stream = nullptr;
+    std::unique_ptr<Workspace> ws;
+    std::atomic<bool> busy{false};
+};
+
+}  // namespace ofx
+
+struct ofx_engine {
+    int device = 0;
+    std::vector<std::unique_ptr<ofx::Lane>> lanes;
+};
+
+namespace ofx {
+
+namespace {
+
+struct LaneGuard {
+    Lane *lane = nullptr;
+    explicit LaneGuard(ofx_engine *e) {
+        if (!e) throw Error(OFX_EINVAL, "engine is NULL");
+        for (auto &l : e->lanes) {
+            bool expected = false;
+            if (l->busy.compare_exchange_strong(expected, true)) {
+                lane = l.get();
+                break;
+            }
+        }
+        if (!lane) throw Error(OFX_EBUSY, "every lane of the GPU engine is busy");
+        HIPCHECK(hipSetDevice(e->device));
+        if (!lane->ws) lane->ws.reset(new Workspace);
+    }
+    ~LaneGuard() {
+        if (lane) lane->busy.store(false);
+    }
+};
+
+double max_scale(const ofx_config &c) {
+    double m = c.scales[0];
+    for (int i = 1; i < c.n_scales; i++) m = std::max(m, c.scales[i]);
+    return m;
+}
+double auto_edge_guard(const ofx_config &c) { return 3.0 * c.lam * max_scale(c) / (2 * M_PI * c.bandwidth); }
+double edge_guard(const ofx_config &c) { return c.edge_guard >= 0 ? c.edge_guard : auto_edge_guard(c); }
+double edge_guard_min_area(const ofx_config &c) {
+    if (c.edge_guard_min_area >= 0) return c.edge_guard_min_area;
+    const double l = c.lam * max_scale(c);
+    return 2.0 * l * l;
+}
+
+// The reference's checks (python/optflow_ref/config.py:166-245) for the keys the engine reads.
+void validate(const ofx_config *cfg) {
+    if (!cfg) throw Error(OFX_EINVAL, "config is NULL");
+    const ofx_config &c = *cfg;
+    if (c.n_scales < 1 || c.n_scales > OFX_MAX_SCALES) throw Error(OFX_EINVAL, "scales must not be empty");
+    if (c.poc_win < 16 || (c.poc_win & (c.poc_win - 1))) throw Error(OFX_EINVAL, "poc_win must be a power of two >= 16");
+    if (c.grid_step < 1 || c.strain_win < 5 || c.strain_win % 2 == 0)
+        throw Error(OFX_EINVAL, "grid_step >= 1 and strain_win odd >= 5 required");
+    if (!(c.dc_cut > 0)) throw Error(OFX_EINVAL, "dc_cut must be positive");
+    if (!(c.phase_tol > 0)) throw Error(OFX_EINVAL, "phase_tol must be positive");
+    if (c.shape_order != 1 && c.shape_order != 2) throw Error(OFX_EINVAL, "shape_order must be 1 or 2");
+    double mins = c.scales[0];
+    for (int i = 1; i < c.n_scales; i++) mins = std::min(mins, c.scales[i]);
+    if (c.lam * mins < 3 || !(c.sigma_w > 0) || !(c.sigma_s > 0))
+        throw Error(OFX_EINVAL, "shortest wavelength lam*min(scales) must be >= 3 px; sigmas must be positive");
+    if (!(c.bandwidth > 0) || !(c.max_disp > 0) || !(c.median_thresh > 0))
+        throw Error(OFX_EINVAL, "bandwidth, max_disp and median_thresh must be positive");
+    if (c.n_orient < 1 || c.outer_iters < 1 || c.inner_iters < 1)
+        throw Error(OFX_EINVAL, "n_orient, outer_iters and inner_iters must be >= 1");
+    if (!(c.conf_min >= 0 && c.conf_min <= 1) || !(c.poc_peak_min >= 0 && c.poc_peak_min <= 1))
+        throw Error(OFX_EINVAL, "conf_min and poc_peak_min must lie in [0, 1]");
+    if (c.prewarp_freeze_at < 0) throw Error(OFX_EINVAL, "prewarp_freeze_at must be >= 0");
+    if (c.n_orient * c.n_scales > 64 || c.sigma_w > 256 || c.sigma_s > 256 || c.strain_win > 255 || c.poc_win > 1024)
+        throw Error(OFX_EINVAL,
+                    "at most 64 filters (n_orient * len(scales)); sigma_w, sigma_s <= 256, strain_win <= 255 and "
+                    "poc_win <= 1024 required");
+    if (c.shape_order == 2)
+        throw Error(OFX_EUNSUPPORTED, "shape_order 2 (quadratic shape function) is not implemented on the GPU yet");
+}
+
+void check_window_impl(int w, int h, const ofx_config &c) {
+    if (w < 1 || h < 1) throw Error(OFX_EINVAL, "empty window");
+    if (w <= PAD || h <= PAD) throw Error(OFX_EINVAL, "image smaller than the 32 px reflect pad");
+    if (w + 2 * PAD > 4096 || h + 2 * PAD > 4096)
+        throw Error(OFX_ETOOLARGE, "window " + std::to_string(w) + "x" + std::to_string(h) +
+                                       " exceeds 4032 px per side (one padded line must fit a workgroup's LDS)");
+    (void)c;
+}
+
+template <class T>
+T *B(DevBuf &b, size_t n) {
+    return b.get<T>(n);
+}
+
+void set_taps(Workspace &ws, const ofx_config &c, hipStream_t s) {
+    if (ws.taps_s_sigma != c.sigma_s) {
+        std::vector<double> k = gauss_taps(c.sigma_s);
+        double *d = ws.btaps_s.get<double>(k.size());
+        HIPCHECK(hipMemcpy(d, k.data(), k.size() * sizeof(double), hipMemcpyHostToDevice));
+        ws.taps_s_sigma = c.sigma_s;
+        ws.taps_s_r = (int)k.size() / 2;
+    }
+    if (ws.taps_w_sigma != c.sigma_w) {
+        std::vector<double> k = gauss_taps(c.sigma_w);
+        const int r = (int)k.size() / 2, klen = 2 * r + 1;
+        std::vector<double> kt(3 * klen);
+        for (int p = 0; p < 3; p++)
+            for (int t = -r; t <= r; t++) kt[p * klen + t + r] = k[t + r] * std::pow((double)t, p);
+        double *d = ws.btaps_w.get<double>(kt.size());
+        HIPCHECK(hipMemcpy(d, kt.data(), kt.size() * sizeof(double), hipMemcpyHostToDevice));
+        ws.taps_w_sigma = c.sigma_w;
+        ws.taps_w_r = r;
+    }
+    (void)s;
+}
+
+// The fine stage (phaseflow.py:427-511, phaseflow.rs:1514-1613) from the start field in
+// ws.bux / ws.buy (double). Leaves ux, uy, energy, resid, the energy median (ws.bmed[0]) and the
+// final confidence (ws.bconf as float, ws.bconf64 as double) in the workspace.
+void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h, const ofx_config &c, hipStream_t s) {
+    const int wp = w + 2 * PAD, hp = h + 2 * PAD;
+    const size_t n = (size_t)w * h, np = (size_t)wp * hp;
+    const Bank bank = make_bank(c);
+    const int F = bank.F;
+    set_taps(ws, c, s);
+    float2 *spec = B<float2>(ws.bspec, np), *tmp = B<float2>(ws.btmp, np);
+    float2 *resp = B<float2>(ws.bresp, std::max<size_t>(F, 3) * np);
+    float4 *refp = B<float4>(ws.brefp, F * n);
+    // reference pack of image 1 (phaseflow.py:100-111): three inverses per filter
+    ws.prof.mark(ST_REFERENCE, s);
+    fft_forward_padded(i1, w, h, spec, tmp, s);
+    for (int k = 0; k < F; k++) {
+        fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G, false, tmp, resp, s);
+        fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G_IFX, false, tmp, resp + np, s);
+        fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G_IFY, false, tmp, resp + 2 * np, s);
+        refpack(resp, resp + np, resp + 2 * np, refp + k * n, w, h, wp, s);
+    }
+    // spline of image 2
+    ws.prof.mark(ST_WARP, s);
+    float *c2 = B<float>(ws.bc2, n);
+    HIPCHECK(hipMemcpyAsync(c2, i2, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    prefilter_real(c2, w, h, s);
+    double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
+    double *nux = B<double>(ws.bnux, n), *nuy = B<double>(ws.bnuy, n);
+    float *sx = B<float>(ws.bsx, n), *sy = B<float>(ws.bsy, n), *csx = B<float>(ws.bcsx, n), *csy = B<float>(ws.bcsy, n);
+    float *warped = B<float>(ws.bwarped, n);
+    double *fx = B<double>(ws.bfx, n), *fy = B<double>(ws.bfy, n), *bt = B<double>(ws.bbt, n);
+    double *sums = B<double>(ws.bsums, 7 * n), *ym = B<double>(ws.bym, 15 * n);
+    double *energy = B<double>(ws.benergy, n), *resid = B<double>(ws.bresid, n), *slope = B<double>(ws.bslope, 4 * n);
+    double *med = B<double>(ws.bmed, 4);
+    void *medscr = ws.bmedscr.get<char>(median_scratch_bytes());
+    uint8_t *low = B<uint8_t>(ws.blow, n), *kept = B<uint8_t>(ws.bkept, n), *within = B<uint8_t>(ws.bwithin, n);
+    uint8_t *ok = B<uint8_t>(ws.bok, n);
+    int *cold2 = B<int>(ws.bcold2, n), *colrow = B<int>(ws.bcolrow, n);
+    Counters *cnt = B<Counters>(ws.bcnt, 1);
+    float *conf = B<float>(ws.bconf, n);
+    double *conf64 = B<double>(ws.bconf64, n);
+    const double R = auto_edge_guard(c);
+    const int freeze = c.prewarp_freeze_at;
+    bool have_kept = false;
+    for (int it = 0; it < c.outer_iters; it++) {
+        if (it == 0) {
+            ws.prof.mark(ST_SMOOTH, s);
+            blur(ux, bt, sx, w, h, (const double *)ws.btaps_s.p, ws.taps_s_r, s);
+            blur(uy, bt, sy, w, h, (const double *)ws.btaps_s.p, ws.taps_s_r, s);
+        } else {
+            // trusted pre-warp (phaseflow.py:326-355; frozen low mask: prewarp.rs:48-115)
+            ws.prof.mark(ST_PREWARP, s);
+            const uint8_t *lowm = low;
+            if (freeze >= 1 && it > freeze && have_kept) {
+                lowm = kept;
+            } else {
+                confidence(energy, resid, med, ux, uy, c.phase_tol, c.conf_min, 0.0, w, h, low, nullptr, nullptr, s);
+                if (freeze >= 1 && it == freeze) {
+                    HIPCHECK(hipMemcpyAsync(kept, low, n, hipMemcpyDeviceToDevice, s));
+                    have_kept = true;
+                }
+            }
+            column_nearest(lowm, w, h, cold2, colrow, s);
+            within_distance(cold2, w, h, R, within, s);
+            prewarp_trusted(lowm, within, w, h, ok, cnt, s);
+            column_nearest(ok, w, h, cold2, colrow, s);
+            prewarp_fill(ux, uy, slope, ok, cold2, colrow, cnt, w, h, fx, fy, s);
+            ws.prof.mark(ST_SMOOTH, s);
+            blur(fx, bt, sx, w, h, (const double *)ws.btaps_s.p, ws.taps_s_r, s);
+            blur(fy, bt, sy, w, h, (const double *)ws.btaps_s.p, ws.taps_s_r, s);
+        }
+        ws.prof.mark(ST_WARP, s);
+        warp(c2, sx, sy, warped, w, h, s);
+        ws.prof.mark(ST_RESPONSES, s);
+        fft_forward_padded(warped, w, h, spec, tmp, s);
+        for (int k = 0; k < F; k++) fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G, true, tmp, resp + k * np, s);
+        ws.prof.mark(ST_SMOOTH, s);
+        HIPCHECK(hipMemcpyAsync(csx, sx, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        HIPCHECK(hipMemcpyAsync(csy, sy, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        prefilter_real(csx, w, h, s);
+        prefilter_real(csy, w, h, s);
+        for (int in = 0; in < c.inner_iters; in++) {
+            ws.prof.mark(ST_ACCUMULATE, s);
+            accumulate(ux, uy, sx, sy, csx, csy, resp, refp, sums, w, h, F, s);
+            ws.prof.mark(ST_MOMENTS, s);
+            moments_solve(sums, ym, ux, uy, nux, nuy, energy, resid, slope, w, h, (const double *)ws.btaps_w.p,
+                          ws.taps_w_r, s);
+            ws.prof.mark(ST_SOLVE, s);
+            median(energy, n, medscr, med, s);
+            energy_reset(energy, med, ux, uy, nux, nuy, resid, slope, n, s);
+            std::swap(ux, nux);
+            std::swap(uy, nuy);
+        }
+    }
+    // the final field lives in ux / uy; keep the workspace's names pointing at it
+    if (ux != (double *)ws.bux.p) {
+        std::swap(ws.bux.p, ws.bnux.p);
+        std::swap(ws.bux.bytes, ws.bnux.bytes);
+        std::swap(ws.buy.p, ws.bnuy.p);
+        std::swap(ws.buy.bytes, ws.bnuy.bytes);
+    }
+    const double Bw = edge_guard(c) > 0 ? 4.0 * c.sigma_w : 0.0;
+    ws.prof.mark(ST_CONFIDENCE, s);
+    confidence(energy, resid, med, ux, uy, c.phase_tol, c.conf_min, Bw, w, h, nullptr, conf, conf64, s);
+}
+
+// Validity, guard and strain (track.py:56-108) of the field (ux, uy) with confidence conf (float).
+void finish_device(Workspace &ws, const double *ux, const double *uy, const float *conf, int w, int h,
+                   const uint8_t *mask, const ofx_config &c, ofx_result *o, hipStream_t s) {
+    const size_t n = (size_t)w * h;
+    uint8_t *low = B<uint8_t>(ws.blow, n), *near = B<uint8_t>(ws.bnear, n);
+    const double guard = edge_guard(c);
+    const int guard_on = guard > 0;
+    ws.prof.mark(ST_GUARD, s);
+    if (guard_on) {
+        uint8_t *large = B<uint8_t>(ws.blarge, n), *src = B<uint8_t>(ws.bsrc, n);
+        int *parent = B<int>(ws.bparent, n);
+        unsigned int *sizes = B<unsigned int>(ws.bsizes, n);
+        int *cold2 = B<int>(ws.bcold2, n), *colrow = B<int>(ws.bcolrow, n);
+        low_mask(conf, c.conf_min, n, low, s);
+        large_components(low, w, h, edge_guard_min_area(c), parent, sizes, large, s);
+        guard_sources(large, mask, w, h, src, s);
+        column_nearest(src, w, h, cold2, colrow, s);
+        within_distance(cold2, w, h, guard, near, s);
+    }
+    double *mx = B<double>(ws.bmx, n), *my = B<double>(ws.bmy, n), *yt = B<double>(ws.bytmp, 11 * n);
+    validity(conf, c.conf_min, mask, near, guard, guard_on, ux, uy, w, h, mx, my, s);
+    ws.prof.mark(ST_STRAIN, s);
+    strain(mx, my, w, h, c.strain_win, yt, o->ux, o->uy, o->valid, o->exx, o->eyy, o->exy, o->gmax, o->vol, s);
+}
+
+__global__ void k_to_f32(const double *a, float *o, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = (float)a[i];
+}
+__global__ void k_to_f64(const float *a, double *o, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) o[i] = (double)a[i];
+}
+void to_f32(const double *a, float *o, size_t n, hipStream_t s) {
+    if (!o) return;
+    hipLaunchKernelGGL(k_to_f32, dim3((n + 255) / 256), dim3(256), 0, s, a, o, n);
+    LAUNCHCHECK();
+}
+void to_f64(const float *a, double *o, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_to_f64, dim3((n + 255) / 256), dim3(256), 0, s, a, o, n);
+    LAUNCHCHECK();
+}
+
+__global__ void k_min_inplace(double *a, const double *b, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = fmin(a[i], b[i]);
+}
+__global__ void k_min_pair(const double *a, const double *b, double *o, float *of, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        o[i] = fmin(a[i], b[i]);
+        of[i] = (float)o[i];
+    }
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, a, b));
+    return ms;
+}
+
+// The pair pipeline (track.py:115-146) on device inputs; init as double (or NULL); outputs to
+// device result arrays. Timing by events on the lane's stream.
+void pair_device(Workspace &ws, const float *i1, const float *i2, int w, int h, const uint8_t *mask,
+                 const double *init_x, const double *init_y, const ofx_config &c, ofx_result *o, hipStream_t s) {
+    const size_t n = (size_t)w * h;
+    for (auto &e : ws.ev)
+        if (!e) HIPCHECK(hipEventCreate(&e));
+    HIPCHECK(hipEventRecord(ws.ev[0], s));
+    ws.prof.mark(ST_COARSE, s);
+    double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
+    coarse_dense(i1, i2, w, h, c, init_x, init_y, ux, uy, ws.coarse, s);
+    HIPCHECK(hipEventRecord(ws.ev[1], s));
+    refine_device(ws, i1, i2, w, h, c, s);
+    HIPCHECK(hipEventRecord(ws.ev[2], s));
+    ux = (double *)ws.bux.p;
+    uy = (double *)ws.buy.p;
+    finish_device(ws, ux, uy, (const float *)ws.bconf.p, w, h, mask, c, o, s);
+    to_f32(ux, o->raw_ux, n, s);
+    to_f32(uy, o->raw_uy, n, s);
+    if (o->conf) HIPCHECK(hipMemcpyAsync(o->conf, ws.bconf.p, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    ws.prof.mark(ST_END, s);
+    HIPCHECK(hipEventRecord(ws.ev[3], s));
+}
+
+void record_times(Workspace &ws, ofx_result *o) {
+    HIPCHECK(hipEventSynchronize(ws.ev[3]));
+    ws.prof.flush();
+    o->t_coarse = elapsed(ws.ev[0], ws.ev[1]) * 1e-3;
+    o->t_fine = elapsed(ws.ev[1], ws.ev[2]) * 1e-3;
+    o->t_strain = elapsed(ws.ev[2], ws.ev[3]) * 1e-3;
+}
+
+// Device staging for the host-pointer API: 10 float planes + the valid plane.
+struct Staged {
+    ofx_result d{};
+};
+
+Staged stage_outputs(Workspace &ws, size_t n, const ofx_result *host) {
+    Staged st;
+    float *base = (float *)ws.bout.get<char>(10 * n * sizeof(float) + n);
+    float **dst[10] = {&st.d.ux, &st.d.uy, &st.d.conf, &st.d.raw_ux, &st.d.raw_uy,
+                       &st.d.exx, &st.d.eyy, &st.d.exy, &st.d.gmax, &st.d.vol};
+    float *const src[10] = {host->ux, host->uy, host->conf, host->raw_ux, host->raw_uy,
+                            host->exx, host->eyy, host->exy, host->gmax, host->vol};
+    for (int k = 0; k < 10; k++) *dst[k] = src[k] ? base + k * n : nullptr;
+    st.d.valid = host->valid ? (uint8_t *)(base + 10 * n) : nullptr;
+    return st;
+}
+
+void download(const Staged &st, ofx_result *host, size_t n, hipStream_t s) {
+    float *const d[10] = {st.d.ux, st.d.uy, st.d.conf, st.d.raw_ux, st.d.raw_uy,
+                          st.d.exx, st.d.eyy, st.d.exy, st.d.gmax, st.d.vol};
+    float *const hst[10] = {host->ux, host->uy, host->conf, host->raw_ux, host->raw_uy,
+                            host->exx, host->eyy, host->exy, host->gmax, host->vol};
+    for (int k = 0; k < 10; k++)
+        if (hst[k]) HIPCHECK(hipMemcpyAsync(hst[k], d[k], n * sizeof(float), hipMemcpyDeviceToHost, s));
+    if (host->valid) HIPCHECK(hipMemcpyAsync(host->valid, st.d.valid, n, hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+}
+
+template <class F>
+int guarded(F &&f) {
+    try {
+        f();
+        return OFX_OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return OFX_EDEVICE;
+    }
+}
+
+}  // namespace
+
+}  // namespace ofx
+
+using namespace ofx;
+
+extern "C" {
+
+void ofx_config_default(ofx_config *c) {
+    std::memset(c, 0, sizeof(*c));
+    c->grid_step = 16;
+    c->poc_win = 64;
+    c->max_disp = 32.0;
+    c->poc_peak_min = 0.1;
+    c->median_thresh = 2.0;
+    c->lam = 8.0;
+    c->n_scales = 3;
+    c->scales[0] = 0.6;
+    c->scales[1] = 1.0;
+    c->scales[2] = 1.6;
+    c->n_orient = 4;
+    c->bandwidth = 0.5;
+    c->dc_cut = 2.0;
+    c->sigma_w = 4.0;
+    c->shape_order = 1;
+    c->sigma_s = 4.0;
+    c->outer_iters = 8;
+    c->inner_iters = 2;
+    c->conf_min = 0.2;
+    c->edge_guard = -1.0;
+    c->edge_guard_min_area = -1.0;
+    c->phase_tol = 0.4;
+    c->strain_win = 15;
+    c->ref_update_conf = 0.5;
+    c->seq_hybrid = 0;
+    c->fb_max = 10;
+    c->fb_radius = -1.0;
+    c->fb_min_support = 20;
+    c->fb_tol = 0.5;
+    c->prewarp_freeze_at = 0;
+}
+
+const char *ofx_last_error(void) { return g_last_error.c_str(); }
+
+int ofx_create(int device, int lanes, ofx_engine **out) {
+    return guarded([&] {
+        if (!out) throw Error(OFX_EINVAL, "out is NULL");
+        *out = nullptr;
+        int count = 0;
+        if (hipGetDeviceCount(&count) != hipSuccess || count == 0) throw Error(OFX_ENODEVICE, "no HIP device found");
+        if (device < 0 || device >= count) throw Error(OFX_ENODEVICE, "HIP device " + std::to_string(device) + " does not exist");
+        if (lanes < 1 || lanes > 8) throw Error(OFX_EINVAL, "lanes must lie in [1, 8]");
+        HIPCHECK(hipSetDevice(device));
+        std::unique_ptr<ofx_engine> e(new ofx_engine);
+        e->device = device;
+        for (int i = 0; i < lanes; i++) {
+            std::unique_ptr<Lane> l(new Lane);
+            HIPCHECK(hipStreamCreateWithFlags(&l->stream, hipStreamNonBlocking));
+            e->lanes.push_back(std::move(l));
+        }
+        *out = e.release();
+    });
+}
+
+void ofx_destroy(ofx_engine *e) {
+    if (!e) return;
+    (void)hipSetDevice(e->device);
+    for (auto &l : e->lanes) {
+        if (l->stream) (void)hipStreamSynchronize(l->stream);
+        l->ws.reset();
+        if (l->stream) (void)hipStreamDestroy(l->stream);
+    }
+    delete e;
+}
+
+int ofx_lanes(const ofx_engine *e) { return e ? (int)e->lanes.size() : 0; }
+
+int ofx_profile(ofx_engine *e, int enable) {
+    return guarded([&] {
+        if (!e) throw Error(OFX_EINVAL, "engine is NULL");
+        for (auto &l : e->lanes) {
+            if (!l->ws) l->ws.reset(new Workspace);
+            l->ws->prof.on = enable != 0;
+            l->ws->prof.reset();
+        }
+    });
+}
+
+int ofx_profile_read(ofx_engine *e, double *ms, int64_t *calls, int n) {
+    return guarded([&] {
+        if (!e) throw Error(OFX_EINVAL, "engine is NULL");
+        for (int i = 0; i < n; i++) {
+            if (ms) ms[i] = 0;
+            if (calls) calls[i] = 0;
+        }
+        for (auto &l : e->lanes) {
+            if (!l->ws) continue;
+            for (int i = 0; i < n && i < ST_COUNT; i++) {
+                if (ms) ms[i] += l->ws->prof.ms[i];
+                if (calls) calls[i] += l->ws->prof.calls[i];
+            }
+        }
+    });
+}
+
+const char *ofx_profile_stage(int i) { return (i >= 0 && i < ST_COUNT) ? STAGE_NAMES[i] : nullptr; }
+
+int ofx_returns_slope(const ofx_engine *) { return 0; }
+
+int ofx_check_window(ofx_engine *e, int w, int h, const ofx_config *cfg) {
+    return guarded([&] {
+        if (!e) throw Error(OFX_EINVAL, "engine is NULL");
+        validate(cfg);
+        check_window_impl(w, h, *cfg);
+        // device memory of one lane's workspace at this size, against what is free
+        const size_t n = (size_t)w * h, np = (size_t)(w + 2 * PAD) * (h + 2 * PAD);
+        const size_t F = (size_t)cfg->n_orient * cfg->n_scales;
+        const size_t need = np * 8 * (std::max<size_t>(F, 3) + 2) + n * (16 * F + 8 * 60 + 4 * 20);
+        HIPCHECK(hipSetDevice(e->device));
+        size_t free_b = 0, total = 0;
+        HIPCHECK(hipMemGetInfo(&free_b, &total));
+        if (need > free_b)
+            throw Error(OFX_ETOOLARGE, "window " + std::to_string(w) + "x" + std::to_string(h) + " needs about " +
+                                           std::to_string(need >> 20) + " MiB of device memory, " +
+                                           std::to_string(free_b >> 20) + " MiB free");
+    });
+}
+
+int ofx_release_caches(ofx_engine *e) {
+    return guarded([&] {
+        if (!e) throw Error(OFX_EINVAL, "engine is NULL");
+        HIPCHECK(hipSetDevice(e->device));
+        for (auto &l : e->lanes) {
+            bool expected = false;
+            if (!l->busy.compare_exchange_strong(expected, true)) continue;  // in use: keep
+            HIPCHECK(hipStreamSynchronize(l->stream));
+            l->ws.reset();
+            l->busy.store(false);
+        }
+    });
+}
+
+int ofx_refine_raw(ofx_engine *e, const float *i1, int w, int h, const float *i2, int w2, int h2, int64_t shift_x,
+                   int64_t shift_y, const float *ux, const float *uy, const ofx_config *cfg, double *out_ux,
+                   double *out_uy, double *out_energy, double *out_resid) {
+    return guarded([&] {
+        validate(cfg);
+        if (!i1 || !i2 || !ux || !uy) throw Error(OFX_EINVAL, "input pointer is NULL");
+        if (w2 != w || h2 != h || shift_x != 0 || shift_y != 0)
+            throw Error(OFX_EUNSUPPORTED, "this engine refines whole frames: image 2 must match image 1 with a zero shift");
+        check_window_impl(w, h, *cfg);
+        LaneGuard g(e);
+        Workspace &ws = *g.lane->ws;
+        hipStream_t s = g.lane->stream;
+        const size_t n = (size_t)w * h;
+        float *d1 = B<float>(ws.bi1, n), *d2 = B<float>(ws.bi2, n);
+        float *dx = B<float>(ws.binx, n), *dy = B<float>(ws.biny, n);
+        HIPCHECK(hipMemcpyAsync(d1, i1, n * sizeof(float), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(d2, i2, n * sizeof(float), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(dx, ux, n * sizeof(float), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(dy, uy, n * sizeof(float), hipMemcpyHostToDevice, s));
+        to_f64(dx, B<double>(ws.bux, n), n, s);
+        to_f64(dy, B<double>(ws.buy, n), n, s);
+        refine_device(ws, d1, d2, w, h, *cfg, s);
+        if (out_ux) HIPCHECK(hipMemcpyAsync(out_ux, ws.bux.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (out_uy) HIPCHECK(hipMemcpyAsync(out_uy, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (out_energy) HIPCHECK(hipMemcpyAsync(out_energy, ws.benergy.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (out_resid) HIPCHECK(hipMemcpyAsync(out_resid, ws.bresid.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+    });
+}
+
+int ofx_track_pair_device(ofx_engine *e, const float *d_i1, const float *d_i2, int w, int h, const uint8_t *d_mask,
+                          const float *d_init_ux, const float *d_init_uy, const ofx_config *cfg, ofx_result *d_out) {
+    return guarded([&] {
+        validate(cfg);
+        if (!d_i1 || !d_i2 || !d_out) throw Error(OFX_EINVAL, "input or output pointer is NULL");
+        if ((d_init_ux == nullptr) != (d_init_uy == nullptr)) throw Error(OFX_EINVAL, "init needs both components");
+        check_window_impl(w, h, *cfg);
+        if (std::min(w, h) < cfg->poc_win)
+            throw Error(OFX_EINVAL, "image smaller than poc_win (" + std::to_string(cfg->poc_win) + ")");
+        LaneGuard g(e);
+        Workspace &ws = *g.lane->ws;
+        hipStream_t s = g.lane->stream;
+        const size_t n = (size_t)w * h;
+        double *ix = nullptr, *iy = nullptr;
+        if (d_init_ux) {
+            ix = B<double>(ws.bstart_x, n);
+            iy = B<double>(ws.bstart_y, n);
+            to_f64(d_init_ux, ix, n, s);
+            to_f64(d_init_uy, iy, n, s);
+        }
+        pair_device(ws, d_i1, d_i2, w, h, d_mask, ix, iy, *cfg, d_out, s);
+        record_times(ws, d_out);
+    });
+}
+
+int ofx_track_pair(ofx_engine *e, const float *i1, const float *i2, int w, int h, const uint8_t *mask,
+                   const float *init_ux, const float *init_uy, const ofx_config *cfg, ofx_result *out) {
+    return guarded([&] {
+        validate(cfg);
+        if (!i1 || !i2 || !out) throw Error(OFX_EINVAL, "input or output pointer is NULL");
+        if ((init_ux == nullptr) != (init_uy == nullptr)) throw Error(OFX_EINVAL, "init needs both components");
+        check_window_impl(w, h, *cfg);
+        if (std::min(w, h) < cfg->poc_win)
+            throw Error(OFX_EINVAL, "image smaller than poc_win (" + std::to_string(cfg->poc_win) + ")");
+        const size_t n = (size_t)w * h;
+        if (mask) {
+            bool any = false;
+            for (size_t i = 0; i < n && !any; i++) any = mask[i] != 0;
+            if (!any) throw Error(OFX_EINVAL, "ROI mask is empty");
+        }
+        LaneGuard g(e);
+        Workspace &ws = *g.lane->ws;
+        hipStream_t s = g.lane->stream;
+        float *d1 = B<float>(ws.bi1, n), *d2 = B<float>(ws.bi2, n);
+        HIPCHECK(hipMemcpyAsync(d1, i1, n * sizeof(float), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(d2, i2, n * sizeof(float), hipMemcpyHostToDevice, s));
+        uint8_t *dm = nullptr;
+        if (mask) {
+            dm = B<uint8_t>(ws.bmask, n);
+            HIPCHECK(hipMemcpyAsync(dm, mask, n, hipMemcpyHostToDevice, s));
+        }
+        double *ix = nullptr, *iy = nullptr;
+        if (init_ux) {
+            float *fx = B<float>(ws.binx, n), *fy = B<float>(ws.biny, n);
+            HIPCHECK(hipMemcpyAsync(fx, init_ux, n * sizeof(float), hipMemcpyHostToDevice, s));
+            HIPCHECK(hipMemcpyAsync(fy, init_uy, n * sizeof(float), hipMemcpyHostToDevice, s));
+            ix = B<double>(ws.bstart_x, n);
+            iy = B<double>(ws.bstart_y, n);
+            to_f64(fx, ix, n, s);
+            to_f64(fy, iy, n, s);
+        }
+        Staged st = stage_outputs(ws, n, out);
+        pair_device(ws, d1, d2, w, h, dm, ix, iy, *cfg, &st.d, s);
+        record_times(ws, &st.d);
+        download(st, out, n, s);
+        out->t_coarse = st.d.t_coarse;
+        out->t_fine = st.d.t_fine;
+        out->t_strain = st.d.t_strain;
+        out->reference = 0;
+        out->mean_conf = 0;
+    });
+}
+
+// track.py:260-315 (seq_mode chain) with the legs, composition and link confidence on the GPU.
+int ofx_track_sequence(ofx_engine *e, const float *frames, int nf, int w, int h, const uint8_t *mask,
+                       const ofx_config *cfg, ofx_result *results) {
+    return guarded([&] {
+        validate(cfg);
+        if (!frames || !results) throw Error(OFX_EINVAL, "frames or results is NULL");
+        if (nf < 1) throw Error(OFX_EINVAL, "no frames");
+        if (cfg->seq_hybrid) throw Error(OFX_EUNSUPPORTED, "seq_mode hybrid is not implemented on the GPU yet");
+        check_window_impl(w, h, *cfg);
+        if (std::min(w, h) < cfg->poc_win)
+            throw Error(OFX_EINVAL, "image smaller than poc_win (" + std::to_string(cfg->poc_win) + ")");
+        const size_t n = (size_t)w * h;
+        LaneGuard g(e);
+        Workspace &ws = *g.lane->ws;
+        hipStream_t s = g.lane->stream;
+        const ofx_config &c = *cfg;
+        uint8_t *dm = nullptr;
+        if (mask) {
+            dm = B<uint8_t>(ws.bmask, n);
+            HIPCHECK(hipMemcpyAsync(dm, mask, n, hipMemcpyHostToDevice, s));
+        }
+        // frame buffers: reference, previous, current
+        float *fref = B<float>(ws.bframe_ref, n), *fprev = B<float>(ws.bframe_prev, n), *fcur = B<float>(ws.bi2, n);
+        double *u0x = B<double>(ws.bu0x, n), *u0y = B<double>(ws.bu0y, n), *c0r = B<double>(ws.bc0r, n);
+        double *cx = B<double>(ws.bcx, n), *cy = B<double>(ws.bcy, n), *conf0 = B<double>(ws.bconf0, n);
+        double *ca = B<double>(ws.bca, n);
+        double *px = B<double>(ws.bprev_x, n), *py = B<double>(ws.bprev_y, n), *pc = B<double>(ws.bprev_c, n);
+        double *lc = B<double>(ws.bleg_c, n);
+        double *macc = B<double>(ws.bmean, 2 + 2 * (size_t)h + n);
+        double *lx = B<double>(ws.bstart_x, n), *ly = B<double>(ws.bstart_y, n);  // leg raw field
+        HIPCHECK(hipMemcpyAsync(fref, frames, n * sizeof(float), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(fprev, frames, n * sizeof(float), hipMemcpyHostToDevice, s));
+        std::vector<double> zeros(n, 0.0), ones(n, 1.0);
+        HIPCHECK(hipMemcpyAsync(u0x, zeros.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(u0y, zeros.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(c0r, ones.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        const double Bw = edge_guard(c) > 0 ? 4.0 * c.sigma_w : 0.0;
+        int ref_idx = 0;
+        bool have_prev = false;
+        ofx_result leg{};  // legs write nothing but the workspace's fields
+        auto run_leg = [&](const float *a, const float *b, bool with_init) {
+            for (auto &ev : ws.ev)
+                if (!ev) HIPCHECK(hipEventCreate(&ev));
+            HIPCHECK(hipEventRecord(ws.ev[0], s));
+            ws.prof.mark(ST_COARSE, s);
+            double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
+            coarse_dense(a, b, w, h, c, with_init ? px : nullptr, with_init ? py : nullptr, ux, uy, ws.coarse, s);
+            HIPCHECK(hipEventRecord(ws.ev[1], s));
+            refine_device(ws, a, b, w, h, c, s);
+            HIPCHECK(hipEventRecord(ws.ev[2], s));
+            HIPCHECK(hipMemcpyAsync(lx, ws.bux.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipMemcpyAsync(ly, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipMemcpyAsync(lc, ws.bconf64.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        };
+        auto leg_mean = [&]() {
+            mean_conf(lc, lx, ly, u0x, u0y, dm, Bw, w, h, macc, s);
+            double acc[2];
+            HIPCHECK(hipMemcpyAsync(acc, macc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            return acc[1] > 0 ? acc[0] / acc[1] : 0.0;
+        };
+        for (int k = 1; k < nf; k++) {
+            HIPCHECK(hipMemcpyAsync(fcur, frames + (size_t)k * n, n * sizeof(float), hipMemcpyHostToDevice, s));
+            run_leg(fref, fcur, have_prev);
+            double cm = leg_mean();
+            if (cm < c.ref_update_conf && ref_idx != k - 1) {
+                // prev is the leg old reference -> k - 1: its confidence extends frame 0's link
+                conf_at(pc, u0x, u0y, w, h, ca, s);
+                hipLaunchKernelGGL(k_min_inplace, dim3((n + 255) / 256), dim3(256), 0, s, c0r, (const double *)ca, n);
+                compose(u0x, u0y, px, py, w, h, cx, cy, s);
+                HIPCHECK(hipMemcpyAsync(u0x, cx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+                HIPCHECK(hipMemcpyAsync(u0y, cy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+                HIPCHECK(hipMemcpyAsync(fref, fprev, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+                ref_idx = k - 1;
+                run_leg(fref, fcur, false);
+                cm = leg_mean();
+            }
+            compose(u0x, u0y, lx, ly, w, h, cx, cy, s);
+            conf_at(lc, u0x, u0y, w, h, ca, s);
+            hipLaunchKernelGGL(k_min_pair, dim3((n + 255) / 256), dim3(256), 0, s, (const double *)c0r,
+                               (const double *)ca, conf0, (float *)ws.bconf.p, n);
+            LAUNCHCHECK();
+            ofx_result *out = &results[k - 1];
+            Staged st = stage_outputs(ws, n, out);
+            finish_device(ws, cx, cy, (const float *)ws.bconf.p, w, h, dm, c, &st.d, s);
+            to_f32(cx, st.d.raw_ux, n, s);
+            to_f32(cy, st.d.raw_uy, n, s);
+            if (st.d.conf) HIPCHECK(hipMemcpyAsync(st.d.conf, ws.bconf.p, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+            ws.prof.mark(ST_END, s);
+            HIPCHECK(hipEventRecord(ws.ev[3], s));
+            download(st, out, n, s);
+            ws.prof.flush();
+            out->t_coarse = elapsed(ws.ev[0], ws.ev[1]) * 1e-3;
+            out->t_fine = elapsed(ws.ev[1], ws.ev[2]) * 1e-3;
+            out->t_strain = elapsed(ws.ev[2], ws.ev[3]) * 1e-3;
+            out->reference = ref_idx;
+            out->mean_conf = cm;
+            // this leg becomes prev
+            HIPCHECK(hipMemcpyAsync(px, lx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipMemcpyAsync(py, ly, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipMemcpyAsync(pc, lc, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipMemcpyAsync(fprev, fcur, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+            have_prev = true;
+        }
+        HIPCHECK(hipStreamSynchronize(s));
+        (void)leg;
+    });
+}
+
+}  // extern "C"

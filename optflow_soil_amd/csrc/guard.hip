@@ -1,0 +1,446 @@
+// Distance transforms, connected components, the trusted pre-warp fill, validity and strain.
+//
+// Distances follow scipy.ndimage.distance_transform_edt as the reference uses it: exact Euclidean
+// distance to the nearest feature pixel, and for the pre-warp the nearest feature itself with
+// scipy's tie rule (smallest column, then smallest row; rust/crates/optflow-core/src/edt.rs:175-183).
+// The column pass is exact and unbounded (two sweeps per column). The row pass has two forms: a
+// thresholded test d <= thr searches only |dx| <= floor(thr) (every column farther away is more
+// than thr away, so the decision is exact), and the nearest-feature search widens |dx| until
+// dx^2 exceeds the best squared distance found.
+#include <cmath>
+
+#include "fine.hpp"
+
+namespace ofx {
+
+namespace {
+
+dim3 grid2(int w, int h) { return dim3((w + 255) / 256, h); }
+
+__global__ void k_column_nearest(const uint8_t *__restrict__ feat, int w, int h, int *__restrict__ col_d2,
+                                 int *__restrict__ col_row) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= w) return;
+    int last = -1;
+    for (int y = 0; y < h; y++) {
+        const size_t i = (size_t)y * w + x;
+        if (feat[i]) last = y;
+        col_row[i] = last;
+    }
+    int next = -1;
+    for (int y = h - 1; y >= 0; y--) {
+        const size_t i = (size_t)y * w + x;
+        if (feat[i]) next = y;
+        const int up = col_row[i];
+        const int du = up >= 0 ? y - up : 0x7fffffff, dd = next >= 0 ? next - y : 0x7fffffff;
+        if (up < 0 && next < 0) {
+            col_d2[i] = -1;
+            col_row[i] = -1;
+        } else if (du <= dd) {  // a tie keeps the smaller row
+            col_d2[i] = du * du;
+            col_row[i] = up;
+        } else {
+            col_d2[i] = dd * dd;
+            col_row[i] = next;
+        }
+    }
+}
+
+__global__ void k_within(const int *__restrict__ col_d2, int w, int h, double thr, int T, uint8_t *__restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const int *row = col_d2 + (size_t)y * w;
+    long long best = -1;
+    for (int dx = -T; dx <= T; dx++) {
+        const int xx = x + dx;
+        if (xx < 0 || xx >= w) continue;
+        const int g = row[xx];
+        if (g < 0) continue;
+        const long long d2 = (long long)dx * dx + g;
+        if (best < 0 || d2 < best) best = d2;
+    }
+    out[(size_t)y * w + x] = best >= 0 && sqrt((double)best) <= thr;
+}
+
+__global__ void k_zero_counters(Counters *cnt) {
+    cnt->n_low = 0;
+    cnt->n_ok = 0;
+}
+
+__global__ void k_trusted(const uint8_t *__restrict__ low, const uint8_t *__restrict__ within_r, size_t n,
+                          uint8_t *__restrict__ ok, Counters *cnt) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned int nl = 0, no = 0;
+    if (i < n) {
+        const uint8_t o = !within_r[i];  // farther than R from every low pixel
+        ok[i] = o;
+        nl = low[i] != 0;
+        no = o;
+    }
+    // wavefront sums, one atomic per wavefront
+    for (int off = 32; off > 0; off >>= 1) {
+        nl += __shfl_down(nl, off);
+        no += __shfl_down(no, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (nl) atomicAdd(&cnt->n_low, nl);
+        if (no) atomicAdd(&cnt->n_ok, no);
+    }
+}
+
+// phaseflow.py:344-355: every untrusted pixel X takes u(N) + grad u(N) (X - N) from its nearest
+// trusted pixel N, unless no pixel is low, none is trusted or all are.
+__global__ void k_prewarp_fill(const double *__restrict__ ux, const double *__restrict__ uy,
+                               const double *__restrict__ slope, const uint8_t *__restrict__ ok,
+                               const int *__restrict__ col_d2, const int *__restrict__ col_row,
+                               const Counters *__restrict__ cnt, int w, int h, double *__restrict__ fx,
+                               double *__restrict__ fy) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const size_t i = (size_t)y * w + x;
+    const unsigned int n_ok = cnt->n_ok;
+    if (cnt->n_low == 0 || n_ok == 0 || n_ok == (unsigned int)((size_t)w * h) || ok[i]) {
+        fx[i] = ux[i];
+        fy[i] = uy[i];
+        return;
+    }
+    const int *row = col_d2 + (size_t)y * w;
+    long long best = -1;
+    int bx = -1;
+    const int maxdx = max(x, w - 1 - x);
+    for (int dx = 0; dx <= maxdx; dx++) {
+        const long long dd = (long long)dx * dx;
+        if (best >= 0 && dd > best) break;
+        // smaller column first, so that a tie keeps it
+        const int c0 = x - dx, c1 = x + dx;
+        if (c0 >= 0) {
+            const int g = row[c0];
+            if (g >= 0 && (best < 0 || dd + g < best)) { best = dd + g; bx = c0; }
+        }
+        if (dx > 0 && c1 < w) {
+            const int g = row[c1];
+            if (g >= 0 && (best < 0 || dd + g < best)) { best = dd + g; bx = c1; }
+        }
+    }
+    if (bx < 0) {  // unreachable while n_ok > 0
+        fx[i] = ux[i];
+        fy[i] = uy[i];
+        return;
+    }
+    const int by = col_row[(size_t)y * w + bx];
+    const size_t j = (size_t)by * w + bx;
+    const double ddx = (double)(x - bx), ddy = (double)(y - by);
+    const double *g = slope + 4 * j;
+    fx[i] = ux[j] + g[0] * ddx + g[2] * ddy;
+    fy[i] = uy[j] + g[1] * ddx + g[3] * ddy;
+}
+
+// ---------------------------------------------------------------- components (union-find)
+__device__ int find_root(const int *parent, int x) {
+    const volatile int *p = parent;
+    while (true) {
+        const int q = p[x];
+        if (q == x) return x;
+        x = q;
+    }
+}
+
+__global__ void k_cc_init(const uint8_t *__restrict__ b, size_t n, int *__restrict__ parent,
+                          unsigned int *__restrict__ sizes) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    parent[i] = b[i] ? (int)i : -1;
+    sizes[i] = 0;
+}
+
+__device__ void unite(int *parent, int a, int c) {
+    while (true) {
+        a = find_root(parent, a);
+        c = find_root(parent, c);
+        if (a == c) return;
+        if (a < c) {
+            const int t = a;
+            a = c;
+            c = t;
+        }
+        // link the larger root below the smaller one; retry if it stopped being a root
+        if (atomicCAS(&parent[a], a, c) == a) return;
+    }
+}
+
+__global__ void k_cc_union(const uint8_t *__restrict__ b, int w, int h, int *parent) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const int i = y * w + x;
+    if (!b[i]) return;
+    if (x + 1 < w && b[i + 1]) unite(parent, i, i + 1);
+    if (y + 1 < h && b[i + w]) unite(parent, i, i + w);
+}
+
+__global__ void k_cc_count(const uint8_t *__restrict__ b, size_t n, int *parent, unsigned int *sizes) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !b[i]) return;
+    const int r = find_root(parent, (int)i);
+    atomicAdd(&sizes[r], 1u);
+}
+
+__global__ void k_cc_keep(const uint8_t *__restrict__ b, size_t n, const int *parent,
+                          const unsigned int *__restrict__ sizes, double min_area, uint8_t *__restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (!b[i]) {
+        out[i] = 0;
+        return;
+    }
+    const int r = find_root(parent, (int)i);
+    out[i] = (double)sizes[r] >= min_area;
+}
+
+__global__ void k_guard_sources(const uint8_t *__restrict__ large, const uint8_t *__restrict__ mask, size_t n,
+                                uint8_t *__restrict__ src) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    src[i] = large[i] || (mask && !mask[i]);
+}
+
+__global__ void k_low(const float *__restrict__ conf, double conf_min, size_t n, uint8_t *__restrict__ low) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    low[i] = (double)conf[i] < conf_min;
+}
+
+// track.py:66-98: invalid = low conf, outside the ROI, within the guard of a source, or within the
+// guard of the image border
+__global__ void k_validity(const float *__restrict__ conf, double conf_min, const uint8_t *__restrict__ mask,
+                           const uint8_t *__restrict__ near_src, double guard, int guard_on,
+                           const double *__restrict__ ux, const double *__restrict__ uy, int w, int h,
+                           double *__restrict__ mx, double *__restrict__ my) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const size_t i = (size_t)y * w + x;
+    bool bad = (double)conf[i] < conf_min || (mask && !mask[i]);
+    if (guard_on) {
+        bad = bad || near_src[i];
+        const int bd = min(min(x, y), min(w - 1 - x, h - 1 - y));
+        bad = bad || (double)bd < guard;
+    }
+    mx[i] = bad ? NAN : ux[i];
+    my[i] = bad ? NAN : uy[i];
+}
+
+// ---------------------------------------------------------------- strain (strain.py:61-112)
+// y pass with zero padding: 11 planes, m t^0..4 | u t^0..2 | v t^0..2 (m = valid, u, v zero-filled)
+__global__ void k_strain_y(const double *__restrict__ mx, const double *__restrict__ my, int w, int h, int r,
+                           double *__restrict__ out) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const size_t n = (size_t)w * h;
+    double a[11];
+#pragma unroll
+    for (int q = 0; q < 11; q++) a[q] = 0;
+    for (int j = -r; j <= r; j++) {
+        const int yy = y + j;
+        if (yy < 0 || yy >= h) continue;
+        const size_t i = (size_t)yy * w + x;
+        const double u = mx[i], v = my[i];
+        const bool ok = isfinite(u) && isfinite(v);
+        const double m = ok ? 1.0 : 0.0, u0 = ok ? u : 0.0, v0 = ok ? v : 0.0;
+        const double t = j, t2 = t * t;
+        const double tp[5] = {1.0, t, t2, t2 * t, t2 * t2};
+#pragma unroll
+        for (int q = 0; q < 5; q++) a[q] += tp[q] * m;
+#pragma unroll
+        for (int q = 0; q < 3; q++) {
+            a[5 + q] += tp[q] * u0;
+            a[8 + q] += tp[q] * v0;
+        }
+    }
+    const size_t i = (size_t)y * w + x;
+#pragma unroll
+    for (int q = 0; q < 11; q++) out[q * n + i] = a[q];
+}
+
+// strain.rs:64-108: Cholesky with a relative pivot test, two right-hand sides
+__device__ bool solve6(double m[6][6], double rhs[6][2], double c[6][2]) {
+    double l[6][6];
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+        for (int j = 0; j < 6; j++) l[i][j] = 0;
+#pragma unroll
+    for (int i = 0; i < 6; i++)
+#pragma unroll
+        for (int j = 0; j <= i; j++) {
+            double s = m[i][j];
+#pragma unroll
+            for (int k = 0; k < j; k++) s -= l[i][k] * l[j][k];
+            if (i == j) {
+                if (isnan(s) || s <= 1e-12 * m[i][i]) return false;
+                l[i][i] = sqrt(s);
+            } else {
+                l[i][j] = s / l[j][j];
+            }
+        }
+#pragma unroll
+    for (int col = 0; col < 2; col++) {
+        double yv[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            double s = rhs[i][col];
+#pragma unroll
+            for (int k = 0; k < i; k++) s -= l[i][k] * yv[k];
+            yv[i] = s / l[i][i];
+        }
+#pragma unroll
+        for (int i = 5; i >= 0; i--) {
+            double s = yv[i];
+#pragma unroll
+            for (int k = i + 1; k < 6; k++) s -= l[k][i] * c[k][col];
+            c[i][col] = s / l[i][i];
+        }
+    }
+    return true;
+}
+
+__global__ void __launch_bounds__(256) k_strain_x(const double *__restrict__ ym, const double *__restrict__ mx,
+                                                  const double *__restrict__ my, int w, int h, int r, int win,
+                                                  float *ux_out, float *uy_out, uint8_t *valid, float *exx, float *eyy,
+                                                  float *exy, float *gmax, float *vol) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const size_t n = (size_t)w * h;
+    double mm[5][5], mu[3][3], mv[3][3];
+#pragma unroll
+    for (int a = 0; a < 5; a++)
+#pragma unroll
+        for (int b = 0; b < 5; b++) mm[a][b] = 0;
+#pragma unroll
+    for (int a = 0; a < 3; a++)
+#pragma unroll
+        for (int b = 0; b < 3; b++) mu[a][b] = mv[a][b] = 0;
+    const double *row = ym + (size_t)y * w;
+    for (int j = -r; j <= r; j++) {
+        const int xx = x + j;
+        if (xx < 0 || xx >= w) continue;
+        const double t = j, t2 = t * t;
+        const double tp[5] = {1.0, t, t2, t2 * t, t2 * t2};
+#pragma unroll
+        for (int b = 0; b < 5; b++) {
+            const double v = row[b * n + xx];
+#pragma unroll
+            for (int a = 0; a + b < 5; a++) mm[a][b] += tp[a] * v;
+        }
+#pragma unroll
+        for (int b = 0; b < 3; b++) {
+            const double vu = row[(5 + b) * n + xx], vv = row[(8 + b) * n + xx];
+#pragma unroll
+            for (int a = 0; a + b < 3; a++) {
+                mu[a][b] += tp[a] * vu;
+                mv[a][b] += tp[a] * vv;
+            }
+        }
+    }
+    const int BA[6] = {0, 1, 0, 2, 1, 0}, BB[6] = {0, 0, 1, 0, 1, 2};
+    double M[6][6], R[6][2], C[6][2];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+#pragma unroll
+        for (int j = 0; j < 6; j++) M[i][j] = mm[BA[i] + BA[j]][BB[i] + BB[j]];
+        R[i][0] = mu[BA[i]][BB[i]];
+        R[i][1] = mv[BA[i]][BB[i]];
+    }
+    const double count = mm[0][0];
+    // strain.rs:37-46: count >= win^2 / 2 decided on integers
+    bool ok = 2.0 * rint(count) >= (double)win * (double)win;
+    if (ok) ok = solve6(M, R, C);
+    double a = NAN, b = NAN, c = NAN, d = NAN;
+    if (ok) {
+        a = C[1][0];
+        b = C[2][0];
+        c = C[1][1];
+        d = C[2][1];
+    }
+    const size_t i = (size_t)y * w + x;
+    const double Exx = a + 0.5 * (a * a + c * c);
+    const double Eyy = d + 0.5 * (b * b + d * d);
+    const double Exy = 0.5 * (b + c) + 0.5 * (a * b + c * d);
+    const double hd = (Exx - Eyy) / 2;
+    if (exx) exx[i] = (float)Exx;
+    if (eyy) eyy[i] = (float)Eyy;
+    if (exy) exy[i] = (float)Exy;
+    if (gmax) gmax[i] = (float)(2.0 * sqrt(hd * hd + Exy * Exy));
+    if (vol) vol[i] = (float)((1 + a) * (1 + d) - b * c - 1.0);
+    const double u = mx[i], v = my[i];
+    if (ux_out) ux_out[i] = (float)u;
+    if (uy_out) uy_out[i] = (float)v;
+    if (valid) valid[i] = isfinite(u) && isfinite(v);
+}
+
+size_t nblk(size_t n) { return (n + 255) / 256; }
+
+}  // namespace
+
+void column_nearest(const uint8_t *feature, int w, int h, int *col_d2, int *col_row, hipStream_t s) {
+    hipLaunchKernelGGL(k_column_nearest, dim3((w + 63) / 64), dim3(64), 0, s, feature, w, h, col_d2, col_row);
+    LAUNCHCHECK();
+}
+
+void within_distance(const int *col_d2, int w, int h, double thr, uint8_t *within, hipStream_t s) {
+    const int T = thr >= 0 ? (int)std::floor(thr) : -1;
+    hipLaunchKernelGGL(k_within, grid2(w, h), dim3(256), 0, s, col_d2, w, h, thr, T, within);
+    LAUNCHCHECK();
+}
+
+void prewarp_trusted(const uint8_t *low, const uint8_t *within_r, int w, int h, uint8_t *ok, Counters *cnt,
+                     hipStream_t s) {
+    const size_t n = (size_t)w * h;
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, s, cnt);
+    hipLaunchKernelGGL(k_trusted, dim3(nblk(n)), dim3(256), 0, s, low, within_r, n, ok, cnt);
+    LAUNCHCHECK();
+}
+
+void prewarp_fill(const double *ux, const double *uy, const double *slope, const uint8_t *ok, const int *col_d2,
+                  const int *col_row, const Counters *cnt, int w, int h, double *fx, double *fy, hipStream_t s) {
+    hipLaunchKernelGGL(k_prewarp_fill, grid2(w, h), dim3(256), 0, s, ux, uy, slope, ok, col_d2, col_row, cnt, w, h, fx,
+                       fy);
+    LAUNCHCHECK();
+}
+
+void large_components(const uint8_t *b, int w, int h, double min_area, int *parent, unsigned int *sizes,
+                      uint8_t *out, hipStream_t s) {
+    const size_t n = (size_t)w * h;
+    hipLaunchKernelGGL(k_cc_init, dim3(nblk(n)), dim3(256), 0, s, b, n, parent, sizes);
+    hipLaunchKernelGGL(k_cc_union, grid2(w, h), dim3(256), 0, s, b, w, h, parent);
+    hipLaunchKernelGGL(k_cc_count, dim3(nblk(n)), dim3(256), 0, s, b, n, parent, sizes);
+    hipLaunchKernelGGL(k_cc_keep, dim3(nblk(n)), dim3(256), 0, s, b, n, parent, sizes, min_area, out);
+    LAUNCHCHECK();
+}
+
+void guard_sources(const uint8_t *large, const uint8_t *mask, int w, int h, uint8_t *src, hipStream_t s) {
+    const size_t n = (size_t)w * h;
+    hipLaunchKernelGGL(k_guard_sources, dim3(nblk(n)), dim3(256), 0, s, large, mask, n, src);
+    LAUNCHCHECK();
+}
+
+void low_mask(const float *conf, double conf_min, size_t n, uint8_t *low, hipStream_t s) {
+    hipLaunchKernelGGL(k_low, dim3(nblk(n)), dim3(256), 0, s, conf, conf_min, n, low);
+    LAUNCHCHECK();
+}
+
+void validity(const float *conf, double conf_min, const uint8_t *mask, const uint8_t *near_src, double guard,
+              int guard_on, const double *ux, const double *uy, int w, int h, double *mx, double *my, hipStream_t s) {
+    hipLaunchKernelGGL(k_validity, grid2(w, h), dim3(256), 0, s, conf, conf_min, mask, near_src, guard, guard_on, ux,
+                       uy, w, h, mx, my);
+    LAUNCHCHECK();
+}
+
+void strain(const double *mx, const double *my, int w, int h, int win, double *ytmp, float *ux_out, float *uy_out,
+            uint8_t *valid, float *exx, float *eyy, float *exy, float *gmax, float *vol, hipStream_t s) {
+    const int r = win / 2;
+    hipLaunchKernelGGL(k_strain_y, grid2(w, h), dim3(256), 0, s, mx, my, w, h, r, ytmp);
+    hipLaunchKernelGGL(k_strain_x, grid2(w, h), dim3(256), 0, s, ytmp, mx, my, w, h, r, win, ux_out, uy_out, valid,
+                       exx, eyy, exy, gmax, vol);
+    LAUNCHCHECK();
+}
+
+}  // namespace ofx

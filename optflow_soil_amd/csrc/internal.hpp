@@ -1,0 +1,66 @@
+// Internal declarations shared by the HIP translation units of libofx.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/ofx.h"
+
+namespace ofx {
+
+constexpr int PAD = 32;               // reflect pad of the filter bank (phaseflow.py:47)
+constexpr double APERTURE_TOL = 1e-3; // phaseflow.py:50
+constexpr double RIDGE = 1e-12;       // phaseflow.py:153
+constexpr int MAX_FILTERS = 64;
+constexpr int MAX_FACTORS = 24;
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIPCHECK(x)                                                                          \
+    do {                                                                                     \
+        hipError_t e_ = (x);                                                                 \
+        if (e_ != hipSuccess)                                                                \
+            throw ::ofx::Error(OFX_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCHCHECK() HIPCHECK(hipGetLastError())
+
+// ---------------------------------------------------------------- FFT plans and passes
+struct FftPlan {
+    int n = 0;
+    int nfac = 0;
+    int fac[MAX_FACTORS] = {0};
+    float2 *tw = nullptr;  // device: exp(-2 pi i e / n), e in [0, n)
+};
+
+const FftPlan &fft_plan(int n);  // cached per length, device tables never freed
+void fft_release();
+
+// Filter bank parameters for the on-the-fly evaluation of G_k (phaseflow.py:73-89).
+struct Bank {
+    int F = 0;
+    float w0x[MAX_FILTERS], w0y[MAX_FILTERS], inv2s2[MAX_FILTERS];
+    float s_hp2_half;  // s_hp^2 / 2
+};
+Bank make_bank(const ofx_config &c);
+
+// What the first (column) pass of an inverse multiplies the spectrum by.
+enum SpecMul { MUL_G = 0, MUL_G_IFX = 1, MUL_G_IFY = 2 };
+
+// Forward 2-D FFT of the reflect-padded real image img (w x h, pitch w) into spec (hp x wp).
+void fft_forward_padded(const float *img, int w, int h, float2 *spec, float2 *tmp, hipStream_t s);
+// Inverse 2-D FFT of spec * G_k (* i fx or i fy) into out (hp x wp). With prefilter, the cubic
+// B-spline prefilter is applied along both axes (the result holds spline coefficients).
+void fft_inverse_filtered(const float2 *spec, int hp, int wp, const Bank &bank, int k, SpecMul mul,
+                          bool prefilter, float2 *tmp, float2 *out, hipStream_t s);
+
+// ---------------------------------------------------------------- fine stage and friends
+struct Workspace;  // per-size device buffers (engine.hip)
+
+}  // namespace ofx

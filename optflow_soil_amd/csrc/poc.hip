@@ -1,0 +1,649 @@
+// Coarse stage on the GPU: windowed, band-limited phase-only correlation on a grid over a Gaussian
+// pyramid, outlier cleaning and fill, and the dense start field (python/optflow_ref/poc.py:18-239,
+// rust/crates/optflow-core/src/poc.rs:141-766, track.py:131-137). One workgroup per grid point
+// holds the two windows' spectra in LDS (float32) and iterates the re-centred match; the cleaning
+// passes run in one workgroup over the whole grid.
+#include <cmath>
+#include <cstring>
+
+#include "fine.hpp"
+#include "lines.hpp"
+
+namespace ofx {
+
+constexpr int POC_MAX = 4096;  // window elements a workgroup holds (64 x 64, 16 x 256, ...)
+
+struct CoarseWs {
+    int w = 0, h = 0, L = 0;
+    std::vector<float *> pa, pb;   // pyramid levels 1..L (level 0 = the inputs)
+    std::vector<int> lw, lh;
+    double *tmp = nullptr;         // blur scratch
+    double *gux = nullptr, *guy = nullptr, *gpk = nullptr, *ix = nullptr, *iy = nullptr;
+    int *gflag = nullptr;          // per point: valid, out flags
+    size_t gcap = 0;
+    size_t tmpcap = 0;
+};
+
+namespace {
+
+int pow2_floor(int n) {
+    if (n < 1) return 0;
+    int p = 1;
+    while (p * 2 <= n) p *= 2;
+    return p;
+}
+
+// poc.py:172-187
+bool level_window(int h, int w, int win, int &wy, int &wx) {
+    const int lim_h = pow2_floor(2 * h / 3), lim_w = pow2_floor(2 * w / 3);
+    if (win <= lim_h && win <= lim_w) {
+        wy = wx = win;
+        return true;
+    }
+    if (h <= w) {
+        wy = std::min(win, lim_h);
+        wx = std::min(lim_w, std::max(win, win * win / std::max(wy, 1)));
+    } else {
+        wx = std::min(win, lim_w);
+        wy = std::min(lim_h, std::max(win, win * win / std::max(wx, 1)));
+    }
+    return std::min(wy, wx) >= 16;
+}
+
+int n_levels(double max_disp, int win, int h, int w) {
+    const double v = std::ceil(std::log2(std::max(max_disp, 1e-9) / (win / 4.0)));
+    int L = (int)std::max(0.0, v);
+    int wy, wx;
+    while (L > 0 && !level_window(h >> L, w >> L, win, wy, wx)) L--;
+    return L;
+}
+
+// rust poc.rs:244-256: (start, count) of the grid positions along an axis of length n
+void grid_axis(int n, int win, int step, int &start, int &count) {
+    const int stop = n + 1 - (win - win / 2);
+    start = win / 2;
+    count = start >= stop ? 0 : (stop - start - 1) / step + 1;
+}
+
+// ------------------------------------------------------------------ pyramid (poc.py:164-165)
+__global__ void k_down_y(const float *__restrict__ in, double *__restrict__ out, int w, int h, int oh) {
+    // blur along y (sigma 1, 9 taps, replicated edges) at the even rows only
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, yo = blockIdx.y;
+    if (x >= w || yo >= oh) return;
+    const int y = 2 * yo;
+    double k[9], s = 0;
+    for (int j = -4; j <= 4; j++) {
+        k[j + 4] = exp(-0.5 * j * j);
+        s += k[j + 4];
+    }
+    double acc = 0;
+    for (int j = -4; j <= 4; j++) acc += (k[j + 4] / s) * (double)in[(size_t)min(max(y + j, 0), h - 1) * w + x];
+    out[(size_t)yo * w + x] = acc;
+}
+
+__global__ void k_down_x(const double *__restrict__ in, float *__restrict__ out, int w, int ow, int oh) {
+    const int xo = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (xo >= ow || y >= oh) return;
+    const int x = 2 * xo;
+    double k[9], s = 0;
+    for (int j = -4; j <= 4; j++) {
+        k[j + 4] = exp(-0.5 * j * j);
+        s += k[j + 4];
+    }
+    double acc = 0;
+    const double *row = in + (size_t)y * w;
+    for (int j = -4; j <= 4; j++) acc += (k[j + 4] / s) * row[min(max(x + j, 0), w - 1)];
+    out[(size_t)y * ow + xo] = (float)acc;
+}
+
+// ------------------------------------------------------------------ POC per grid point
+struct PocArgs {
+    int W, H;        // level image size
+    int wy, wx;      // window
+    int x0, y0, step, nx, ny;  // grid positions x0 + j step, y0 + i step
+    int nfx, nfy;    // FFT factorisations
+    int facx[MAX_FACTORS], facy[MAX_FACTORS];
+    int have_init;
+};
+
+// 2-D forward DFT of X (wy x wx, row-major) in LDS; result in column-major order ([kx][ky]) in
+// the returned buffer, Y and Z are scratch.
+__device__ float2 *fft2_lds(float2 *X, float2 *Y, int wy, int wx, const PocArgs &P, const float2 *twx,
+                            const float2 *twy) {
+    float2 *r = fft_lines_lds(X, Y, wx, wy, P.facx, P.nfx, twx, threadIdx.x, blockDim.x);
+    float2 *o = (r == X) ? Y : X;
+    for (int i = threadIdx.x; i < wy * wx; i += blockDim.x) {
+        const int y = i / wx, x = i - y * wx;
+        o[x * wy + y] = r[i];
+    }
+    __syncthreads();
+    return fft_lines_lds(o, r, wy, wx, P.facy, P.nfy, twy, threadIdx.x, blockDim.x);
+}
+
+__device__ double block_sum(double v, double *red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wid] = v;
+    __syncthreads();
+    double s = 0;
+    for (int i = 0; i < (int)(blockDim.x >> 6); i++) s += red[i];
+    __syncthreads();
+    return s;
+}
+
+__device__ float hannf(int i, int n) { return (float)(0.5 - 0.5 * cos(6.283185307179586 * (double)i / (double)n)); }
+
+// Loads the mean-removed, Hann-windowed window at (y0, x0) of img into X (complex).
+__device__ void load_window(const float *img, int W, int y0, int x0, int wy, int wx, float2 *X, double *red) {
+    double s = 0;
+    for (int i = threadIdx.x; i < wy * wx; i += blockDim.x) {
+        const int y = i / wx, x = i - y * wx;
+        s += (double)img[(size_t)(y0 + y) * W + x0 + x];
+    }
+    const double mean = block_sum(s, red) / (double)(wy * wx);
+    for (int i = threadIdx.x; i < wy * wx; i += blockDim.x) {
+        const int y = i / wx, x = i - y * wx;
+        const double v = ((double)img[(size_t)(y0 + y) * W + x0 + x] - mean) * (double)hannf(y, wy) * (double)hannf(x, wx);
+        X[i] = make_float2((float)v, 0.f);
+    }
+    __syncthreads();
+}
+
+__device__ bool lp_pass(int i, int n) {
+    const int k = (i <= (n - 1) / 2) ? i : i - n;
+    return fabs((double)k / n) <= 0.25;
+}
+
+__device__ double parabola(double m, double c, double p) {
+    const double den = m - 2.0 * c + p;
+    return den == 0 ? 0.0 : 0.5 * (m - p) / den;
+}
+
+// poc.py:67-92 with poc_shift (poc.py:39-54) inside
+__global__ void __launch_bounds__(256) k_poc_grid(const float *__restrict__ I1, const float *__restrict__ I2, PocArgs P,
+                                                  const float2 *__restrict__ twx, const float2 *__restrict__ twy,
+                                                  const double *__restrict__ ix, const double *__restrict__ iy,
+                                                  double *__restrict__ gux, double *__restrict__ guy,
+                                                  double *__restrict__ gpk) {
+    extern __shared__ float2 lds[];
+    const int n = P.wy * P.wx;
+    float2 *X = lds, *Y = lds + n;
+    double *red = (double *)Y;  // block sums run while Y is free
+    float2 A[POC_MAX / 256];  // spectrum of the first window, element tid + 256 q
+    const int pt = blockIdx.x;
+    const int gi = pt / P.nx, gj = pt - gi * P.nx;
+    const int y = P.y0 + gi * P.step, x = P.x0 + gj * P.step;
+    const int hy = P.wy / 2, hx = P.wx / 2;
+    double dx = P.have_init ? ix[pt] : 0.0, dy = P.have_init ? iy[pt] : 0.0;
+    // spectrum of the first window, kept in registers for every iteration
+    load_window(I1, P.W, y - hy, x - hx, P.wy, P.wx, X, red);
+    {
+        float2 *r = fft2_lds(X, Y, P.wy, P.wx, P, twx, twy);
+#pragma unroll
+        for (int q = 0; q < POC_MAX / 256; q++) {
+            const int i = threadIdx.x + 256 * q;
+            A[q] = i < n ? r[i] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+    }
+    int nlp = 0;  // low-pass mask size (lp.mean() = nlp / n)
+    {
+        int cy = 0, cx = 0;
+        for (int i = 0; i < P.wy; i++) cy += lp_pass(i, P.wy);
+        for (int i = 0; i < P.wx; i++) cx += lp_pass(i, P.wx);
+        nlp = cy * cx;
+    }
+    const double lpm = (double)nlp / (double)n;
+    double pk = 0.0;
+    for (int t = 0; t < 3; t++) {
+        const int ox = (int)rint(dx), oy = (int)rint(dy);
+        const int wy0 = y - hy + oy, wx0 = x - hx + ox;
+        if (wy0 < 0 || wx0 < 0 || wy0 + P.wy > P.H || wx0 + P.wx > P.W) {
+            pk = 0.0;
+            break;
+        }
+        load_window(I2, P.W, wy0, wx0, P.wy, P.wx, X, red);
+        float2 *B = fft2_lds(X, Y, P.wy, P.wx, P, twx, twy);
+        // normalised cross power spectrum, low-passed, conjugated for the inverse (layout [kx][ky])
+#pragma unroll
+        for (int q = 0; q < POC_MAX / 256; q++) {
+            const int i = threadIdx.x + 256 * q;
+            if (i >= n) break;
+            const int kx = i / P.wy, ky = i - kx * P.wy;
+            const float2 b = B[i], a = A[q];
+            float2 c = make_float2(b.x * a.x + b.y * a.y, b.y * a.x - b.x * a.y);
+            const float m = sqrtf(c.x * c.x + c.y * c.y) + 1e-12f;
+            c = make_float2(c.x / m, c.y / m);
+            if (!(lp_pass(kx, P.wx) && lp_pass(ky, P.wy))) c = make_float2(0.f, 0.f);
+            B[i] = cconj(c);
+        }
+        __syncthreads();
+        // inverse: along ky (lines of length wy), transpose back to [ky][kx], along kx
+        float2 *o = (B == X) ? Y : X;
+        float2 *r1 = fft_lines_lds(B, o, P.wy, P.wx, P.facy, P.nfy, twy, threadIdx.x, blockDim.x);
+        float2 *o2 = (r1 == B) ? o : B;
+        for (int i = threadIdx.x; i < n; i += blockDim.x) {
+            const int kx = i / P.wy, yy = i - kx * P.wy;
+            o2[yy * P.wx + kx] = r1[i];
+        }
+        __syncthreads();
+        float2 *r2 = fft_lines_lds(o2, r1, P.wx, P.wy, P.facx, P.nfx, twx, threadIdx.x, blockDim.x);
+        // the buffer r2 does not occupy holds the argmax partials
+        float *bval = (float *)((r2 == X) ? Y : X);
+        int *bidx = (int *)(bval + 256);
+        // r = fftshift(real(ifft2)) / lpm; argmax in the shifted array (first maximum, NaN wins)
+        const double sc = 1.0 / ((double)n * lpm);
+        const int shy = (P.wy + 1) / 2, shx = (P.wx + 1) / 2;
+        float best = 0.f;
+        int bi = -1;
+        bool bnan = false;
+        for (int s = threadIdx.x; s < n; s += blockDim.x) {
+            const int sy = s / P.wx, sx = s - sy * P.wx;
+            const int uy = (sy + shy) % P.wy, ux = (sx + shx) % P.wx;
+            const float v = (float)((double)r2[uy * P.wx + ux].x * sc);
+            if (bnan) continue;
+            if (isnan(v)) {
+                bnan = true;
+                best = v;
+                bi = s;
+            } else if (bi < 0 || v > best) {
+                best = v;
+                bi = s;
+            }
+        }
+        bval[threadIdx.x] = bnan ? NAN : best;
+        bidx[threadIdx.x] = bi;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float bv = 0.f;
+            int bb = -1;
+            bool bn = false;
+            for (int k = 0; k < (int)blockDim.x; k++) {
+                const int ki = bidx[k];
+                if (ki < 0) continue;
+                const float kv = bval[k];
+                const bool kn = isnan(kv);
+                bool take;
+                if (bb < 0) take = true;
+                else if (bn) take = kn && ki < bb;
+                else if (kn) take = true;
+                else take = kv > bv || (kv == bv && ki < bb);
+                if (take) {
+                    bv = kv;
+                    bb = ki;
+                    bn = kn;
+                }
+            }
+            bidx[0] = bb;
+        }
+        __syncthreads();
+        const int bsel = bidx[0];
+        int py = bsel / P.wx, px = bsel - py * P.wx;
+        py = min(max(py, 1), P.wy - 2);
+        px = min(max(px, 1), P.wx - 2);
+        auto R = [&](int sy, int sx) {
+            const int uy = (sy + shy) % P.wy, ux = (sx + shx) % P.wx;
+            return (double)(float)((double)r2[uy * P.wx + ux].x * sc);
+        };
+        const double c0 = R(py, px);
+        const double sx = parabola(R(py, px - 1), c0, R(py, px + 1));
+        const double sy = parabola(R(py - 1, px), c0, R(py + 1, px));
+        pk = c0;
+        dx = ox + (px - P.wx / 2 + sx);
+        dy = oy + (py - P.wy / 2 + sy);
+        __syncthreads();
+        if (fabs(px - P.wx / 2 + sx) < 0.5 && fabs(py - P.wy / 2 + sy) < 0.5) break;
+    }
+    if (threadIdx.x == 0) {
+        gux[pt] = dx;
+        guy[pt] = dy;
+        gpk[pt] = pk;
+    }
+}
+
+// ------------------------------------------------------------------ clean (poc.py:95-140)
+__device__ double median_small(double *v, int n) {
+    for (int i = 1; i < n; i++) {
+        const double t = v[i];
+        int j = i - 1;
+        while (j >= 0 && v[j] > t) {
+            v[j + 1] = v[j];
+            j--;
+        }
+        v[j + 1] = t;
+    }
+    return (n & 1) ? v[n / 2] : 0.5 * (v[n / 2 - 1] + v[n / 2]);
+}
+
+// flags: bit 0 valid (peak >= peak_min), bit 1 out (current pass), bit 2 out (next pass)
+__global__ void __launch_bounds__(1024) k_clean(double *ux, double *uy, double *peak, const double *fx,
+                                                const double *fy, int have_fb, int *flags, int ny, int nx,
+                                                double peak_min, double thresh) {
+    __shared__ int any_out, all_out;
+    const int n = ny * nx;
+    const int OFF[8][2] = {{-1, -1}, {-1, 0}, {-1, 1}, {0, -1}, {0, 1}, {1, -1}, {1, 0}, {1, 1}};
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        const int v = peak[k] >= peak_min;
+        flags[k] = v | (v << 1);
+    }
+    __syncthreads();
+    // outlier test against the valid neighbours' medians (ux, uy unchanged during the test)
+    for (int k = threadIdx.x; k < n; k += blockDim.x) {
+        if (!(flags[k] & 1)) continue;
+        const int i = k / nx, j = k - i * nx;
+        int nb[8], c = 0;
+        for (int q = 0; q < 8; q++) {
+            const int a = i + OFF[q][0], b = j + OFF[q][1];
+            if (a >= 0 && a < ny && b >= 0 && b < nx && (flags[a * nx + b] & 1)) nb[c++] = a * nx + b;
+        }
+        if (c < 3) continue;
+        bool outlier = false;
+        for (int comp = 0; comp < 2; comp++) {
+            const double *u = comp ? uy : ux;
+            double vals[8], dev[8];
+            for (int q = 0; q < c; q++) vals[q] = u[nb[q]];
+            const double med = median_small(vals, c);
+            for (int q = 0; q < c; q++) dev[q] = fabs(u[nb[q]] - med);
+            const double res = median_small(dev, c);
+            if (fabs(u[k] - med) / (res + 0.1) > thresh) outlier = true;
+        }
+        if (outlier) atomicAnd(&flags[k], ~2);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) any_out = 0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x)
+        if (flags[k] & 2) any_out = 1;
+    __syncthreads();
+    if (!any_out) {
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            ux[k] = have_fb ? fx[k] : 0.0;
+            uy[k] = have_fb ? fy[k] : 0.0;
+            peak[k] = peak[k] * 0;
+        }
+        return;
+    }
+    for (int pass = 0; pass < 100; pass++) {
+        if (threadIdx.x == 0) all_out = 1;
+        __syncthreads();
+        for (int k = threadIdx.x; k < n; k += blockDim.x)
+            if (!(flags[k] & 2)) all_out = 0;
+        __syncthreads();
+        if (all_out) break;
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            int f = flags[k];
+            if (f & 2) {
+                f |= 4;
+            } else {
+                const int i = k / nx, j = k - i * nx;
+                double vx[8], vy[8];
+                int c = 0;
+                for (int q = 0; q < 8; q++) {
+                    const int a = i + OFF[q][0], b = j + OFF[q][1];
+                    if (a >= 0 && a < ny && b >= 0 && b < nx && (flags[a * nx + b] & 2)) {
+                        vx[c] = ux[a * nx + b];
+                        vy[c] = uy[a * nx + b];
+                        c++;
+                    }
+                }
+                if (c) {
+                    ux[k] = median_small(vx, c);
+                    uy[k] = median_small(vy, c);
+                    f |= 4;
+                }
+            }
+            flags[k] = f;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < n; k += blockDim.x) {
+            const int f = flags[k];
+            flags[k] = (f & 1) | ((f >> 2) & 1) << 1;
+        }
+        __syncthreads();
+    }
+    for (int k = threadIdx.x; k < n; k += blockDim.x)
+        if (!(flags[k] & 1)) peak[k] = 0.0;
+}
+
+// ------------------------------------------------------------------ grid interpolation (poc.py:143-161)
+struct GridGeom {
+    int x0, y0, step, nx, ny;
+};
+
+__device__ void grid_frac(double q, int q0, int step, int nn, double &t, int &i0) {
+    if (nn == 1) {
+        t = 0;
+        i0 = 0;
+        return;
+    }
+    double f = (q - q0) / (double)step;
+    f = fmin(fmax(f, 0.0), (double)(nn - 1));
+    i0 = min((int)floor(f), nn - 2);
+    t = f - i0;
+}
+
+__device__ void grid_bilinear(const GridGeom &g, const double *ax, const double *ay, double xq, double yq, double &ox,
+                              double &oy) {
+    double tx, ty;
+    int x0, y0;
+    grid_frac(xq, g.x0, g.step, g.nx, tx, x0);
+    grid_frac(yq, g.y0, g.step, g.ny, ty, y0);
+    const int x1 = min(x0 + 1, g.nx - 1), y1 = min(y0 + 1, g.ny - 1);
+    const double *A[2] = {ax, ay};
+    double o[2];
+    for (int c = 0; c < 2; c++) {
+        const double *a = A[c];
+        o[c] = (1 - ty) * ((1 - tx) * a[y0 * g.nx + x0] + tx * a[y0 * g.nx + x1]) +
+               ty * ((1 - tx) * a[y1 * g.nx + x0] + tx * a[y1 * g.nx + x1]);
+    }
+    ox = o[0];
+    oy = o[1];
+}
+
+// init of level lev from the cleaned grid of level lev + 1 (halve coordinates, double the field)
+__global__ void k_level_init(GridGeom prev, const double *pux, const double *puy, GridGeom cur, double *ix,
+                             double *iy) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cur.nx * cur.ny) return;
+    const int i = k / cur.nx, j = k - i * cur.nx;
+    double ox, oy;
+    grid_bilinear(prev, pux, puy, (cur.x0 + j * cur.step) / 2.0, (cur.y0 + i * cur.step) / 2.0, ox, oy);
+    ix[k] = 2.0 * ox;
+    iy[k] = 2.0 * oy;
+}
+
+// coarsest level from a dense guess: init[Y sc, X sc] / sc
+__global__ void k_dense_init(const double *gx, const double *gy, int W, GridGeom cur, int sc, double *ix, double *iy) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= cur.nx * cur.ny) return;
+    const int i = k / cur.nx, j = k - i * cur.nx;
+    const size_t p = (size_t)((cur.y0 + i * cur.step) * sc) * W + (size_t)(cur.x0 + j * cur.step) * sc;
+    ix[k] = (double)gx[p] / sc;
+    iy[k] = (double)gy[p] / sc;
+}
+
+// correction relative to the dense guess at the grid nodes (track.py:133-137)
+__global__ void k_grid_minus_init(const double *gux, const double *guy, const double *gx, const double *gy, int W,
+                                  GridGeom g, double *cx, double *cy) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= g.nx * g.ny) return;
+    const int i = k / g.nx, j = k - i * g.nx;
+    const size_t p = (size_t)(g.y0 + i * g.step) * W + (g.x0 + j * g.step);
+    cx[k] = gux[k] - (double)gx[p];
+    cy[k] = guy[k] - (double)gy[p];
+}
+
+__global__ void k_dense(GridGeom g, const double *gux, const double *guy, const double *init_x, const double *init_y,
+                        int W, int H, double *ux, double *uy) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W) return;
+    double ox, oy;
+    grid_bilinear(g, gux, guy, x, y, ox, oy);
+    const size_t i = (size_t)y * W + x;
+    if (init_x) {
+        ox += (double)init_x[i];
+        oy += (double)init_y[i];
+    }
+    ux[i] = ox;
+    uy[i] = oy;
+}
+
+template <class T>
+void ensure(T *&p, size_t n, size_t &cap) {
+    if (p && cap >= n) return;
+    if (p) (void)hipFree(p);
+    HIPCHECK(hipMalloc(&p, n * sizeof(T)));
+    cap = n;
+}
+
+}  // namespace
+
+void coarse_release(CoarseWs *&ws) {
+    if (!ws) return;
+    for (auto p : ws->pa) (void)hipFree(p);
+    for (auto p : ws->pb) (void)hipFree(p);
+    (void)hipFree(ws->tmp);
+    (void)hipFree(ws->gux);
+    (void)hipFree(ws->guy);
+    (void)hipFree(ws->gpk);
+    (void)hipFree(ws->ix);
+    (void)hipFree(ws->iy);
+    (void)hipFree(ws->gflag);
+    delete ws;
+    ws = nullptr;
+}
+
+void coarse_dense(const float *i1, const float *i2, int w, int h, const ofx_config &c, const double *init_ux,
+                  const double *init_uy, double *ux, double *uy, CoarseWs *&ws, hipStream_t s) {
+    const int L = n_levels(c.max_disp, c.poc_win, h, w);
+    if (ws && (ws->w != w || ws->h != h || ws->L != L)) coarse_release(ws);
+    if (!ws) {
+        ws = new CoarseWs;
+        ws->w = w;
+        ws->h = h;
+        ws->L = L;
+        int lw = w, lh = h;
+        for (int l = 0; l < L; l++) {
+            const int nw = (lw + 1) / 2, nh = (lh + 1) / 2;
+            float *a, *b;
+            HIPCHECK(hipMalloc(&a, (size_t)nw * nh * sizeof(float)));
+            HIPCHECK(hipMalloc(&b, (size_t)nw * nh * sizeof(float)));
+            ws->pa.push_back(a);
+            ws->pb.push_back(b);
+            ws->lw.push_back(nw);
+            ws->lh.push_back(nh);
+            lw = nw;
+            lh = nh;
+        }
+        size_t cap = 0;
+        ensure(ws->tmp, (size_t)w * ((h + 1) / 2), cap);
+    }
+    // pyramid
+    for (int l = 0; l < L; l++) {
+        const int pw = l == 0 ? w : ws->lw[l - 1], ph = l == 0 ? h : ws->lh[l - 1];
+        const float *sa = l == 0 ? i1 : ws->pa[l - 1], *sb = l == 0 ? i2 : ws->pb[l - 1];
+        const int nw = ws->lw[l], nh = ws->lh[l];
+        for (int img = 0; img < 2; img++) {
+            hipLaunchKernelGGL(k_down_y, dim3((pw + 255) / 256, nh), dim3(256), 0, s, img ? sb : sa, ws->tmp, pw, ph, nh);
+            hipLaunchKernelGGL(k_down_x, dim3((nw + 255) / 256, nh), dim3(256), 0, s, ws->tmp,
+                               img ? ws->pb[l] : ws->pa[l], pw, nw, nh);
+        }
+        LAUNCHCHECK();
+    }
+    // grid buffers sized for the largest level; the previous level's cleaned grid is kept in the
+    // upper halves of gux / guy
+    size_t npmax = 0;
+    for (int lev = L; lev >= 0; lev--) {
+        const int lw = lev == 0 ? w : ws->lw[lev - 1], lh = lev == 0 ? h : ws->lh[lev - 1];
+        int wy = c.poc_win, wx = c.poc_win;
+        if (lev > 0) level_window(lh, lw, c.poc_win, wy, wx);
+        int x0, nx, y0, ny;
+        grid_axis(lw, wx, c.grid_step, x0, nx);
+        grid_axis(lh, wy, c.grid_step, y0, ny);
+        npmax = std::max(npmax, (size_t)nx * ny);
+    }
+    if (!ws->gux || ws->gcap < npmax) {
+        (void)hipFree(ws->gux); (void)hipFree(ws->guy); (void)hipFree(ws->gpk);
+        (void)hipFree(ws->ix); (void)hipFree(ws->iy); (void)hipFree(ws->gflag);
+        HIPCHECK(hipMalloc(&ws->gux, npmax * sizeof(double) * 2));
+        HIPCHECK(hipMalloc(&ws->guy, npmax * sizeof(double) * 2));
+        HIPCHECK(hipMalloc(&ws->gpk, npmax * sizeof(double)));
+        HIPCHECK(hipMalloc(&ws->ix, npmax * sizeof(double)));
+        HIPCHECK(hipMalloc(&ws->iy, npmax * sizeof(double)));
+        HIPCHECK(hipMalloc(&ws->gflag, npmax * sizeof(int)));
+        ws->gcap = npmax;
+    }
+    GridGeom prev{};
+    bool have_prev = false;
+    for (int lev = L; lev >= 0; lev--) {
+        const int lw = lev == 0 ? w : ws->lw[lev - 1], lh = lev == 0 ? h : ws->lh[lev - 1];
+        const float *a = lev == 0 ? i1 : ws->pa[lev - 1], *b = lev == 0 ? i2 : ws->pb[lev - 1];
+        int wy = c.poc_win, wx = c.poc_win;
+        if (lev > 0) level_window(lh, lw, c.poc_win, wy, wx);
+        if ((long)wy * wx > POC_MAX)
+            throw Error(OFX_EUNSUPPORTED, "POC window " + std::to_string(wy) + "x" + std::to_string(wx) +
+                                              " exceeds the 4096 samples a workgroup holds");
+        int x0, nx, y0, ny;
+        grid_axis(lw, wx, c.grid_step, x0, nx);
+        grid_axis(lh, wy, c.grid_step, y0, ny);
+        const GridGeom cur{x0, y0, c.grid_step, nx, ny};
+        const size_t np = (size_t)nx * ny;
+        if (np == 0) throw Error(OFX_EINVAL, "no POC grid point fits the image");
+        // the previous level's cleaned grid lives in the upper halves of gux / guy
+        double *pux = ws->gux + ws->gcap, *puy = ws->guy + ws->gcap;
+        int have_init = 0;
+        if (have_prev) {
+            HIPCHECK(hipMemcpyAsync(pux, ws->gux, (size_t)prev.nx * prev.ny * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipMemcpyAsync(puy, ws->guy, (size_t)prev.nx * prev.ny * sizeof(double), hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(k_level_init, dim3((np + 255) / 256), dim3(256), 0, s, prev, pux, puy, cur, ws->ix, ws->iy);
+            have_init = 1;
+        } else if (init_ux) {
+            hipLaunchKernelGGL(k_dense_init, dim3((np + 255) / 256), dim3(256), 0, s, init_ux, init_uy, w, cur, 1 << lev,
+                               ws->ix, ws->iy);
+            have_init = 1;
+        }
+        LAUNCHCHECK();
+        const FftPlan &PX = fft_plan(wx), &PY = fft_plan(wy);
+        PocArgs P;
+        std::memset(&P, 0, sizeof(P));
+        P.W = lw;
+        P.H = lh;
+        P.wy = wy;
+        P.wx = wx;
+        P.x0 = x0;
+        P.y0 = y0;
+        P.step = c.grid_step;
+        P.nx = nx;
+        P.ny = ny;
+        P.nfx = PX.nfac;
+        P.nfy = PY.nfac;
+        for (int i = 0; i < MAX_FACTORS; i++) {
+            P.facx[i] = PX.fac[i];
+            P.facy[i] = PY.fac[i];
+        }
+        P.have_init = have_init;
+        const size_t lds = 2 * (size_t)wy * wx * sizeof(float2);
+        hipLaunchKernelGGL(k_poc_grid, dim3((unsigned)np), dim3(256), lds, s, a, b, P, PX.tw, PY.tw, ws->ix, ws->iy,
+                           ws->gux, ws->guy, ws->gpk);
+        LAUNCHCHECK();
+        hipLaunchKernelGGL(k_clean, dim3(1), dim3(1024), 0, s, ws->gux, ws->guy, ws->gpk, ws->ix, ws->iy, have_init,
+                           ws->gflag, ny, nx, c.poc_peak_min, c.median_thresh);
+        LAUNCHCHECK();
+        prev = cur;
+        have_prev = true;
+    }
+    // dense start (track.py:131-137)
+    double *gx = ws->gux, *gy = ws->guy;
+    if (init_ux) {
+        double *cx = ws->gux + ws->gcap, *cy = ws->guy + ws->gcap;
+        hipLaunchKernelGGL(k_grid_minus_init, dim3((prev.nx * prev.ny + 255) / 256), dim3(256), 0, s, ws->gux, ws->guy,
+                           init_ux, init_uy, w, prev, cx, cy);
+        gx = cx;
+        gy = cy;
+    }
+    hipLaunchKernelGGL(k_dense, dim3((w + 255) / 256, h), dim3(256), 0, s, prev, gx, gy, init_ux, init_uy, w, h, ux, uy);
+    LAUNCHCHECK();
+}
+
+}  // namespace ofx

@@ -1,0 +1,108 @@
+"""Parity of the HIP engine (through the C ABI) with the FP64 oracle on the same inputs, at sizes
+the oracle finishes in seconds, and against the reference's golden fixtures.
+
+Bar (the reference's Rust-vs-Python contract, rust/README.md:3-5, and the parity tests
+rust/crates/optflow-core/tests/parity_*.rs): displacement RMS <= 1e-3 px, strain RMS <= 1e-5,
+confidence RMS <= 1e-3, valid masks identical on >= 99.9 % of pixels."""
+import numpy as np
+import pytest
+
+import oracle as O
+from conftest import load, meta, rms
+from optflow_soil_amd.config import Config
+
+pytestmark = pytest.mark.gpu
+
+DISP_TOL = 1e-3
+STRAIN_TOL = 1e-5
+CONF_TOL = 1e-3
+VALID_AGREE = 0.999
+
+
+@pytest.fixture(scope="module")
+def eng():
+    import optflow_soil_amd as ofx
+    e = ofx.Engine(0)
+    yield e
+    e.close()
+
+
+def f32(a):
+    return np.asarray(a, dtype=np.float32)
+
+
+def f64(a):
+    return np.asarray(a, dtype=np.float32).astype(np.float64)
+
+
+def compare_pair(r, o, label):
+    valid_g, valid_o = np.isfinite(r.ux), np.isfinite(o["ux"])
+    agree = float((valid_g == valid_o).mean())
+    both = valid_g & valid_o
+    d = {"agree": agree, "n_valid": int(both.sum()),
+         "ux": rms(r.ux.astype(np.float64), o["ux"], both), "uy": rms(r.uy.astype(np.float64), o["uy"], both),
+         "conf": rms(r.conf.astype(np.float64), o["conf"], np.ones(both.shape, bool))}
+    fin = np.isfinite(r.strain["Exx"]) & np.isfinite(o["Exx"])
+    d["Exx"] = rms(r.strain["Exx"].astype(np.float64), o["Exx"], fin)
+    d["gmax"] = rms(r.strain["gmax"].astype(np.float64), o["gmax"], fin)
+    print(label, " ".join(f"{k}={v:.3g}" for k, v in d.items()), flush=True)
+    assert agree >= VALID_AGREE
+    assert d["ux"] <= DISP_TOL and d["uy"] <= DISP_TOL
+    assert d["Exx"] <= STRAIN_TOL and d["gmax"] <= STRAIN_TOL
+    assert d["conf"] <= CONF_TOL
+    return d
+
+
+@pytest.mark.parametrize("order", [1])
+def test_refine_fixture(eng, order):
+    a, b = load("refine_a"), load("refine_b")
+    ix, iy = load("refine_init_ux"), load("refine_init_uy")
+    cfg = Config(shape_order=order)
+    ux, uy, en, res = eng.refine_raw(f32(a), f32(b), f32(ix), f32(iy), cfg)
+    oux, ouy, oconf, ores, oen = O.refine(f64(a), f64(b), f64(ix), f64(iy), cfg)
+    e = {"ux": rms(ux, oux), "uy": rms(uy, ouy), "resid": rms(res, ores), "energy_rel": rms(en, oen) / np.abs(oen).mean(),
+         "fixture_ux": rms(ux, load(f"refine_o{order}_ux"))}
+    print("refine", " ".join(f"{k}={v:.3g}" for k, v in e.items()), flush=True)
+    assert e["ux"] <= DISP_TOL and e["uy"] <= DISP_TOL and e["fixture_ux"] <= DISP_TOL
+    assert e["resid"] <= 1e-3 and e["energy_rel"] <= 1e-4
+
+
+def test_pair_fixture(eng):
+    a, b, roi = load("pair_a"), load("pair_b"), load("pair_roi") > 0.5
+    r = eng.track_pair(f32(a), f32(b), Config(), mask=roi)
+    o = O.track_pair(f64(a), f64(b), Config(), mask=roi)
+    compare_pair(r, o, "pair")
+    # and against the reference's own fixture
+    assert (np.isfinite(r.ux) == np.isfinite(load("pair_ux"))).mean() >= VALID_AGREE
+    assert rms(r.ux.astype(np.float64), load("pair_ux")) <= DISP_TOL
+    assert np.isnan(r.ux[~roi]).all()
+
+
+def test_sequence_chain_fixture(eng):
+    frames = [f32(f) for f in load("seq_frames")]
+    cfg = Config(ref_update_conf=2.0)
+    out = eng.track_sequence(frames, cfg)
+    assert [res.timing["reference"] for _, res in out] == meta()["seq_reference"]
+    for k, (_, res) in enumerate(out):
+        ref = load("seq_ux")[k]
+        assert (np.isfinite(res.ux) == np.isfinite(ref)).mean() >= VALID_AGREE
+        assert rms(res.ux.astype(np.float64), ref) <= DISP_TOL
+        assert rms(res.uy.astype(np.float64), load("seq_uy")[k]) <= DISP_TOL
+
+
+def test_c1_chain_workload(eng):
+    """256 x 256 speckle, 5 frames, seq_mode chain, uint8 input read as v / 255."""
+    from optflow_soil_amd import synth
+    frames, truth = synth.workload_c1()
+    ff = [synth.as_float(f) for f in frames]
+    cfg = Config()
+    out = eng.track_sequence(ff, cfg)
+    ref = O.track_sequence([f.astype(np.float64) for f in ff], cfg)
+    for (k, r), o in zip(out, ref):
+        assert r.timing["reference"] == o["timing"]["reference"]
+        d = compare_pair(r, o, f"c1 frame {k}")
+        gx, gy = truth(k)
+        fin = np.isfinite(r.ux)
+        epe = float(np.sqrt(np.mean((r.ux[fin] - gx[fin]) ** 2 + (r.uy[fin] - gy[fin]) ** 2)))
+        print(f"c1 frame {k}: EPE vs truth {epe:.4f} px over {int(fin.sum())} px", flush=True)
+        assert epe < 0.2
