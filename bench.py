@@ -6,7 +6,7 @@ and a 40 px wide shear band at 35 deg with a 3 px tanh slip, noise sigma 0.005, 
 8-gon ROI of ~60 %, default settings. One step = one whole pair (coarse POC, fine stage,
 confidence, guard, strain) with the inputs already resident in HBM, through the device-pointer
 C ABI. Reports the per-stage GPU times, the roofline of the dominant kernel, the EPE against the
-synthetic truth and a CPU baseline (the FP64 oracle on a 512 x 512 crop of the same pair, timed on
+synthetic truth and a CPU baseline (the FP64 oracle on a 1024 x 1024 crop of the same pair, timed on
 the host's cores).
 """
 import argparse
@@ -40,6 +40,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only copy MIB MiB device to device (twice) and exit: the reference kernel that scales "
+                         "the HBM byte counters of a profiler run")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this engine and its benchmark have no multi-GPU path "
@@ -47,6 +50,16 @@ def main():
         sys.exit(2)
 
     import torch
+
+    if args.copy_probe:
+        n = args.copy_probe * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(2):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": n * 4}))
+        return
 
     import optflow_soil_amd as ofx
     from optflow_soil_amd import synth
@@ -127,13 +140,13 @@ def main():
 
 
 def cpu_baseline(a, b, roi, cfg):
-    """The FP64 oracle (C++/OpenMP restatement of the reference CPU path) on the central 512 x 512
+    """The FP64 oracle (C++/OpenMP restatement of the reference CPU path) on the central 1024 x 1024
     crop of the same pair, all host cores up to 16, one timed run."""
     import oracle
     from optflow_soil_amd import synth
     cores = min(os.cpu_count() or 1, 16)
     oracle.set_threads(cores)
-    c = 512
+    c = 1024
     y0 = (a.shape[0] - c) // 2
     sl = np.s_[y0:y0 + c, y0:y0 + c]
     A = synth.as_float(a[sl]).astype(np.float64)

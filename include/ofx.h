@@ -105,10 +105,12 @@ int ofx_check_window(ofx_engine *e, int w, int h, const ofx_config *cfg);
 /* FineEngine::release_caches (engine.rs:114-117): frees the per-size device workspaces. */
 int ofx_release_caches(ofx_engine *e);
 
-/* FineEngine::refine_raw (engine.rs:31-125, output RawFlow of phaseflow.rs:1408-1414): the fine
- * stage of image pair (i1, i2) from the start field (ux, uy). i2 is w2 x h2 and is sampled at
- * X + shift + s(X); this engine refines whole frames, so it requires w2 == w, h2 == h and a zero
- * shift (OFX_EUNSUPPORTED otherwise). Outputs are f64, w x h each. */
+/* FineEngine::refine_raw (engine.rs:31-41, contract of phaseflow.rs:1441-1456, output RawFlow of
+ * phaseflow.rs:1408-1414): the fine stage of i1 (w x h) from the start field (ux, uy, w x h)
+ * against i2, any w2 x h2 window of the second frame in which pixel (x, y) of i1 sits at
+ * (x + shift_x, y + shift_y) before the field displaces it (positions outside i2 are clamped to
+ * its border, as the reference's spline does). w2, h2 >= 1 and <= 7680 (OFX_ETOOLARGE above).
+ * Outputs are f64, w x h each; energy and resid are genuine f64 (planes_exact_in_f32 = false). */
 int ofx_refine_raw(ofx_engine *e, const float *i1, int w, int h, const float *i2, int w2, int h2,
                    int64_t shift_x, int64_t shift_y, const float *ux, const float *uy,
                    const ofx_config *cfg, double *out_ux, double *out_uy, double *out_energy,

@@ -175,14 +175,19 @@ class Engine:
     def release_caches(self):
         _check(lib().ofx_release_caches(self._h))
 
-    def refine_raw(self, i1, i2, ux, uy, cfg):
-        """The fine stage from the start field (ux, uy): returns (ux, uy, energy, resid), float64."""
+    def refine_raw(self, i1, i2, ux, uy, cfg, shift=(0, 0)):
+        """FineEngine::refine_raw (rust/crates/optflow-core/src/engine.rs:31-41): the fine stage from
+        the start field (ux, uy) on i1's grid; i2 is any window of the second frame, pixel (x, y) of
+        i1 sitting at (x + shift[0], y + shift[1]) in it. Returns (ux, uy, energy, resid), float64."""
         i1, i2, ux, uy = _f32(i1), _f32(i2), _f32(ux), _f32(uy)
         h, w = i1.shape
+        if ux.shape != (h, w) or uy.shape != (h, w):
+            raise ValueError(f"refine: i1 is {w}x{h} but the field is {ux.shape[1]}x{ux.shape[0]}")
         o = [np.empty((h, w)) for _ in range(4)]
         c = to_c(cfg)
-        _check(lib().ofx_refine_raw(self._h, _p(i1), w, h, _p(i2), i2.shape[1], i2.shape[0], ctypes.c_int64(0),
-                                    ctypes.c_int64(0), _p(ux), _p(uy), ctypes.byref(c), *[_p(a) for a in o]))
+        _check(lib().ofx_refine_raw(self._h, _p(i1), w, h, _p(i2), i2.shape[1], i2.shape[0],
+                                    ctypes.c_int64(int(shift[0])), ctypes.c_int64(int(shift[1])), _p(ux), _p(uy),
+                                    ctypes.byref(c), *[_p(a) for a in o]))
         return tuple(o)
 
     def track_pair(self, I1, I2, cfg, mask=None, init=None) -> Result:
@@ -254,5 +259,5 @@ def track_sequence(frames, cfg=None, mask=None):
     return default_engine().track_sequence(frames, cfg or Config(), mask=mask)
 
 
-def refine(I1, I2, ux, uy, cfg=None):
-    return default_engine().refine_raw(I1, I2, ux, uy, cfg or Config())
+def refine(I1, I2, ux, uy, cfg=None, shift=(0, 0)):
+    return default_engine().refine_raw(I1, I2, ux, uy, cfg or Config(), shift)

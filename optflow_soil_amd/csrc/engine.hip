@@ -17,6 +17,10 @@
 
 namespace ofx {
 
+// Largest side of an image-2 window: a row of its spline prefilter is held in LDS (two padded
+// float lines within 64 KiB).
+constexpr int MAX_I2_SIDE = 7680;
+
 namespace {
 
 thread_local std::string g_last_error;
@@ -112,15 +116,16 @@ struct Workspace {
     Profiler prof;
     int w = 0, h = 0, F = 0;
     DevBuf bi1, bi2, bmask, binx, biny;                 // staging of host inputs
-    DevBuf bc2, bspec, btmp, bresp, brefp;
+    DevBuf bc2, bspec, btmp, btmp2, bresp, brefp;
     DevBuf bux, buy, bnux, bnuy, bsx, bsy, bcsx, bcsy, bwarped, bfx, bfy, bbt;
     DevBuf bsums, bym, benergy, bresid, bslope, bmed, bmedscr;
     DevBuf bconf, bconf64, blow, bkept, bwithin, bok, bcold2, bcolrow, bcnt;
-    DevBuf bparent, bsizes, blarge, bsrc, bnear, bmx, bmy, bytmp;
+    DevBuf bparent, bsizes, blarge, bsrc, bnear, bmx, bmy, bytmp, bcolsum, bpftmp;
     DevBuf bstart_x, bstart_y;
     DevBuf bout;                                       // staging of host outputs (f32 x 10 + u8)
     DevBuf btaps_s, btaps_w;
     double taps_s_sigma = -1, taps_w_sigma = -1;
+    std::vector<double> taps_s_host, taps_w_host;
     int taps_s_r = 0, taps_w_r = 0;
     // sequence state
     DevBuf bu0x, bu0y, bc0r, bcx, bcy, bconf0, bca, bprev_x, bprev_y, bprev_c, bleg_c, bmean, bframe_prev, bframe_ref;
@@ -232,6 +237,7 @@ void set_taps(Workspace &ws, const ofx_config &c, hipStream_t s) {
         std::vector<double> k = gauss_taps(c.sigma_s);
         double *d = ws.btaps_s.get<double>(k.size());
         HIPCHECK(hipMemcpy(d, k.data(), k.size() * sizeof(double), hipMemcpyHostToDevice));
+        ws.taps_s_host = k;
         ws.taps_s_sigma = c.sigma_s;
         ws.taps_s_r = (int)k.size() / 2;
     }
@@ -243,6 +249,7 @@ void set_taps(Workspace &ws, const ofx_config &c, hipStream_t s) {
             for (int t = -r; t <= r; t++) kt[p * klen + t + r] = k[t + r] * std::pow((double)t, p);
         double *d = ws.btaps_w.get<double>(kt.size());
         HIPCHECK(hipMemcpy(d, kt.data(), kt.size() * sizeof(double), hipMemcpyHostToDevice));
+        ws.taps_w_host = kt;
         ws.taps_w_sigma = c.sigma_w;
         ws.taps_w_r = r;
     }
@@ -252,29 +259,34 @@ void set_taps(Workspace &ws, const ofx_config &c, hipStream_t s) {
 // The fine stage (phaseflow.py:427-511, phaseflow.rs:1514-1613) from the start field in
 // ws.bux / ws.buy (double). Leaves ux, uy, energy, resid, the energy median (ws.bmed[0]) and the
 // final confidence (ws.bconf as float, ws.bconf64 as double) in the workspace.
-void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h, const ofx_config &c, hipStream_t s) {
+// The fine stage (phaseflow.py:427-511, phaseflow.rs:1441-1613) on image 1 (w x h) against image 2
+// (w2 x h2), pixel (x, y) of image 1 sitting at (x + shx, y + shy) in image 2 before the field
+// displaces it. The start field is in ws.bux / ws.buy; the result replaces it.
+void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h, int w2, int h2, long shx, long shy,
+                   const ofx_config &c, hipStream_t s) {
     const int wp = w + 2 * PAD, hp = h + 2 * PAD;
     const size_t n = (size_t)w * h, np = (size_t)wp * hp;
     const Bank bank = make_bank(c);
     const int F = bank.F;
     set_taps(ws, c, s);
-    float2 *spec = B<float2>(ws.bspec, np), *tmp = B<float2>(ws.btmp, np);
+    float2 *spec = B<float2>(ws.bspec, np), *tmp = B<float2>(ws.btmp, np), *tmp2 = B<float2>(ws.btmp2, np);
     float2 *resp = B<float2>(ws.bresp, std::max<size_t>(F, 3) * np);
     float4 *refp = B<float4>(ws.brefp, F * n);
     // reference pack of image 1 (phaseflow.py:100-111): three inverses per filter
     ws.prof.mark(ST_REFERENCE, s);
-    fft_forward_padded(i1, w, h, spec, tmp, s);
+    fft_forward_padded(i1, w, h, spec, tmp, tmp2, s);
     for (int k = 0; k < F; k++) {
-        fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G, false, tmp, resp, s);
-        fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G_IFX, false, tmp, resp + np, s);
-        fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G_IFY, false, tmp, resp + 2 * np, s);
+        fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G, false, tmp, tmp2, resp, s);
+        fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G_IFX, false, tmp, tmp2, resp + np, s);
+        fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G_IFY, false, tmp, tmp2, resp + 2 * np, s);
         refpack(resp, resp + np, resp + 2 * np, refp + k * n, w, h, wp, s);
     }
     // spline of image 2
     ws.prof.mark(ST_WARP, s);
-    float *c2 = B<float>(ws.bc2, n);
-    HIPCHECK(hipMemcpyAsync(c2, i2, n * sizeof(float), hipMemcpyDeviceToDevice, s));
-    prefilter_real(c2, w, h, s);
+    const size_t n2 = (size_t)w2 * h2;
+    float *c2 = B<float>(ws.bc2, n2);
+    HIPCHECK(hipMemcpyAsync(c2, i2, n2 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    prefilter_real(c2, B<float>(ws.bpftmp, std::max(n, n2)), w2, h2, s);
     double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
     double *nux = B<double>(ws.bnux, n), *nuy = B<double>(ws.bnuy, n);
     float *sx = B<float>(ws.bsx, n), *sy = B<float>(ws.bsy, n), *csx = B<float>(ws.bcsx, n), *csy = B<float>(ws.bcsy, n);
@@ -296,8 +308,8 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
     for (int it = 0; it < c.outer_iters; it++) {
         if (it == 0) {
             ws.prof.mark(ST_SMOOTH, s);
-            blur(ux, bt, sx, w, h, (const double *)ws.btaps_s.p, ws.taps_s_r, s);
-            blur(uy, bt, sy, w, h, (const double *)ws.btaps_s.p, ws.taps_s_r, s);
+            blur(ux, bt, sx, w, h, ws.taps_s_host.data(), ws.taps_s_r, s);
+            blur(uy, bt, sy, w, h, ws.taps_s_host.data(), ws.taps_s_r, s);
         } else {
             // trusted pre-warp (phaseflow.py:326-355; frozen low mask: prewarp.rs:48-115)
             ws.prof.mark(ST_PREWARP, s);
@@ -311,31 +323,32 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
                     have_kept = true;
                 }
             }
-            column_nearest(lowm, w, h, cold2, colrow, s);
+            int *colsum = B<int>(ws.bcolsum, column_summary_ints(w, h));
+            column_nearest(lowm, w, h, cold2, colrow, colsum, s);
             within_distance(cold2, w, h, R, within, s);
             prewarp_trusted(lowm, within, w, h, ok, cnt, s);
-            column_nearest(ok, w, h, cold2, colrow, s);
+            column_nearest(ok, w, h, cold2, colrow, colsum, s);
             prewarp_fill(ux, uy, slope, ok, cold2, colrow, cnt, w, h, fx, fy, s);
             ws.prof.mark(ST_SMOOTH, s);
-            blur(fx, bt, sx, w, h, (const double *)ws.btaps_s.p, ws.taps_s_r, s);
-            blur(fy, bt, sy, w, h, (const double *)ws.btaps_s.p, ws.taps_s_r, s);
+            blur(fx, bt, sx, w, h, ws.taps_s_host.data(), ws.taps_s_r, s);
+            blur(fy, bt, sy, w, h, ws.taps_s_host.data(), ws.taps_s_r, s);
         }
         ws.prof.mark(ST_WARP, s);
-        warp(c2, sx, sy, warped, w, h, s);
+        warp(c2, w2, h2, shx, shy, sx, sy, warped, w, h, s);
         ws.prof.mark(ST_RESPONSES, s);
-        fft_forward_padded(warped, w, h, spec, tmp, s);
-        for (int k = 0; k < F; k++) fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G, true, tmp, resp + k * np, s);
+        fft_forward_padded(warped, w, h, spec, tmp, tmp2, s);
+        for (int k = 0; k < F; k++) fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G, true, tmp, tmp2, resp + k * np, s);
         ws.prof.mark(ST_SMOOTH, s);
         HIPCHECK(hipMemcpyAsync(csx, sx, n * sizeof(float), hipMemcpyDeviceToDevice, s));
         HIPCHECK(hipMemcpyAsync(csy, sy, n * sizeof(float), hipMemcpyDeviceToDevice, s));
-        prefilter_real(csx, w, h, s);
-        prefilter_real(csy, w, h, s);
+        prefilter_real(csx, B<float>(ws.bpftmp, n), w, h, s);
+        prefilter_real(csy, B<float>(ws.bpftmp, n), w, h, s);
         for (int in = 0; in < c.inner_iters; in++) {
             ws.prof.mark(ST_ACCUMULATE, s);
             accumulate(ux, uy, sx, sy, csx, csy, resp, refp, sums, w, h, F, s);
             ws.prof.mark(ST_MOMENTS, s);
             moments_solve(sums, ym, ux, uy, nux, nuy, energy, resid, slope, w, h, (const double *)ws.btaps_w.p,
-                          ws.taps_w_r, s);
+                          ws.taps_w_host.data(), ws.taps_w_r, s);
             ws.prof.mark(ST_SOLVE, s);
             median(energy, n, medscr, med, s);
             energy_reset(energy, med, ux, uy, nux, nuy, resid, slope, n, s);
@@ -371,7 +384,7 @@ void finish_device(Workspace &ws, const double *ux, const double *uy, const floa
         low_mask(conf, c.conf_min, n, low, s);
         large_components(low, w, h, edge_guard_min_area(c), parent, sizes, large, s);
         guard_sources(large, mask, w, h, src, s);
-        column_nearest(src, w, h, cold2, colrow, s);
+        column_nearest(src, w, h, cold2, colrow, B<int>(ws.bcolsum, column_summary_ints(w, h)), s);
         within_distance(cold2, w, h, guard, near, s);
     }
     double *mx = B<double>(ws.bmx, n), *my = B<double>(ws.bmy, n), *yt = B<double>(ws.bytmp, 11 * n);
@@ -428,7 +441,7 @@ void pair_device(Workspace &ws, const float *i1, const float *i2, int w, int h, 
     double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
     coarse_dense(i1, i2, w, h, c, init_x, init_y, ux, uy, ws.coarse, s);
     HIPCHECK(hipEventRecord(ws.ev[1], s));
-    refine_device(ws, i1, i2, w, h, c, s);
+    refine_device(ws, i1, i2, w, h, w, h, 0, 0, c, s);
     HIPCHECK(hipEventRecord(ws.ev[2], s));
     ux = (double *)ws.bux.p;
     uy = (double *)ws.buy.p;
@@ -638,22 +651,24 @@ int ofx_refine_raw(ofx_engine *e, const float *i1, int w, int h, const float *i2
     return guarded([&] {
         validate(cfg);
         if (!i1 || !i2 || !ux || !uy) throw Error(OFX_EINVAL, "input pointer is NULL");
-        if (w2 != w || h2 != h || shift_x != 0 || shift_y != 0)
-            throw Error(OFX_EUNSUPPORTED, "this engine refines whole frames: image 2 must match image 1 with a zero shift");
+        if (w2 < 1 || h2 < 1) throw Error(OFX_EINVAL, "refine: empty image i2");
+        if (w2 > MAX_I2_SIDE || h2 > MAX_I2_SIDE)
+            throw Error(OFX_ETOOLARGE, "image 2 window " + std::to_string(w2) + "x" + std::to_string(h2) +
+                                           " exceeds the " + std::to_string(MAX_I2_SIDE) + " px side the spline rows hold");
         check_window_impl(w, h, *cfg);
         LaneGuard g(e);
         Workspace &ws = *g.lane->ws;
         hipStream_t s = g.lane->stream;
-        const size_t n = (size_t)w * h;
-        float *d1 = B<float>(ws.bi1, n), *d2 = B<float>(ws.bi2, n);
+        const size_t n = (size_t)w * h, n2 = (size_t)w2 * h2;
+        float *d1 = B<float>(ws.bi1, n), *d2 = B<float>(ws.bi2, n2);
         float *dx = B<float>(ws.binx, n), *dy = B<float>(ws.biny, n);
         HIPCHECK(hipMemcpyAsync(d1, i1, n * sizeof(float), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(d2, i2, n * sizeof(float), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(d2, i2, n2 * sizeof(float), hipMemcpyHostToDevice, s));
         HIPCHECK(hipMemcpyAsync(dx, ux, n * sizeof(float), hipMemcpyHostToDevice, s));
         HIPCHECK(hipMemcpyAsync(dy, uy, n * sizeof(float), hipMemcpyHostToDevice, s));
         to_f64(dx, B<double>(ws.bux, n), n, s);
         to_f64(dy, B<double>(ws.buy, n), n, s);
-        refine_device(ws, d1, d2, w, h, *cfg, s);
+        refine_device(ws, d1, d2, w, h, w2, h2, (long)shift_x, (long)shift_y, *cfg, s);
         if (out_ux) HIPCHECK(hipMemcpyAsync(out_ux, ws.bux.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
         if (out_uy) HIPCHECK(hipMemcpyAsync(out_uy, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
         if (out_energy) HIPCHECK(hipMemcpyAsync(out_energy, ws.benergy.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
@@ -784,7 +799,7 @@ int ofx_track_sequence(ofx_engine *e, const float *frames, int nf, int w, int h,
             double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
             coarse_dense(a, b, w, h, c, with_init ? px : nullptr, with_init ? py : nullptr, ux, uy, ws.coarse, s);
             HIPCHECK(hipEventRecord(ws.ev[1], s));
-            refine_device(ws, a, b, w, h, c, s);
+            refine_device(ws, a, b, w, h, w, h, 0, 0, c, s);
             HIPCHECK(hipEventRecord(ws.ev[2], s));
             HIPCHECK(hipMemcpyAsync(lx, ws.bux.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
             HIPCHECK(hipMemcpyAsync(ly, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));

@@ -17,48 +17,74 @@ namespace {
 constexpr double ZP = -0.26794919243112270647;  // sqrt(3) - 2
 
 // ------------------------------------------------------------------ B-spline prefilter, real planes
-// Along y: one thread per column walks the column (coalesced across the wavefront), sequential
-// recursion in double (interp.py:12-26).
-__global__ void k_prefilter_cols(float *c, int w, int h) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
-    if (x >= w || h == 1) return;
-    const int hz = min(h, PF_HORIZON);
-    double acc = 0, zk = 1;
-    for (int k = 0; k < hz; k++) {
-        acc += zk * 6.0 * (double)c[(size_t)k * w + x];
-        zk *= ZP;
+// Along y: one thread per (column, segment of PF_SEG rows), coalesced across the wavefront; each
+// segment starts its recursion PF_WARM rows early (error z^PF_WARM, as the LDS form) in double.
+// Causal pass src -> mid, anti-causal pass mid -> dst (dst may be src).
+__global__ void k_prefilter_cols_causal(const float *__restrict__ src, float *__restrict__ mid, int w, int h) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (x >= w) return;
+    const int a = j * PF_SEG, b = min(a + PF_SEG, h);
+    double acc = 0;
+    int i;
+    if (a <= PF_WARM) {
+        const int hz = min(h, PF_HORIZON);
+        double zk = 1;
+        for (int k = 0; k < hz; k++) {
+            acc += zk * 6.0 * (double)src[(size_t)k * w + x];
+            zk *= ZP;
+        }
+        if (a == 0) mid[x] = (float)acc;
+        i = 1;
+    } else {
+        i = a - PF_WARM;
     }
-    double prev = acc;
-    c[x] = (float)acc;
-    for (int y = 1; y < h; y++) {
-        prev = 6.0 * (double)c[(size_t)y * w + x] + ZP * prev;
-        c[(size_t)y * w + x] = (float)prev;
+    for (; i < a; i++) acc = 6.0 * (double)src[(size_t)i * w + x] + ZP * acc;
+    for (i = max(a, 1); i < b; i++) {
+        acc = 6.0 * (double)src[(size_t)i * w + x] + ZP * acc;
+        mid[(size_t)i * w + x] = (float)acc;
     }
-    const double cm2 = (double)c[(size_t)(h - 2) * w + x];
-    double d = (ZP / (ZP * ZP - 1.0)) * (prev + ZP * cm2);
-    c[(size_t)(h - 1) * w + x] = (float)d;
-    for (int y = h - 2; y >= 0; y--) {
-        d = ZP * (d - (double)c[(size_t)y * w + x]);
-        c[(size_t)y * w + x] = (float)d;
+}
+
+__global__ void k_prefilter_cols_anti(const float *__restrict__ mid, float *__restrict__ dst, int w, int h) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (x >= w) return;
+    const int a = j * PF_SEG, b = min(a + PF_SEG, h);
+    double acc = 0;
+    int i;
+    if (b + PF_WARM >= h) {
+        acc = (ZP / (ZP * ZP - 1.0)) * ((double)mid[(size_t)(h - 1) * w + x] + ZP * (double)mid[(size_t)(h - 2) * w + x]);
+        if (b == h) dst[(size_t)(h - 1) * w + x] = (float)acc;
+        i = h - 2;
+    } else {
+        i = b + PF_WARM - 1;
+    }
+    for (; i >= b; i--) acc = ZP * (acc - (double)mid[(size_t)i * w + x]);
+    for (i = min(b - 1, h - 2); i >= a; i--) {
+        acc = ZP * (acc - (double)mid[(size_t)i * w + x]);
+        dst[(size_t)i * w + x] = (float)acc;
     }
 }
 
 // Along x: one workgroup per row, segmented recursion in LDS.
 __global__ void __launch_bounds__(256) k_prefilter_rows(float *c, int w) {
     extern __shared__ float smem[];
-    float *a = smem, *b = smem + w;
+    constexpr int SH = 4;
+    float *a = smem, *b = smem + lp_len<SH>(w);
     float *row = c + (size_t)blockIdx.x * w;
-    for (int x = threadIdx.x; x < w; x += blockDim.x) a[x] = row[x];
+    for (int x = threadIdx.x; x < w; x += blockDim.x) a[lp<SH>(x)] = row[x];
     __syncthreads();
-    prefilter_lines<float>(a, b, a, w, 1, 1.0, threadIdx.x, blockDim.x);
-    for (int x = threadIdx.x; x < w; x += blockDim.x) row[x] = a[x];
+    prefilter_lines<float, SH>(a, b, a, w, 1, 1.0, threadIdx.x, blockDim.x);
+    for (int x = threadIdx.x; x < w; x += blockDim.x) row[x] = a[lp<SH>(x)];
 }
 
 // ------------------------------------------------------------------ Gaussian smoothing
 // Separable correlation with replicated edges (interp.py:80-91): y pass double -> double, x pass
 // double -> float (the pre-warp is stored as float, phaseflow.rs:1538-1555).
-__global__ void k_blur_y(const double *__restrict__ in, double *__restrict__ out, int w, int h,
-                         const double *__restrict__ k, int r) {
+struct BT {
+    double k[2 * 64 + 1];
+};
+__global__ void k_blur_y(const double *__restrict__ in, double *__restrict__ out, int w, int h, const BT bt, int r) {
+    const double *k = bt.k;
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= w) return;
     double s = 0;
@@ -69,8 +95,8 @@ __global__ void k_blur_y(const double *__restrict__ in, double *__restrict__ out
     out[(size_t)y * w + x] = s;
 }
 
-__global__ void k_blur_x(const double *__restrict__ in, float *__restrict__ out, int w, const double *__restrict__ k,
-                         int r) {
+__global__ void k_blur_x(const double *__restrict__ in, float *__restrict__ out, int w, const BT bt, int r) {
+    const double *k = bt.k;
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= w) return;
     const double *row = in + (size_t)y * w;
@@ -80,14 +106,17 @@ __global__ void k_blur_x(const double *__restrict__ in, float *__restrict__ out,
 }
 
 // ------------------------------------------------------------------ warp of image 2
-// warped(X) = I2(X + s(X)) through the spline of I2 (phaseflow.py:455, interp.py:72-77).
-__global__ void k_warp(const float *__restrict__ c2, const float *__restrict__ sx, const float *__restrict__ sy,
-                       float *__restrict__ out, int w, int h) {
+// warped(X) = I2((x + shx) + s_x, (y + shy) + s_y) from the spline c2 of image 2 (w2 x h2), the
+// window of the second frame displaced by the shift (phaseflow.rs:1416-1439; a zero shift is
+// phaseflow.py:455 with interp.py:72-77).
+__global__ void k_warp(const float *__restrict__ c2, int w2, int h2, double shx, double shy,
+                       const float *__restrict__ sx, const float *__restrict__ sy, float *__restrict__ out, int w,
+                       int h) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= w) return;
     const size_t i = (size_t)y * w + x;
-    const Taps tx = taps_at((double)x + (double)sx[i], w), ty = taps_at((double)y + (double)sy[i], h);
-    out[i] = sample_real(c2, w, tx, ty);
+    const Taps tx = taps_at(((double)x + shx) + (double)sx[i], w2), ty = taps_at(((double)y + shy) + (double)sy[i], h2);
+    out[i] = sample_real(c2, w2, tx, ty);
 }
 
 // ------------------------------------------------------------------ reference pack
@@ -113,47 +142,184 @@ __global__ void k_refpack(const float2 *__restrict__ r, const float2 *__restrict
 // against the reference and the seven sums of the local solve (phaseflow.py:473-483):
 // gxx, gxy, gyy, gxc, gyc, gcc, w with c = dphi + g . u. Samples and the phase are float; c and
 // the sums are double because c carries g . u (tens of rad) whose square enters the residual.
-__global__ void __launch_bounds__(256) k_accumulate(const double *__restrict__ ux, const double *__restrict__ uy,
-                                                    const float *__restrict__ sx, const float *__restrict__ sy,
-                                                    const float *__restrict__ csx, const float *__restrict__ csy,
-                                                    const float2 *__restrict__ resp, const float4 *__restrict__ refp,
-                                                    double *__restrict__ sums, int w, int h, int wp, int hp, int F) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x >= w) return;
+// Tiled accumulation: a 16 x 16 pixel tile finds the bounding box of its 4 x 4 response taps, and
+// if it holds at most ACC_BOX nodes (smooth fields: about (16 + 3 + |spread of u - s|)^2), every
+// filter's box is staged in LDS with coalesced loads (double-buffered: the box of filter f + 1 is
+// fetched while filter f is evaluated) and the taps are read from LDS. Tiles whose taps spread
+// wider, or reach the mirrored border of the padded plane, gather their taps from global memory.
+constexpr int ACC_T = 16;
+constexpr int ACC_BOX = 1600;  // nodes per staged box (40 x 40)
+
+__global__ void __launch_bounds__(256) k_accumulate_tiled(const double *__restrict__ ux, const double *__restrict__ uy,
+                                                          const float *__restrict__ sx, const float *__restrict__ sy,
+                                                          const float *__restrict__ csx, const float *__restrict__ csy,
+                                                          const float2 *__restrict__ resp,
+                                                          const float4 *__restrict__ refp, double *__restrict__ sums,
+                                                          int w, int h, int wp, int hp, int nf) {
+    __shared__ float2 box[2][ACC_BOX];
+    __shared__ int red[4][8];
+    __shared__ int geom[4];
+    const int lx = threadIdx.x & (ACC_T - 1), ly = threadIdx.x >> 4;
+    const int x = blockIdx.x * ACC_T + lx, y = blockIdx.y * ACC_T + ly;
+    const bool inside = x < w && y < h;
     const size_t n = (size_t)w * h, np = (size_t)wp * hp;
-    const size_t i = (size_t)y * w + x;
-    const double u = ux[i], v = uy[i];
-    const double tx = x + u, ty = y + v;
-    double px = tx - (double)sx[i], py = ty - (double)sy[i];
-    for (int it = 0; it < 2; it++) {
-        const Taps ax = taps_at(px, w), ay = taps_at(py, h);
-        const double nx = tx - (double)sample_real(csx, w, ax, ay);
-        const double ny = ty - (double)sample_real(csy, w, ax, ay);
-        px = nx;
-        py = ny;
+    const size_t i = inside ? (size_t)y * w + x : 0;
+    double u = 0, v = 0, px = 0, py = 0;
+    if (inside) {
+        u = ux[i];
+        v = uy[i];
+        const double tx = x + u, ty = y + v;
+        px = tx - (double)sx[i];
+        py = ty - (double)sy[i];
+        for (int it = 0; it < 2; it++) {
+            const Taps ax = taps_at(px, w), ay = taps_at(py, h);
+            const double nx = tx - (double)sample_real(csx, w, ax, ay);
+            const double ny = ty - (double)sample_real(csy, w, ax, ay);
+            px = nx;
+            py = ny;
+        }
     }
     const Taps qx = taps_at(px + PAD, wp), qy = taps_at(py + PAD, hp);
-    double gxx = 0, gxy = 0, gyy = 0, gxc = 0, gyc = 0, gcc = 0, gw = 0;
-    for (int f = 0; f < F; f++) {
-        const float2 r2 = sample_cplx(resp + (size_t)f * np, wp, qx, qy);
-        const float4 p = refp[(size_t)f * n + i];
-        // r1 * conj(r2)
-        const float re = p.x * r2.x + p.y * r2.y;
-        const float im = p.y * r2.x - p.x * r2.y;
-        const double dphi = (double)atan2f(im, re);
-        const double a1 = sqrt((double)p.x * p.x + (double)p.y * p.y + 1e-12);
-        const double wk = a1 * (double)hypotf(r2.x, r2.y);
-        const double gx = p.z, gy = p.w;
-        const double c = dphi + gx * u + gy * v;
-        const double wgx = wk * gx, wgy = wk * gy;
-        gxx += wgx * gx;
-        gxy += wgx * gy;
-        gyy += wgy * gy;
-        gxc += wgx * c;
-        gyc += wgy * c;
-        gcc += wk * c * c;
-        gw += wk;
+    // tap range of this pixel; a mirrored tap (border of the padded plane) sends the tile to the
+    // global path
+    int x0 = qx.idx[0], x1 = qx.idx[3], y0 = qy.idx[0], y1 = qy.idx[3];
+    bool plain = qx.idx[1] == x0 + 1 && qx.idx[2] == x0 + 2 && x1 == x0 + 3 && qy.idx[1] == y0 + 1 &&
+                 qy.idx[2] == y0 + 2 && y1 == y0 + 3;
+    if (!inside) {
+        x0 = y0 = 0x7fffffff;
+        x1 = y1 = -1;
+        plain = true;
     }
+    // block min / max of the tap ranges (wavefront shuffles, then the 4 wavefronts)
+    int mnx = x0, mxx = x1, mny = y0, mxy = y1, pl = plain ? 1 : 0;
+    for (int off = 32; off > 0; off >>= 1) {
+        mnx = min(mnx, __shfl_xor(mnx, off));
+        mxx = max(mxx, __shfl_xor(mxx, off));
+        mny = min(mny, __shfl_xor(mny, off));
+        mxy = max(mxy, __shfl_xor(mxy, off));
+        pl = min(pl, __shfl_xor(pl, off));
+    }
+    const int wv = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        red[wv][0] = mnx;
+        red[wv][1] = mxx;
+        red[wv][2] = mny;
+        red[wv][3] = mxy;
+        red[wv][4] = pl;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int a = red[0][0], b = red[0][1], c = red[0][2], d = red[0][3], e = red[0][4];
+        for (int k = 1; k < 4; k++) {
+            a = min(a, red[k][0]);
+            b = max(b, red[k][1]);
+            c = min(c, red[k][2]);
+            d = max(d, red[k][3]);
+            e = min(e, red[k][4]);
+        }
+        const int bw = b - a + 1, bh = d - c + 1;
+        geom[0] = a;
+        geom[1] = c;
+        geom[2] = bw;
+        geom[3] = (e && bw > 0 && bh > 0 && (long)bw * bh <= ACC_BOX) ? bh : -1;
+    }
+    __syncthreads();
+    const int bx0 = geom[0], by0 = geom[1], bw = geom[2], bh = geom[3];
+    double gxx = 0, gxy = 0, gyy = 0, gxc = 0, gyc = 0, gcc = 0, gw = 0;
+    if (bh > 0) {
+        const int bn = bw * bh;
+        // tap offsets inside the box (pixels outside the frame evaluate nothing)
+        const int ox = inside ? x0 - bx0 : 0, oy = inside ? y0 - by0 : 0;
+        int o[16];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) o[4 * j + k] = (oy + j) * bw + ox + k;
+        for (int e = threadIdx.x; e < bn; e += blockDim.x) {
+            const int r = e / bw, c = e - r * bw;
+            box[0][e] = resp[(size_t)(by0 + r) * wp + bx0 + c];
+        }
+        float4 pc = inside ? refp[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int f = 0; f < nf; f++) {
+            __syncthreads();
+            float2 nx[ACC_BOX / 256 + 1];
+            float4 pn = pc;
+            if (f + 1 < nf) {
+                const float2 *plane = resp + (size_t)(f + 1) * np;
+#pragma unroll
+                for (int q = 0; q < ACC_BOX / 256 + 1; q++) {
+                    const int e = threadIdx.x + 256 * q;
+                    if (e < bn) {
+                        const int r = e / bw, c = e - r * bw;
+                        nx[q] = plane[(size_t)(by0 + r) * wp + bx0 + c];
+                    }
+                }
+                if (inside) pn = refp[(size_t)(f + 1) * n + i];
+            }
+            if (inside) {
+                const float2 *bb = box[f & 1];
+                float r2x = 0.f, r2y = 0.f;
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    float ax = 0.f, ay = 0.f;
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const float2 t = bb[o[4 * j + k]];
+                        ax = fmaf(qx.w[k], t.x, ax);
+                        ay = fmaf(qx.w[k], t.y, ay);
+                    }
+                    r2x = fmaf(qy.w[j], ax, r2x);
+                    r2y = fmaf(qy.w[j], ay, r2y);
+                }
+                const float4 p = pc;
+                const float re = p.x * r2x + p.y * r2y;
+                const float im = p.y * r2x - p.x * r2y;
+                const double dphi = (double)atan2f(im, re);
+                const double a1 = sqrt((double)p.x * p.x + (double)p.y * p.y + 1e-12);
+                const double wk = a1 * (double)hypotf(r2x, r2y);
+                const double gx = p.z, gy = p.w;
+                const double c = dphi + gx * u + gy * v;
+                const double wgx = wk * gx, wgy = wk * gy;
+                gxx += wgx * gx;
+                gxy += wgx * gy;
+                gyy += wgy * gy;
+                gxc += wgx * c;
+                gyc += wgy * c;
+                gcc += wk * c * c;
+                gw += wk;
+            }
+            if (f + 1 < nf) {
+                float2 *dst = box[(f + 1) & 1];
+#pragma unroll
+                for (int q = 0; q < ACC_BOX / 256 + 1; q++) {
+                    const int e = threadIdx.x + 256 * q;
+                    if (e < bn) dst[e] = nx[q];
+                }
+            }
+            pc = pn;
+        }
+    } else if (inside) {
+        for (int f = 0; f < nf; f++) {
+            const float2 r2 = sample_cplx(resp + (size_t)f * np, wp, qx, qy);
+            const float4 p = refp[(size_t)f * n + i];
+            const float re = p.x * r2.x + p.y * r2.y;
+            const float im = p.y * r2.x - p.x * r2.y;
+            const double dphi = (double)atan2f(im, re);
+            const double a1 = sqrt((double)p.x * p.x + (double)p.y * p.y + 1e-12);
+            const double wk = a1 * (double)hypotf(r2.x, r2.y);
+            const double gx = p.z, gy = p.w;
+            const double c = dphi + gx * u + gy * v;
+            const double wgx = wk * gx, wgy = wk * gy;
+            gxx += wgx * gx;
+            gxy += wgx * gy;
+            gyy += wgy * gy;
+            gxc += wgx * c;
+            gyc += wgy * c;
+            gcc += wk * c * c;
+            gw += wk;
+        }
+    }
+    if (!inside) return;
     sums[0 * n + i] = gxx;
     sums[1 * n + i] = gxy;
     sums[2 * n + i] = gyy;
@@ -200,6 +366,12 @@ __global__ void k_moments_y(const double *__restrict__ sums, double *__restrict_
 #pragma unroll
     for (int q = 0; q < 15; q++) out[q * n + i] = acc[q];
 }
+
+__device__ __forceinline__ void solve_pixel(const double (&mp)[3][3][3], const double (&mr)[2][2][2], double mcc,
+                                            double mw, size_t i, const double *__restrict__ ux_in,
+                                            const double *__restrict__ uy_in, double *__restrict__ ux_out,
+                                            double *__restrict__ uy_out, double *__restrict__ energy,
+                                            double *__restrict__ resid, double *__restrict__ slope);
 
 // x pass of the moments, then the 6 x 6 system of the affine shape function per pixel
 // (phaseflow.py:132-161; Cholesky as rust phaseflow.rs:394-455): unknowns
@@ -255,6 +427,16 @@ __global__ void __launch_bounds__(256) k_moments_x_solve(const double *__restric
         mcc += k0 * row[13 * n + xx];
         mw += k0 * row[14 * n + xx];
     }
+    solve_pixel(mp, mr, mcc, mw, (size_t)y * w + x, ux_in, uy_in, ux_out, uy_out, energy, resid, slope);
+}
+
+// The 6 x 6 system of the affine shape function at pixel i from its moments (phaseflow.py:132-161;
+// Cholesky as rust phaseflow.rs:394-455): unknowns (ux, uy, dux/dx, duy/dx, dux/dy, duy/dy).
+__device__ __forceinline__ void solve_pixel(const double (&mp)[3][3][3], const double (&mr)[2][2][2], double mcc,
+                                            double mw, size_t i, const double *__restrict__ ux_in,
+                                            const double *__restrict__ uy_in, double *__restrict__ ux_out,
+                                            double *__restrict__ uy_out, double *__restrict__ energy,
+                                            double *__restrict__ resid, double *__restrict__ slope) {
     // unknown i -> (component, x power, y power)
     const int uc[6] = {0, 1, 0, 1, 0, 1}, ua[6] = {0, 0, 1, 1, 0, 0}, ub[6] = {0, 0, 0, 0, 1, 1};
     double M[6][6], bv[6], b0[6];
@@ -267,7 +449,6 @@ __global__ void __launch_bounds__(256) k_moments_x_solve(const double *__restric
         }
         bv[a] = mr[uc[a]][ua[a]][ub[a]];
     }
-    const size_t i = (size_t)y * w + x;
     const double e = M[0][0] + M[1][1];
     const double det2 = M[0][0] * M[1][1] - M[0][1] * M[0][1];
     energy[i] = e;
@@ -340,6 +521,262 @@ __global__ void __launch_bounds__(256) k_moments_x_solve(const double *__restric
         uy_out[i] = uy_in[i];
         sl[0] = sl[1] = sl[2] = sl[3] = 0;
     }
+}
+
+// Tiled forms of the two moment passes (radius r <= MT_MAXR). y pass: a 64-column x 64-row output
+// tile, one input plane at a time staged with its (64 + 2r)-row halo in LDS; each thread covers one
+// column and 16 rows. x pass: a 256-pixel row segment, all 15 y-pass planes with the 2r-column halo
+// in LDS, then the solve.
+constexpr int MT_MAXR = 32;
+constexpr int MT_TW = 64, MT_TH = 64;
+// the window weights k t^p (p = 0, 1, 2) as a kernel argument, read through the scalar cache
+struct KT {
+    double k[3][2 * MT_MAXR + 1];
+};
+
+__global__ void __launch_bounds__(256) k_moments_y_tiled(const double *__restrict__ sums, double *__restrict__ out,
+                                                         int w, int h, const KT kt, int r) {
+    extern __shared__ double tile[];  // (MT_TH + 2r) x MT_TW
+    const int x0 = blockIdx.x * MT_TW, y0 = blockIdx.y * MT_TH;
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const size_t n = (size_t)w * h;
+    const int klen = 2 * r + 1, rows = MT_TH + 2 * r;
+    const int x = x0 + tx;
+    // input plane q feeds y powers 0..npow-1 into outputs base .. base + npow - 1
+    const int npow[7] = {3, 3, 3, 2, 2, 1, 1}, base[7] = {0, 3, 6, 9, 11, 13, 14};
+    for (int q = 0; q < 7; q++) {
+        const double *src = sums + q * n;
+        __syncthreads();
+        for (int idx = threadIdx.x; idx < rows * MT_TW; idx += blockDim.x) {
+            const int ry = idx / MT_TW, cx = idx - ry * MT_TW;
+            const int yy = min(max(y0 - r + ry, 0), h - 1), xx = min(x0 + cx, w - 1);
+            tile[idx] = src[(size_t)yy * w + xx];
+        }
+        __syncthreads();
+        if (x >= w) continue;
+        for (int k = 0; k < MT_TH / 4; k++) {
+            const int ly = ty + 4 * k, y = y0 + ly;
+            if (y >= h) break;
+            double a0 = 0, a1 = 0, a2 = 0;
+            const double *col = tile + ly * MT_TW + tx;  // input row y - r at col[0]
+            if (npow[q] == 3) {
+                for (int j = 0; j < klen; j++) {
+                    const double v = col[j * MT_TW];
+                    a0 += kt.k[0][j] * v;
+                    a1 += kt.k[1][j] * v;
+                    a2 += kt.k[2][j] * v;
+                }
+            } else if (npow[q] == 2) {
+                for (int j = 0; j < klen; j++) {
+                    const double v = col[j * MT_TW];
+                    a0 += kt.k[0][j] * v;
+                    a1 += kt.k[1][j] * v;
+                }
+            } else {
+                for (int j = 0; j < klen; j++) a0 += kt.k[0][j] * col[j * MT_TW];
+            }
+            const size_t i = (size_t)y * w + x;
+            out[(size_t)base[q] * n + i] = a0;
+            if (npow[q] > 1) out[(size_t)(base[q] + 1) * n + i] = a1;
+            if (npow[q] > 2) out[(size_t)(base[q] + 2) * n + i] = a2;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_moments_x_solve_tiled(const double *__restrict__ ym,
+                                                               const double *__restrict__ ux_in,
+                                                               const double *__restrict__ uy_in,
+                                                               double *__restrict__ ux_out, double *__restrict__ uy_out,
+                                                               double *__restrict__ energy, double *__restrict__ resid,
+                                                               double *__restrict__ slope, int w, int h, const KT kt,
+                                                               int r) {
+    extern __shared__ double seg[];  // 15 planes x (256 + 2r)
+    const int x0 = blockIdx.x * 256, y = blockIdx.y;
+    const int cols = 256 + 2 * r, klen = 2 * r + 1;
+    const size_t n = (size_t)w * h;
+    for (int idx = threadIdx.x; idx < 15 * cols; idx += blockDim.x) {
+        const int q = idx / cols, c = idx - q * cols;
+        const int xx = min(max(x0 - r + c, 0), w - 1);
+        seg[idx] = ym[(size_t)q * n + (size_t)y * w + xx];
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x;
+    if (x >= w) return;
+    double mp[3][3][3], mr[2][2][2], mcc = 0, mw = 0;
+#pragma unroll
+    for (int p = 0; p < 3; p++)
+#pragma unroll
+        for (int a = 0; a < 3; a++)
+#pragma unroll
+            for (int b = 0; b < 3; b++) mp[p][a][b] = 0;
+#pragma unroll
+    for (int p = 0; p < 2; p++)
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int b = 0; b < 2; b++) mr[p][a][b] = 0;
+    const double *s0 = seg + threadIdx.x;  // column x - r
+    for (int j = 0; j < klen; j++) {
+        const double k0 = kt.k[0][j], k1 = kt.k[1][j], k2 = kt.k[2][j];
+#pragma unroll
+        for (int p = 0; p < 3; p++) {
+            const double v0 = s0[(3 * p + 0) * cols + j], v1 = s0[(3 * p + 1) * cols + j], v2 = s0[(3 * p + 2) * cols + j];
+            mp[p][0][0] += k0 * v0;
+            mp[p][1][0] += k1 * v0;
+            mp[p][2][0] += k2 * v0;
+            mp[p][0][1] += k0 * v1;
+            mp[p][1][1] += k1 * v1;
+            mp[p][0][2] += k0 * v2;
+        }
+#pragma unroll
+        for (int p = 0; p < 2; p++) {
+            const double v0 = s0[(9 + 2 * p) * cols + j], v1 = s0[(10 + 2 * p) * cols + j];
+            mr[p][0][0] += k0 * v0;
+            mr[p][1][0] += k1 * v0;
+            mr[p][0][1] += k0 * v1;
+        }
+        mcc += k0 * s0[13 * cols + j];
+        mw += k0 * s0[14 * cols + j];
+    }
+    solve_pixel(mp, mr, mcc, mw, (size_t)y * w + x, ux_in, uy_in, ux_out, uy_out, energy, resid, slope);
+}
+
+// Register-blocked forms for a compile-time radius R (the default sigma_w = 4 gives R = 16).
+// y pass: one thread per (column, 16 consecutive output rows, input plane); the 16 + 2R input rows
+// are read once straight from global memory (coalesced across the wavefront) and each is added
+// into every output row it reaches.
+constexpr int RB_ROWS = 16;
+
+template <int R, int NP>
+__device__ __forceinline__ void y_block(const double *__restrict__ src, double *__restrict__ out, size_t n, int w,
+                                        int h, int x, int y0, const KT &kt) {
+    double acc[RB_ROWS][NP];
+#pragma unroll
+    for (int q = 0; q < RB_ROWS; q++)
+#pragma unroll
+        for (int p = 0; p < NP; p++) acc[q][p] = 0;
+#pragma unroll
+    for (int j = 0; j < RB_ROWS + 2 * R; j++) {
+        const int yy = min(max(y0 - R + j, 0), h - 1);
+        const double v = src[(size_t)yy * w + x];
+#pragma unroll
+        for (int q = 0; q < RB_ROWS; q++) {
+            const int t = j - q;  // tap index 0 .. 2R of output row y0 + q
+            if (t >= 0 && t <= 2 * R) {
+#pragma unroll
+                for (int p = 0; p < NP; p++) acc[q][p] += kt.k[p][t] * v;
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RB_ROWS; q++) {
+        const int y = y0 + q;
+        if (y < h) {
+#pragma unroll
+            for (int p = 0; p < NP; p++) out[(size_t)p * n + (size_t)y * w + x] = acc[q][p];
+        }
+    }
+}
+
+template <int R>
+__global__ void __launch_bounds__(256) k_moments_y_rb(const double *__restrict__ sums, double *__restrict__ out, int w,
+                                                      int h, const KT kt) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y0 = blockIdx.y * RB_ROWS, q = blockIdx.z;
+    if (x >= w) return;
+    const size_t n = (size_t)w * h;
+    // plane q of the sums feeds the y powers 0 .. NP - 1 of outputs base ..
+    const double *src = sums + (size_t)q * n;
+    switch (q) {
+        case 0: y_block<R, 3>(src, out + 0 * n, n, w, h, x, y0, kt); break;
+        case 1: y_block<R, 3>(src, out + 3 * n, n, w, h, x, y0, kt); break;
+        case 2: y_block<R, 3>(src, out + 6 * n, n, w, h, x, y0, kt); break;
+        case 3: y_block<R, 2>(src, out + 9 * n, n, w, h, x, y0, kt); break;
+        case 4: y_block<R, 2>(src, out + 11 * n, n, w, h, x, y0, kt); break;
+        case 5: y_block<R, 1>(src, out + 13 * n, n, w, h, x, y0, kt); break;
+        default: y_block<R, 1>(src, out + 14 * n, n, w, h, x, y0, kt); break;
+    }
+}
+
+// x pass for R = 16 with 2 adjacent pixels per thread from the LDS row segment: each loaded value
+// serves both pixels, halving the LDS reads of the one-pixel form.
+__global__ void __launch_bounds__(128) k_moments_x_solve_tiled2(const double *__restrict__ ym,
+                                                                const double *__restrict__ ux_in,
+                                                                const double *__restrict__ uy_in,
+                                                                double *__restrict__ ux_out,
+                                                                double *__restrict__ uy_out,
+                                                                double *__restrict__ energy,
+                                                                double *__restrict__ resid, double *__restrict__ slope,
+                                                                int w, int h, const KT kt) {
+    constexpr int R = 16;
+    __shared__ double seg[15 * (256 + 2 * R)];
+    const int x0 = blockIdx.x * 256, y = blockIdx.y;
+    constexpr int cols = 256 + 2 * R;
+    const size_t n = (size_t)w * h;
+    // column c of plane q at q * cols + (c & 1) * cols / 2 + c / 2: even and odd columns apart, so
+    // that the lanes reading columns 2 t + j touch consecutive addresses
+    for (int idx = threadIdx.x; idx < 15 * cols; idx += blockDim.x) {
+        const int q = idx / cols, c = idx - q * cols;
+        const int xx = min(max(x0 - R + c, 0), w - 1);
+        seg[q * cols + (c & 1) * (cols / 2) + (c >> 1)] = ym[(size_t)q * n + (size_t)y * w + xx];
+    }
+    __syncthreads();
+    const int xa = x0 + 2 * threadIdx.x;
+    if (xa >= w) return;
+    double mp[2][3][3][3], mr[2][2][2][2], mcc[2] = {0, 0}, mw[2] = {0, 0};
+#pragma unroll
+    for (int e = 0; e < 2; e++) {
+#pragma unroll
+        for (int p = 0; p < 3; p++)
+#pragma unroll
+            for (int a = 0; a < 3; a++)
+#pragma unroll
+                for (int b = 0; b < 3; b++) mp[e][p][a][b] = 0;
+#pragma unroll
+        for (int p = 0; p < 2; p++)
+#pragma unroll
+            for (int a = 0; a < 2; a++)
+#pragma unroll
+                for (int b = 0; b < 2; b++) mr[e][p][a][b] = 0;
+    }
+    const double *se = seg + threadIdx.x, *so = seg + cols / 2 + threadIdx.x;  // columns xa - R (+1)
+    for (int jj = 0; jj < 1 + R; jj++) {
+#pragma unroll
+      for (int par = 0; par < 2; par++) {
+        const int j = 2 * jj + par;
+        double v[15];
+#pragma unroll
+        for (int q = 0; q < 15; q++) v[q] = (par ? so : se)[q * cols + jj];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int t = j - e;
+            if (t >= 0 && t <= 2 * R) {
+                const double k0 = kt.k[0][t], k1 = kt.k[1][t], k2 = kt.k[2][t];
+#pragma unroll
+                for (int p = 0; p < 3; p++) {
+                    mp[e][p][0][0] += k0 * v[3 * p];
+                    mp[e][p][1][0] += k1 * v[3 * p];
+                    mp[e][p][2][0] += k2 * v[3 * p];
+                    mp[e][p][0][1] += k0 * v[3 * p + 1];
+                    mp[e][p][1][1] += k1 * v[3 * p + 1];
+                    mp[e][p][0][2] += k0 * v[3 * p + 2];
+                }
+#pragma unroll
+                for (int p = 0; p < 2; p++) {
+                    mr[e][p][0][0] += k0 * v[9 + 2 * p];
+                    mr[e][p][1][0] += k1 * v[9 + 2 * p];
+                    mr[e][p][0][1] += k0 * v[10 + 2 * p];
+                }
+                mcc[e] += k0 * v[13];
+                mw[e] += k0 * v[14];
+            }
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 2; e++)
+        if (xa + e < w)
+            solve_pixel(mp[e], mr[e], mcc[e], mw[e], (size_t)y * w + xa + e, ux_in, uy_in, ux_out, uy_out, energy,
+                        resid, slope);
 }
 
 // Energy threshold (phaseflow.py:149): pixels with energy <= 1e-9 median + 1e-30 (or NaN) keep the
@@ -431,15 +868,25 @@ __global__ void __launch_bounds__(256) k_select_below(const double *__restrict__
             mx = k > mx ? k : mx;
         }
     }
-    // wavefront reduction, then one atomic per wavefront
+    // wavefront reduction, then one atomic per workgroup
+    __shared__ unsigned long long rc[4], rm[4];
     for (int off = 32; off > 0; off >>= 1) {
         cnt += __shfl_down(cnt, off);
         unsigned long long o = __shfl_down(mx, off);
         mx = o > mx ? o : mx;
     }
-    if ((threadIdx.x & 63) == 0 && cnt) {
-        atomicAdd(&st->below_count, cnt);
-        atomicMax(&st->below_max, mx);
+    if ((threadIdx.x & 63) == 0) {
+        rc[threadIdx.x >> 6] = cnt;
+        rm[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long c = rc[0] + rc[1] + rc[2] + rc[3];
+        unsigned long long m = max(max(rm[0], rm[1]), max(rm[2], rm[3]));
+        if (c) {
+            atomicAdd(&st->below_count, c);
+            atomicMax(&st->below_max, m);
+        }
     }
 }
 
@@ -491,21 +938,30 @@ __global__ void k_confidence(const double *__restrict__ energy, const double *__
 // ------------------------------------------------------------------ host wrappers
 static dim3 grid2(int w, int h) { return dim3((w + 255) / 256, h); }
 
-void prefilter_real(float *c, int w, int h, hipStream_t s) {
-    if (h > 1) hipLaunchKernelGGL(k_prefilter_cols, dim3((w + 255) / 256), dim3(256), 0, s, c, w, h);
+void prefilter_real(float *c, float *tmp, int w, int h, hipStream_t s) {
+    if (h > 1) {
+        const dim3 g((w + 255) / 256, (h + PF_SEG - 1) / PF_SEG);
+        hipLaunchKernelGGL(k_prefilter_cols_causal, g, dim3(256), 0, s, (const float *)c, tmp, w, h);
+        hipLaunchKernelGGL(k_prefilter_cols_anti, g, dim3(256), 0, s, (const float *)tmp, c, w, h);
+    }
     LAUNCHCHECK();
-    hipLaunchKernelGGL(k_prefilter_rows, dim3(h), dim3(256), 2 * w * sizeof(float), s, c, w);
-    LAUNCHCHECK();
-}
-
-void blur(const double *in, double *tmp, float *out, int w, int h, const double *taps, int r, hipStream_t s) {
-    hipLaunchKernelGGL(k_blur_y, grid2(w, h), dim3(256), 0, s, in, tmp, w, h, taps, r);
-    hipLaunchKernelGGL(k_blur_x, grid2(w, h), dim3(256), 0, s, tmp, out, w, taps, r);
+    hipLaunchKernelGGL(k_prefilter_rows, dim3(h), dim3(256), 2 * lp_len<4>(w) * sizeof(float), s, c, w);
     LAUNCHCHECK();
 }
 
-void warp(const float *c2, const float *sx, const float *sy, float *out, int w, int h, hipStream_t s) {
-    hipLaunchKernelGGL(k_warp, grid2(w, h), dim3(256), 0, s, c2, sx, sy, out, w, h);
+void blur(const double *in, double *tmp, float *out, int w, int h, const double *taps_host, int r, hipStream_t s) {
+    if (r > 64) throw Error(OFX_EUNSUPPORTED, "sigma_s above 16 px is not supported by the GPU blur");
+    BT bt;
+    for (int j = 0; j <= 2 * r; j++) bt.k[j] = taps_host[j];
+    hipLaunchKernelGGL(k_blur_y, grid2(w, h), dim3(256), 0, s, in, tmp, w, h, bt, r);
+    hipLaunchKernelGGL(k_blur_x, grid2(w, h), dim3(256), 0, s, tmp, out, w, bt, r);
+    LAUNCHCHECK();
+}
+
+void warp(const float *c2, int w2, int h2, long shx, long shy, const float *sx, const float *sy, float *out, int w,
+          int h, hipStream_t s) {
+    hipLaunchKernelGGL(k_warp, grid2(w, h), dim3(256), 0, s, c2, w2, h2, (double)shx, (double)shy, sx, sy, out, w,
+                       h);
     LAUNCHCHECK();
 }
 
@@ -518,17 +974,38 @@ void accumulate(const double *ux, const double *uy, const float *sx, const float
                 const float *csy, const float2 *resp, const float4 *refp, double *sums, int w, int h, int F,
                 hipStream_t s) {
     const int wp = w + 2 * PAD, hp = h + 2 * PAD;
-    hipLaunchKernelGGL(k_accumulate, grid2(w, h), dim3(256), 0, s, ux, uy, sx, sy, csx, csy, resp, refp, sums, w, h,
-                       wp, hp, F);
+    hipLaunchKernelGGL(k_accumulate_tiled, dim3((w + ACC_T - 1) / ACC_T, (h + ACC_T - 1) / ACC_T), dim3(256), 0, s,
+                       ux, uy, sx, sy, csx, csy, resp, refp, sums, w, h, wp, hp, F);
     LAUNCHCHECK();
 }
 
 void moments_solve(const double *sums, double *ym, const double *ux_in, const double *uy_in, double *ux_out,
                    double *uy_out, double *energy, double *resid, double *slope, int w, int h, const double *kt,
-                   int r, hipStream_t s) {
-    hipLaunchKernelGGL(k_moments_y, grid2(w, h), dim3(256), 0, s, sums, ym, w, h, kt, r);
-    hipLaunchKernelGGL(k_moments_x_solve, grid2(w, h), dim3(256), 0, s, ym, ux_in, uy_in, ux_out, uy_out, energy,
-                       resid, slope, w, h, kt, r);
+                   const double *kt_host, int r, hipStream_t s) {
+    if (r == 16 || r <= MT_MAXR) {
+        KT kt_arg;
+        const int klen = 2 * r + 1;
+        for (int p = 0; p < 3; p++)
+            for (int j = 0; j < klen; j++) kt_arg.k[p][j] = kt_host[p * klen + j];
+        if (r == 16) {
+            hipLaunchKernelGGL(k_moments_y_rb<16>, dim3((w + 255) / 256, (h + RB_ROWS - 1) / RB_ROWS, 7), dim3(256), 0,
+                               s, sums, ym, w, h, kt_arg);
+            hipLaunchKernelGGL(k_moments_x_solve_tiled, dim3((w + 255) / 256, h), dim3(256),
+                               (size_t)15 * (256 + 2 * r) * sizeof(double), s, ym, ux_in, uy_in, ux_out, uy_out, energy,
+                               resid, slope, w, h, kt_arg, r);
+            LAUNCHCHECK();
+            return;
+        }
+        hipLaunchKernelGGL(k_moments_y_tiled, dim3((w + MT_TW - 1) / MT_TW, (h + MT_TH - 1) / MT_TH), dim3(256),
+                           (size_t)(MT_TH + 2 * r) * MT_TW * sizeof(double), s, sums, ym, w, h, kt_arg, r);
+        hipLaunchKernelGGL(k_moments_x_solve_tiled, dim3((w + 255) / 256, h), dim3(256),
+                           (size_t)15 * (256 + 2 * r) * sizeof(double), s, ym, ux_in, uy_in, ux_out, uy_out, energy,
+                           resid, slope, w, h, kt_arg, r);
+    } else {
+        hipLaunchKernelGGL(k_moments_y, grid2(w, h), dim3(256), 0, s, sums, ym, w, h, kt, r);
+        hipLaunchKernelGGL(k_moments_x_solve, grid2(w, h), dim3(256), 0, s, ym, ux_in, uy_in, ux_out, uy_out, energy,
+                           resid, slope, w, h, kt, r);
+    }
     LAUNCHCHECK();
 }
 

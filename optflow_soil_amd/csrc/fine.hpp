@@ -5,16 +5,20 @@
 namespace ofx {
 
 // fine.hip
-void prefilter_real(float *c, int w, int h, hipStream_t s);
-void blur(const double *in, double *tmp, float *out, int w, int h, const double *taps, int r, hipStream_t s);
-void warp(const float *c2, const float *sx, const float *sy, float *out, int w, int h, hipStream_t s);
+// cubic B-spline coefficients of a real plane in place (tmp: w x h scratch)
+void prefilter_real(float *c, float *tmp, int w, int h, hipStream_t s);
+// taps_host: the 2r + 1 weights on the host (passed as a kernel argument)
+void blur(const double *in, double *tmp, float *out, int w, int h, const double *taps_host, int r, hipStream_t s);
+// out = I2((x + shx) + sx, (y + shy) + sy) from the spline c2 (w2 x h2) of image 2.
+void warp(const float *c2, int w2, int h2, long shx, long shy, const float *sx, const float *sy, float *out, int w,
+          int h, hipStream_t s);
 void refpack(const float2 *r, const float2 *rx, const float2 *ry, float4 *out, int w, int h, int wp, hipStream_t s);
 void accumulate(const double *ux, const double *uy, const float *sx, const float *sy, const float *csx,
                 const float *csy, const float2 *resp, const float4 *refp, double *sums, int w, int h, int F,
                 hipStream_t s);
 void moments_solve(const double *sums, double *ym, const double *ux_in, const double *uy_in, double *ux_out,
                    double *uy_out, double *energy, double *resid, double *slope, int w, int h, const double *kt,
-                   int r, hipStream_t s);
+                   const double *kt_host, int r, hipStream_t s);
 void energy_reset(const double *energy, const double *med, const double *ux_in, const double *uy_in, double *ux,
                   double *uy, double *resid, double *slope, size_t n, hipStream_t s);
 size_t median_scratch_bytes();
@@ -28,7 +32,8 @@ struct Counters {
     unsigned int n_low, n_ok;
 };
 // Exact 1-D nearest feature per column (squared distance, row), -1 where the column has none.
-void column_nearest(const uint8_t *feature, int w, int h, int *col_d2, int *col_row, hipStream_t s);
+void column_nearest(const uint8_t *feature, int w, int h, int *col_d2, int *col_row, int *summary, hipStream_t s);
+size_t column_summary_ints(int w, int h);
 // within[i] = (Euclidean distance to the nearest feature) <= thr, from column_nearest's output.
 void within_distance(const int *col_d2, int w, int h, double thr, uint8_t *within, hipStream_t s);
 // pre-warp (phaseflow.py:326-355): counts, trusted mask, fill of the untrusted pixels

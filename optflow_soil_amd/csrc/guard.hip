@@ -17,31 +17,55 @@ namespace {
 
 dim3 grid2(int w, int h) { return dim3((w + 255) / 256, h); }
 
-__global__ void k_column_nearest(const uint8_t *__restrict__ feat, int w, int h, int *__restrict__ col_d2,
-                                 int *__restrict__ col_row) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+// Column pass in chunks of CHUNK rows, one thread per (column, chunk): the first and last feature
+// row of every chunk, then every chunk finds its nearest feature above and below from those
+// summaries and sweeps its own rows (a tie between the two keeps the smaller row).
+constexpr int CHUNK = 32;
+
+__global__ void k_col_summary(const uint8_t *__restrict__ feat, int w, int h, int *__restrict__ first,
+                              int *__restrict__ last) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
     if (x >= w) return;
-    int last = -1;
-    for (int y = 0; y < h; y++) {
+    const int y0 = c * CHUNK, y1 = min(y0 + CHUNK, h);
+    int f = -1, l = -1;
+    for (int y = y0; y < y1; y++)
+        if (feat[(size_t)y * w + x]) {
+            if (f < 0) f = y;
+            l = y;
+        }
+    first[(size_t)c * w + x] = f;
+    last[(size_t)c * w + x] = l;
+}
+
+__global__ void k_column_nearest(const uint8_t *__restrict__ feat, const int *__restrict__ first,
+                                 const int *__restrict__ last, int w, int h, int *__restrict__ col_d2,
+                                 int *__restrict__ col_row) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+    if (x >= w) return;
+    const int nch = (h + CHUNK - 1) / CHUNK;
+    int up = -1, down = -1;
+    for (int cc = c - 1; cc >= 0 && up < 0; cc--) up = last[(size_t)cc * w + x];
+    for (int cc = c + 1; cc < nch && down < 0; cc++) down = first[(size_t)cc * w + x];
+    const int y0 = c * CHUNK, y1 = min(y0 + CHUNK, h);
+    for (int y = y0; y < y1; y++) {
         const size_t i = (size_t)y * w + x;
-        if (feat[i]) last = y;
-        col_row[i] = last;
+        if (feat[i]) up = y;
+        col_row[i] = up;
     }
-    int next = -1;
-    for (int y = h - 1; y >= 0; y--) {
+    for (int y = y1 - 1; y >= y0; y--) {
         const size_t i = (size_t)y * w + x;
-        if (feat[i]) next = y;
-        const int up = col_row[i];
-        const int du = up >= 0 ? y - up : 0x7fffffff, dd = next >= 0 ? next - y : 0x7fffffff;
-        if (up < 0 && next < 0) {
+        if (feat[i]) down = y;
+        const int u = col_row[i];
+        const int du = u >= 0 ? y - u : 0x7fffffff, dd = down >= 0 ? down - y : 0x7fffffff;
+        if (u < 0 && down < 0) {
             col_d2[i] = -1;
             col_row[i] = -1;
         } else if (du <= dd) {  // a tie keeps the smaller row
             col_d2[i] = du * du;
-            col_row[i] = up;
+            col_row[i] = u;
         } else {
             col_d2[i] = dd * dd;
-            col_row[i] = next;
+            col_row[i] = down;
         }
     }
 }
@@ -67,24 +91,30 @@ __global__ void k_zero_counters(Counters *cnt) {
     cnt->n_ok = 0;
 }
 
-__global__ void k_trusted(const uint8_t *__restrict__ low, const uint8_t *__restrict__ within_r, size_t n,
-                          uint8_t *__restrict__ ok, Counters *cnt) {
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_trusted(const uint8_t *__restrict__ low, const uint8_t *__restrict__ within_r,
+                                                 size_t n, uint8_t *__restrict__ ok, Counters *cnt) {
+    __shared__ unsigned int red[2][4];
     unsigned int nl = 0, no = 0;
-    if (i < n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const uint8_t o = !within_r[i];  // farther than R from every low pixel
         ok[i] = o;
-        nl = low[i] != 0;
-        no = o;
+        nl += low[i] != 0;
+        no += o;
     }
-    // wavefront sums, one atomic per wavefront
     for (int off = 32; off > 0; off >>= 1) {
         nl += __shfl_down(nl, off);
         no += __shfl_down(no, off);
     }
     if ((threadIdx.x & 63) == 0) {
-        if (nl) atomicAdd(&cnt->n_low, nl);
-        if (no) atomicAdd(&cnt->n_ok, no);
+        red[0][threadIdx.x >> 6] = nl;
+        red[1][threadIdx.x >> 6] = no;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const unsigned int a = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        const unsigned int b = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        if (a) atomicAdd(&cnt->n_low, a);
+        if (b) atomicAdd(&cnt->n_ok, b);
     }
 }
 
@@ -136,13 +166,23 @@ __global__ void k_prewarp_fill(const double *__restrict__ ux, const double *__re
 }
 
 // ---------------------------------------------------------------- components (union-find)
-__device__ int find_root(const int *parent, int x) {
-    const volatile int *p = parent;
+__device__ int find_root(int *parent, int x) {
+    volatile int *p = parent;
+    int r = x;
     while (true) {
+        const int q = p[r];
+        if (q == r) break;
+        r = q;
+    }
+    // path compression: every node on the path gets an ancestor (r, a root when read) as parent;
+    // roots are never written here, so the linking CAS of unite stays the only write to a root
+    while (x != r) {
         const int q = p[x];
-        if (q == x) return x;
+        if (q == x) break;
+        if (q != r) p[x] = r;
         x = q;
     }
+    return r;
 }
 
 __global__ void k_cc_init(const uint8_t *__restrict__ b, size_t n, int *__restrict__ parent,
@@ -177,14 +217,29 @@ __global__ void k_cc_union(const uint8_t *__restrict__ b, int w, int h, int *par
     if (y + 1 < h && b[i + w]) unite(parent, i, i + w);
 }
 
-__global__ void k_cc_count(const uint8_t *__restrict__ b, size_t n, int *parent, unsigned int *sizes) {
+__global__ void k_cc_flatten(const uint8_t *__restrict__ b, size_t n, int *parent) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !b[i]) return;
-    const int r = find_root(parent, (int)i);
-    atomicAdd(&sizes[r], 1u);
+    find_root(parent, (int)i);
 }
 
-__global__ void k_cc_keep(const uint8_t *__restrict__ b, size_t n, const int *parent,
+// Component sizes; lanes of a wavefront that share a root add their count with one atomic (the
+// border strips and large holes are single components of many thousand pixels).
+__global__ void k_cc_count(const uint8_t *__restrict__ b, size_t n, int *parent, unsigned int *sizes) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = (i < n && b[i]) ? find_root(parent, (int)i) : -1;
+    const int lane = threadIdx.x & 63;
+    unsigned long long active = __ballot(r >= 0);
+    while (active) {
+        const int leader = __ffsll((long long)active) - 1;
+        const int rl = __shfl(r, leader);
+        const unsigned long long same = __ballot(r == rl);
+        if (lane == leader) atomicAdd(&sizes[rl], (unsigned int)__popcll(same));
+        active &= ~same;
+    }
+}
+
+__global__ void k_cc_keep(const uint8_t *__restrict__ b, size_t n, int *parent,
                           const unsigned int *__restrict__ sizes, double min_area, uint8_t *__restrict__ out) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
@@ -380,10 +435,16 @@ size_t nblk(size_t n) { return (n + 255) / 256; }
 
 }  // namespace
 
-void column_nearest(const uint8_t *feature, int w, int h, int *col_d2, int *col_row, hipStream_t s) {
-    hipLaunchKernelGGL(k_column_nearest, dim3((w + 63) / 64), dim3(64), 0, s, feature, w, h, col_d2, col_row);
+void column_nearest(const uint8_t *feature, int w, int h, int *col_d2, int *col_row, int *summary, hipStream_t s) {
+    const int nch = (h + CHUNK - 1) / CHUNK;
+    int *first = summary, *last = summary + (size_t)nch * w;
+    hipLaunchKernelGGL(k_col_summary, dim3((w + 255) / 256, nch), dim3(256), 0, s, feature, w, h, first, last);
+    hipLaunchKernelGGL(k_column_nearest, dim3((w + 255) / 256, nch), dim3(256), 0, s, feature, (const int *)first,
+                       (const int *)last, w, h, col_d2, col_row);
     LAUNCHCHECK();
 }
+
+size_t column_summary_ints(int w, int h) { return 2 * (size_t)((h + CHUNK - 1) / CHUNK) * w; }
 
 void within_distance(const int *col_d2, int w, int h, double thr, uint8_t *within, hipStream_t s) {
     const int T = thr >= 0 ? (int)std::floor(thr) : -1;
@@ -395,7 +456,8 @@ void prewarp_trusted(const uint8_t *low, const uint8_t *within_r, int w, int h, 
                      hipStream_t s) {
     const size_t n = (size_t)w * h;
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(1), 0, s, cnt);
-    hipLaunchKernelGGL(k_trusted, dim3(nblk(n)), dim3(256), 0, s, low, within_r, n, ok, cnt);
+    hipLaunchKernelGGL(k_trusted, dim3((unsigned)std::min<size_t>(1024, nblk(n))), dim3(256), 0, s, low, within_r, n, ok,
+                       cnt);
     LAUNCHCHECK();
 }
 
@@ -411,6 +473,7 @@ void large_components(const uint8_t *b, int w, int h, double min_area, int *pare
     const size_t n = (size_t)w * h;
     hipLaunchKernelGGL(k_cc_init, dim3(nblk(n)), dim3(256), 0, s, b, n, parent, sizes);
     hipLaunchKernelGGL(k_cc_union, grid2(w, h), dim3(256), 0, s, b, w, h, parent);
+    hipLaunchKernelGGL(k_cc_flatten, dim3(nblk(n)), dim3(256), 0, s, b, n, parent);
     hipLaunchKernelGGL(k_cc_count, dim3(nblk(n)), dim3(256), 0, s, b, n, parent, sizes);
     hipLaunchKernelGGL(k_cc_keep, dim3(nblk(n)), dim3(256), 0, s, b, n, parent, sizes, min_area, out);
     LAUNCHCHECK();

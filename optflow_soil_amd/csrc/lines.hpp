@@ -15,6 +15,19 @@ __device__ __forceinline__ float2 cconj(float2 a) { return make_float2(a.x, -a.y
 __device__ __forceinline__ float2 mul_minus_i(float2 a) { return make_float2(a.y, -a.x); }  // -i a
 __device__ __forceinline__ float2 mul_plus_i(float2 a) { return make_float2(-a.y, a.x); }   // +i a
 
+// ---------------------------------------------------------------- LDS line layout
+// With SH > 0, element e of an LDS line buffer lives at e + (e >> SH): one spare slot every 2^SH
+// elements, so that passes writing at strides of 2..16 elements spread over the LDS banks.
+template <int SH>
+__device__ __forceinline__ int lp(int e) {
+    if constexpr (SH > 0) return e + (e >> SH);
+    else return e;
+}
+template <int SH>
+__host__ __device__ constexpr int lp_len(int n) {
+    return SH > 0 ? n + (n >> SH) + 1 : n;
+}
+
 // ---------------------------------------------------------------- forward DFT kernels of small radix
 // y_q = sum_t x_t exp(-2 pi i q t / R)
 __device__ __forceinline__ void dft2(float2 *x) {
@@ -71,6 +84,40 @@ __device__ __forceinline__ void dft8(float2 *x) {
     }
 }
 
+// y_q = sum_t x_t W_R^{q t} in registers for an odd prime R, with W_R^m = w[m] (m < R) taken from
+// the line's twiddle table. The pairs t, R - t share cos and flip sin: with a_t = x_t + x_{R-t} and
+// b_t = x_t - x_{R-t} (t = 1 .. (R-1)/2), P_q = x_0 + sum_t a_t cos(2 pi q t / R) and
+// Q_q = sum_t b_t sin(2 pi q t / R) give y_q = P_q - i Q_q and y_{R-q} = P_q + i Q_q: (R-1)^2 real
+// multiply-adds instead of the (R-1)^2 complex ones of the direct sum.
+template <int R>
+__device__ __forceinline__ void dft_prime(float2 *x, const float2 *w) {
+    constexpr int H = (R - 1) / 2;
+    float2 a[H + 1], b[H + 1];
+#pragma unroll
+    for (int t = 1; t <= H; t++) {
+        a[t] = cadd(x[t], x[R - t]);
+        b[t] = csub(x[t], x[R - t]);
+    }
+    float2 y0 = x[0];
+#pragma unroll
+    for (int t = 1; t <= H; t++) y0 = cadd(y0, a[t]);
+#pragma unroll
+    for (int q = 1; q <= H; q++) {
+        float px = x[0].x, py = x[0].y, qx = 0.f, qy = 0.f;
+#pragma unroll
+        for (int t = 1; t <= H; t++) {
+            const float2 c = w[(q * t) % R];  // (cos, -sin) of 2 pi q t / R
+            px = fmaf(a[t].x, c.x, px);
+            py = fmaf(a[t].y, c.x, py);
+            qx = fmaf(b[t].x, -c.y, qx);
+            qy = fmaf(b[t].y, -c.y, qy);
+        }
+        x[q] = make_float2(px + qy, py - qx);
+        x[R - q] = make_float2(px - qy, py + qx);
+    }
+    x[0] = y0;
+}
+
 template <int R>
 __device__ __forceinline__ void dft_small(float2 *x) {
     if constexpr (R == 2) dft2(x);
@@ -84,50 +131,222 @@ __device__ __forceinline__ void dft_small(float2 *x) {
 // lines of length n stored line-major at src / dst (line stride n). Butterfly i of a line reads
 // src[i + q m] (m = n / R), applies W_{pR}^{q k} (k = i mod p), transforms, and writes
 // dst[(i / p) p R + k + q p].
-template <int R>
+template <int R, int SH = 0>
 __device__ __forceinline__ void stockham_small(const float2 *src, float2 *dst, int n, int lines, int p,
                                                const float2 *__restrict__ tw, int tid, int nt) {
     const int m = n / R;
     const int step = n / (p * R);
     for (int item = tid; item < m * lines; item += nt) {
         const int line = item / m, i = item - line * m;
-        const float2 *s = src + line * n;
-        float2 *d = dst + line * n;
+        const int base = line * n;
         const int k = i % p;
         float2 x[R];
 #pragma unroll
-        for (int q = 0; q < R; q++) x[q] = s[i + q * m];
+        for (int q = 0; q < R; q++) x[q] = src[lp<SH>(base + i + q * m)];
 #pragma unroll
         for (int q = 1; q < R; q++) x[q] = cmul(x[q], tw[q * k * step]);
         dft_small<R>(x);
         const int j = (i / p) * p * R + k;
 #pragma unroll
-        for (int q = 0; q < R; q++) d[j + q * p] = x[q];
+        for (int q = 0; q < R; q++) dst[lp<SH>(base + j + q * p)] = x[q];
+    }
+}
+
+// Stockham pass for a prime radix R in {7, 11, 13} with the butterfly in registers.
+template <int R, int SH = 0>
+__device__ __forceinline__ void stockham_prime(const float2 *src, float2 *dst, int n, int lines, int p,
+                                               const float2 *__restrict__ tw, int tid, int nt) {
+    const int m = n / R;
+    const int step = n / (p * R);
+    float2 wr[R];
+#pragma unroll
+    for (int q = 0; q < R; q++) wr[q] = tw[q * (n / R)];
+    for (int item = tid; item < m * lines; item += nt) {
+        const int line = item / m, i = item - line * m;
+        const int base = line * n;
+        const int k = i % p;
+        float2 x[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) x[q] = src[lp<SH>(base + i + q * m)];
+#pragma unroll
+        for (int q = 1; q < R; q++) x[q] = cmul(x[q], tw[q * k * step]);
+        dft_prime<R>(x, wr);
+        const int j = (i / p) * p * R + k;
+#pragma unroll
+        for (int q = 0; q < R; q++) dst[lp<SH>(base + j + q * p)] = x[q];
     }
 }
 
 // The same pass for any radix R, one output per thread: output o = sum_t x_t W_{pR}^{t (o mod pR)}
 // with x_t = src[i + t m], i = (o / pR) p + o mod p.
+template <int SH = 0>
 __device__ __forceinline__ void stockham_generic(const float2 *src, float2 *dst, int n, int lines, int p, int R,
                                                  const float2 *__restrict__ tw, int tid, int nt) {
     const int m = n / R, pR = p * R, step = n / pR;
     for (int item = tid; item < n * lines; item += nt) {
         const int line = item / n, o = item - line * n;
-        const float2 *s = src + line * n;
+        const int base = line * n;
         const int r = o % pR;
         const int i = (o / pR) * p + (o % p);
         float ax = 0.f, ay = 0.f;
         int e = 0;  // t * r mod pR
         for (int t = 0; t < R; t++) {
             float2 w = tw[e * step];
-            float2 v = s[i + t * m];
+            float2 v = src[lp<SH>(base + i + t * m)];
             ax += v.x * w.x - v.y * w.y;
             ay += v.x * w.y + v.y * w.x;
             e += r;
             if (e >= pR) e -= pR;
         }
-        dst[line * n + o] = make_float2(ax, ay);
+        dst[lp<SH>(base + o)] = make_float2(ax, ay);
     }
+}
+
+// i mod p for i < 2^16, p < 2^16, from magic = floor(2^32 / p) + 1 (0 for p = 1): the quotient
+// umulhi(i, magic) is exact because i (magic p - 2^32) < 2^32.
+__host__ __device__ inline unsigned int mod_magic(unsigned int p) {
+    return p <= 1 ? 0u : (unsigned int)((0x100000000ull / p) + 1);
+}
+__device__ __forceinline__ int mod_by(int i, int p, unsigned int magic) {
+    return p == 1 ? 0 : i - (int)__umulhi((unsigned int)i, magic) * p;
+}
+
+// One Stockham pass of radix R over a single line in LDS (p = product of the radices already
+// applied, magic = mod_magic(p)). Butterfly i reads src[i + q m] (m = n / R), applies
+// W_{pR}^{q k} (k = i mod p), transforms, and writes dst[i R - k (R - 1) + q p]
+// (= (i / p) p R + k + q p).
+template <int R, int SH>
+__device__ __forceinline__ void stockham_line(const float2 *src, float2 *dst, int n, int p, unsigned int magic,
+                                              const float2 *tw, int tid, int nt) {
+    const int m = n / R;
+    const int step = n / (p * R);
+    float2 wr[R];
+    if constexpr (R == 7 || R == 11 || R == 13) {
+#pragma unroll
+        for (int q = 0; q < R; q++) wr[q] = tw[q * m];
+    }
+    for (int i = tid; i < m; i += nt) {
+        const int k = mod_by(i, p, magic);
+        float2 x[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) x[q] = src[lp<SH>(i + q * m)];
+        const int ks = k * step;
+#pragma unroll
+        for (int q = 1; q < R; q++) x[q] = cmul(x[q], tw[q * ks]);
+        if constexpr (R == 7 || R == 11 || R == 13) dft_prime<R>(x, wr);
+        else dft_small<R>(x);
+        const int j = i * R - k * (R - 1);
+#pragma unroll
+        for (int q = 0; q < R; q++) dst[lp<SH>(j + q * p)] = x[q];
+    }
+}
+
+// First Stockham pass (p = 1, no twiddles) of one line whose element e is load(e), written to dst.
+template <int R, int SH, class L>
+__device__ __forceinline__ void first_pass_small(L load, float2 *dst, int n, const float2 *tw, int tid, int nt) {
+    const int m = n / R;
+    float2 wr[R];
+    if constexpr (R == 7 || R == 11 || R == 13) {
+#pragma unroll
+        for (int q = 0; q < R; q++) wr[q] = tw[q * m];
+    }
+    for (int i = tid; i < m; i += nt) {
+        float2 x[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) x[q] = load(i + q * m);
+        if constexpr (R == 7 || R == 11 || R == 13) dft_prime<R>(x, wr);
+        else dft_small<R>(x);
+#pragma unroll
+        for (int q = 0; q < R; q++) dst[lp<SH>(i * R + q)] = x[q];
+    }
+}
+
+// Forward DFT of one line of length n whose element e is load(e): the first pass reads straight from
+// the loader (global memory) into registers, the others run in LDS buffers a / b with the twiddle
+// table tw (LDS). magic[s] = mod_magic(fac[0] ... fac[s-1]). Returns the buffer holding the
+// result; ends with a barrier. Radices without a register butterfly load the line into a first.
+template <int SH, class L>
+__device__ __forceinline__ float2 *fft_line_from(L load, float2 *a, float2 *b, int n, const int *fac,
+                                                 const unsigned int *magic, int nfac, const float2 *tw, int tid,
+                                                 int nt) {
+    const int R0 = nfac > 0 ? fac[0] : 1;
+    int s0 = 1;
+    switch (R0) {
+        case 2: first_pass_small<2, SH>(load, b, n, tw, tid, nt); break;
+        case 3: first_pass_small<3, SH>(load, b, n, tw, tid, nt); break;
+        case 4: first_pass_small<4, SH>(load, b, n, tw, tid, nt); break;
+        case 5: first_pass_small<5, SH>(load, b, n, tw, tid, nt); break;
+        case 7: first_pass_small<7, SH>(load, b, n, tw, tid, nt); break;
+        case 8: first_pass_small<8, SH>(load, b, n, tw, tid, nt); break;
+        case 11: first_pass_small<11, SH>(load, b, n, tw, tid, nt); break;
+        case 13: first_pass_small<13, SH>(load, b, n, tw, tid, nt); break;
+        default:
+            for (int e = tid; e < n; e += nt) a[lp<SH>(e)] = load(e);
+            s0 = 0;
+            break;
+    }
+    __syncthreads();
+    if (s0 == 1) {
+        float2 *t = a;
+        a = b;
+        b = t;
+    }
+    int p = s0 == 1 ? R0 : 1;
+    for (int s = s0; s < nfac; s++) {
+        const int R = fac[s];
+        const unsigned int mg = magic[s];
+        switch (R) {
+            case 2: stockham_line<2, SH>(a, b, n, p, mg, tw, tid, nt); break;
+            case 3: stockham_line<3, SH>(a, b, n, p, mg, tw, tid, nt); break;
+            case 4: stockham_line<4, SH>(a, b, n, p, mg, tw, tid, nt); break;
+            case 5: stockham_line<5, SH>(a, b, n, p, mg, tw, tid, nt); break;
+            case 8: stockham_line<8, SH>(a, b, n, p, mg, tw, tid, nt); break;
+            case 7: stockham_line<7, SH>(a, b, n, p, mg, tw, tid, nt); break;
+            case 11: stockham_line<11, SH>(a, b, n, p, mg, tw, tid, nt); break;
+            case 13: stockham_line<13, SH>(a, b, n, p, mg, tw, tid, nt); break;
+            default: stockham_generic<SH>(a, b, n, 1, p, R, tw, tid, nt); break;
+        }
+        __syncthreads();
+        float2 *t = a;
+        a = b;
+        b = t;
+        p *= R;
+    }
+    return a;
+}
+
+// Stockham pass with compile-time radix R, line length N and p = P over `lines` lines (line stride N):
+// every index operation is a shift or a mask when N and P are powers of two.
+template <int R, int N, int P>
+__device__ __forceinline__ void stockham_ct(const float2 *src, float2 *dst, int lines, const float2 *tw, int tid,
+                                            int nt) {
+    constexpr int M = N / R, STEP = N / (P * R);
+    for (int item = tid; item < M * lines; item += nt) {
+        const int line = item / M, i = item - line * M;
+        const int base = line * N;
+        const int k = i % P;
+        float2 x[R];
+#pragma unroll
+        for (int q = 0; q < R; q++) x[q] = src[base + i + q * M];
+        if constexpr (P > 1) {
+#pragma unroll
+            for (int q = 1; q < R; q++) x[q] = cmul(x[q], tw[q * k * STEP]);
+        }
+        dft_small<R>(x);
+        const int j = (i / P) * P * R + k;
+#pragma unroll
+        for (int q = 0; q < R; q++) dst[base + j + q * P] = x[q];
+    }
+}
+
+// Forward DFT of `lines` lines of length 64 (radix 8, 8) in LDS buffer a; b is scratch. Returns
+// the buffer holding the result (a). Ends with a barrier.
+__device__ __forceinline__ float2 *fft64_lines_lds(float2 *a, float2 *b, int lines, const float2 *tw, int tid, int nt) {
+    stockham_ct<8, 64, 1>(a, b, lines, tw, tid, nt);
+    __syncthreads();
+    stockham_ct<8, 64, 8>(b, a, lines, tw, tid, nt);
+    __syncthreads();
+    return a;
 }
 
 // Forward DFT of `lines` lines of length n in LDS buffer a (line-major); b is scratch of the same
@@ -143,6 +362,9 @@ __device__ __forceinline__ float2 *fft_lines_lds(float2 *a, float2 *b, int n, in
             case 4: stockham_small<4>(a, b, n, lines, p, tw, tid, nt); break;
             case 5: stockham_small<5>(a, b, n, lines, p, tw, tid, nt); break;
             case 8: stockham_small<8>(a, b, n, lines, p, tw, tid, nt); break;
+            case 7: stockham_prime<7>(a, b, n, lines, p, tw, tid, nt); break;
+            case 11: stockham_prime<11>(a, b, n, lines, p, tw, tid, nt); break;
+            case 13: stockham_prime<13>(a, b, n, lines, p, tw, tid, nt); break;
             default: stockham_generic(a, b, n, lines, p, R, tw, tid, nt); break;
         }
         __syncthreads();
@@ -158,10 +380,13 @@ __device__ __forceinline__ float2 *fft_lines_lds(float2 *a, float2 *b, int n, in
 // Coefficients of python/optflow_ref/interp.py:12-26 (x 6, causal start over a 21-sample horizon,
 // causal and anti-causal recursions with z = sqrt(3) - 2, mirror end), evaluated with the line
 // split into segments of SEG samples: each segment starts its recursion WARM samples early from
-// zero, which leaves an error of z^WARM (< 2e-14) relative to the sequential recursion, far
-// below float32 storage. Arithmetic is double; the result is multiplied by `scale`.
-constexpr int PF_SEG = 32;
-constexpr int PF_WARM = 24;
+// zero, which leaves an error of z^WARM (7e-10) relative to the sequential recursion, far below
+// float32 storage (6e-8). Arithmetic is double; the result is multiplied by `scale`.
+// 17 = 1 mod 16: the lanes of a wavefront walking neighbouring segments of an LDS line of 8-byte
+// elements start 136 bytes apart, i.e. on different banks (a multiple of 16 would put every lane on
+// the same bank).
+constexpr int PF_SEG = 17;
+constexpr int PF_WARM = 16;
 constexpr int PF_HORIZON = 21;  // ceil(log(1e-12) / log|z|)
 
 template <class T>
@@ -189,13 +414,27 @@ struct Dbl<float2> {
     __device__ float2 out(double s) const { return make_float2((float)(x * s), (float)(y * s)); }
 };
 
+// Single-precision accumulator with the interface of Dbl: the recursions are contractive (|z| < 1),
+// so float arithmetic stays within about two ulp of the exact coefficients.
+struct Sgl2 {
+    float x, y;
+    __device__ Sgl2() : x(0.f), y(0.f) {}
+    __device__ explicit Sgl2(float2 a) : x(a.x), y(a.y) {}
+    __device__ Sgl2(float a, float b) : x(a), y(b) {}
+    __device__ Sgl2 operator+(const Sgl2 &o) const { return Sgl2(x + o.x, y + o.y); }
+    __device__ Sgl2 operator-(const Sgl2 &o) const { return Sgl2(x - o.x, y - o.y); }
+    __device__ Sgl2 operator*(double s) const { return Sgl2(x * (float)s, y * (float)s); }
+    __device__ float2 out(double s) const { return make_float2(x * (float)s, y * (float)s); }
+};
+
 // `lines` lines of length n, line-major with stride n. src: input; mid: scratch (distinct from
 // src); dst: output, which may alias src (src is only read before the barrier between passes).
-template <class T>
+// A: the accumulator (Dbl<T>: double arithmetic; Sgl2: float).
+template <class T, int SH = 0, class A = Dbl<T>>
 __device__ void prefilter_lines(const T *src, T *mid, T *dst, int n, int lines, double scale, int tid, int nt) {
     const double z = -0.26794919243112270647;  // sqrt(3) - 2
     if (n == 1) {
-        for (int l = tid; l < lines; l += nt) dst[l] = Dbl<T>(src[l]).out(scale);
+        for (int l = tid; l < lines; l += nt) dst[lp<SH>(l)] = A(src[lp<SH>(l)]).out(scale);
         __syncthreads();
         return;
     }
@@ -203,27 +442,30 @@ __device__ void prefilter_lines(const T *src, T *mid, T *dst, int n, int lines, 
     // causal pass: mid[i] = 6 src[i] + z mid[i-1], mid[0] = sum_{k<h} z^k 6 src[k]
     for (int item = tid; item < nseg * lines; item += nt) {
         const int line = item / nseg, j = item - line * nseg;
-        const T *sl = src + (size_t)line * n;
-        T *ml = mid + (size_t)line * n;
+        const int base = line * n;
+        auto sl = [&](int i) { return src[lp<SH>(base + i)]; };
+        auto mset = [&](int i, const A &v) { mid[lp<SH>(base + i)] = v.out(1.0); };
         const int a = j * PF_SEG, b = min(a + PF_SEG, n);
-        Dbl<T> acc;
+        A acc;
         int i;
         if (a <= PF_WARM) {
             const int h = min(n, PF_HORIZON);
             double zk = 1.0;
             for (int k = 0; k < h; k++) {
-                acc = acc + Dbl<T>(sl[k]) * (6.0 * zk);
+                acc = acc + A(sl(k)) * (6.0 * zk);
                 zk *= z;
             }
-            if (a == 0) ml[0] = acc.out(1.0);
+            if (a == 0) mset(0, acc);
             i = 1;
         } else {
             i = a - PF_WARM;
         }
-        for (; i < a; i++) acc = Dbl<T>(sl[i]) * 6.0 + acc * z;
+#pragma unroll 4
+        for (; i < a; i++) acc = A(sl(i)) * 6.0 + acc * z;
+#pragma unroll 4
         for (i = max(a, 1); i < b; i++) {
-            acc = Dbl<T>(sl[i]) * 6.0 + acc * z;
-            ml[i] = acc.out(1.0);
+            acc = A(sl(i)) * 6.0 + acc * z;
+            mset(i, acc);
         }
     }
     __syncthreads();
@@ -231,22 +473,25 @@ __device__ void prefilter_lines(const T *src, T *mid, T *dst, int n, int lines, 
     const double zend = z / (z * z - 1.0);
     for (int item = tid; item < nseg * lines; item += nt) {
         const int line = item / nseg, j = item - line * nseg;
-        const T *ml = mid + (size_t)line * n;
-        T *dl = dst + (size_t)line * n;
+        const int base = line * n;
+        auto ml = [&](int i) { return mid[lp<SH>(base + i)]; };
+        auto dset = [&](int i, const A &v) { dst[lp<SH>(base + i)] = v.out(scale); };
         const int a = j * PF_SEG, b = min(a + PF_SEG, n);
-        Dbl<T> acc;
+        A acc;
         int i;
         if (b + PF_WARM >= n) {
-            acc = (Dbl<T>(ml[n - 1]) + Dbl<T>(ml[n - 2]) * z) * zend;
-            if (b == n) dl[n - 1] = acc.out(scale);
+            acc = (A(ml(n - 1)) + A(ml(n - 2)) * z) * zend;
+            if (b == n) dset(n - 1, acc);
             i = n - 2;
         } else {
             i = b + PF_WARM - 1;
         }
-        for (; i >= b; i--) acc = (acc - Dbl<T>(ml[i])) * z;
+#pragma unroll 4
+        for (; i >= b; i--) acc = (acc - A(ml(i))) * z;
+#pragma unroll 4
         for (i = min(b - 1, n - 2); i >= a; i--) {
-            acc = (acc - Dbl<T>(ml[i])) * z;
-            dl[i] = acc.out(scale);
+            acc = (acc - A(ml(i))) * z;
+            dset(i, acc);
         }
     }
     __syncthreads();

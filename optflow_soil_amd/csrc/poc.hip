@@ -5,6 +5,7 @@
 // passes run in one workgroup over the whole grid.
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "fine.hpp"
 #include "lines.hpp"
@@ -20,6 +21,7 @@ struct CoarseWs {
     double *tmp = nullptr;         // blur scratch
     double *gux = nullptr, *guy = nullptr, *gpk = nullptr, *ix = nullptr, *iy = nullptr;
     int *gflag = nullptr;          // per point: valid, out flags
+    float *hann = nullptr;         // Hann tables of the current level (wy + wx floats)
     size_t gcap = 0;
     size_t tmpcap = 0;
 };
@@ -107,7 +109,7 @@ struct PocArgs {
 };
 
 // 2-D forward DFT of X (wy x wx, row-major) in LDS; result in column-major order ([kx][ky]) in
-// the returned buffer, Y and Z are scratch.
+// the returned buffer, Y is scratch.
 __device__ float2 *fft2_lds(float2 *X, float2 *Y, int wy, int wx, const PocArgs &P, const float2 *twx,
                             const float2 *twy) {
     float2 *r = fft_lines_lds(X, Y, wx, wy, P.facx, P.nfx, twx, threadIdx.x, blockDim.x);
@@ -118,6 +120,63 @@ __device__ float2 *fft2_lds(float2 *X, float2 *Y, int wy, int wx, const PocArgs 
     }
     __syncthreads();
     return fft_lines_lds(o, r, wy, wx, P.facy, P.nfy, twy, threadIdx.x, blockDim.x);
+}
+
+// ---------------------------------------------------------------- 64 x 64 windows
+// Element (y, x) of a 64 x 64 LDS tile lives at y * 65 + x: rows handed to consecutive lanes
+// (row pass) and columns handed to consecutive lanes (column pass) both spread over the banks.
+constexpr int T64 = 65;
+
+// Radix-8 Stockham pass (p = P) along the rows: butterfly item -> row item % 64, index item / 64.
+template <int P>
+__device__ __forceinline__ void pass64_rows(const float2 *src, float2 *dst, const float2 *tw, int tid, int nt) {
+    for (int item = tid; item < 512; item += nt) {
+        const int row = item & 63, i = item >> 6;
+        const int k = i % P;
+        float2 x[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) x[q] = src[row * T64 + i + 8 * q];
+        if constexpr (P > 1) {
+#pragma unroll
+            for (int q = 1; q < 8; q++) x[q] = cmul(x[q], tw[q * k]);
+        }
+        dft8(x);
+        const int j = (i / P) * P * 8 + k;
+#pragma unroll
+        for (int q = 0; q < 8; q++) dst[row * T64 + j + q * P] = x[q];
+    }
+}
+
+// The same along the columns: butterfly item -> column item % 64, index item / 64.
+template <int P>
+__device__ __forceinline__ void pass64_cols(const float2 *src, float2 *dst, const float2 *tw, int tid, int nt) {
+    for (int item = tid; item < 512; item += nt) {
+        const int col = item & 63, i = item >> 6;
+        const int k = i % P;
+        float2 x[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) x[q] = src[(i + 8 * q) * T64 + col];
+        if constexpr (P > 1) {
+#pragma unroll
+            for (int q = 1; q < 8; q++) x[q] = cmul(x[q], tw[q * k]);
+        }
+        dft8(x);
+        const int j = (i / P) * P * 8 + k;
+#pragma unroll
+        for (int q = 0; q < 8; q++) dst[(j + q * P) * T64 + col] = x[q];
+    }
+}
+
+// Forward 2-D DFT of the 64 x 64 tile X (natural layout in and out, result in X; Y scratch).
+__device__ __forceinline__ void fft64x64(float2 *X, float2 *Y, const float2 *tw) {
+    pass64_rows<1>(X, Y, tw, threadIdx.x, blockDim.x);
+    __syncthreads();
+    pass64_rows<8>(Y, X, tw, threadIdx.x, blockDim.x);
+    __syncthreads();
+    pass64_cols<1>(X, Y, tw, threadIdx.x, blockDim.x);
+    __syncthreads();
+    pass64_cols<8>(Y, X, tw, threadIdx.x, blockDim.x);
+    __syncthreads();
 }
 
 __device__ double block_sum(double v, double *red) {
@@ -132,10 +191,19 @@ __device__ double block_sum(double v, double *red) {
     return s;
 }
 
-__device__ float hannf(int i, int n) { return (float)(0.5 - 0.5 * cos(6.283185307179586 * (double)i / (double)n)); }
+// Hann tables of a level: hann[0 .. wy) then hann[wy .. wy + wx), 0.5 - 0.5 cos(2 pi i / n) as float
+// (poc.py:45-46).
+__global__ void k_hann(float *hann, int wy, int wx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= wy + wx) return;
+    const int n = i < wy ? wy : wx, e = i < wy ? i : i - wy;
+    hann[i] = (float)(0.5 - 0.5 * cos(6.283185307179586 * (double)e / (double)n));
+}
 
-// Loads the mean-removed, Hann-windowed window at (y0, x0) of img into X (complex).
-__device__ void load_window(const float *img, int W, int y0, int x0, int wy, int wx, float2 *X, double *red) {
+// Loads the mean-removed, Hann-windowed window at (y0, x0) of img into X (complex); element (y, x)
+// goes to X[y * pitch + x].
+__device__ void load_window(const float *img, int W, int y0, int x0, int wy, int wx, int pitch, const float *hy,
+                            const float *hx, float2 *X, double *red) {
     double s = 0;
     for (int i = threadIdx.x; i < wy * wx; i += blockDim.x) {
         const int y = i / wx, x = i - y * wx;
@@ -144,15 +212,16 @@ __device__ void load_window(const float *img, int W, int y0, int x0, int wy, int
     const double mean = block_sum(s, red) / (double)(wy * wx);
     for (int i = threadIdx.x; i < wy * wx; i += blockDim.x) {
         const int y = i / wx, x = i - y * wx;
-        const double v = ((double)img[(size_t)(y0 + y) * W + x0 + x] - mean) * (double)hannf(y, wy) * (double)hannf(x, wx);
-        X[i] = make_float2((float)v, 0.f);
+        const double v = ((double)img[(size_t)(y0 + y) * W + x0 + x] - mean) * (double)hy[y] * (double)hx[x];
+        X[y * pitch + x] = make_float2((float)v, 0.f);
     }
     __syncthreads();
 }
 
-__device__ bool lp_pass(int i, int n) {
+// |fftfreq(n)[i]| <= 0.25, i.e. 4 |k| <= n (exact: k / n rounds to 0.25 only when 4 |k| = n)
+__device__ __forceinline__ bool lp_pass(int i, int n) {
     const int k = (i <= (n - 1) / 2) ? i : i - n;
-    return fabs((double)k / n) <= 0.25;
+    return 4 * abs(k) <= n;
 }
 
 __device__ double parabola(double m, double c, double p) {
@@ -160,88 +229,147 @@ __device__ double parabola(double m, double c, double p) {
     return den == 0 ? 0.0 : 0.5 * (m - p) / den;
 }
 
-// poc.py:67-92 with poc_shift (poc.py:39-54) inside
-__global__ void __launch_bounds__(256) k_poc_grid(const float *__restrict__ I1, const float *__restrict__ I2, PocArgs P,
-                                                  const float2 *__restrict__ twx, const float2 *__restrict__ twy,
-                                                  const double *__restrict__ ix, const double *__restrict__ iy,
-                                                  double *__restrict__ gux, double *__restrict__ guy,
-                                                  double *__restrict__ gpk) {
+constexpr int POC_NT = 512;               // threads of a POC workgroup
+constexpr int POC_PER = POC_MAX / POC_NT;  // spectrum elements a thread keeps
+
+// poc.py:67-92 with poc_shift (poc.py:39-54) inside. WIN = 64: 64 x 64 windows with the tiles in
+// natural layout and every index a shift; WIN = 0: any window of at most POC_MAX samples, spectra
+// transposed ([kx][ky]).
+template <int WIN>
+__global__ void __launch_bounds__(POC_NT) k_poc_grid(const float *__restrict__ I1, const float *__restrict__ I2,
+                                                     PocArgs P, const float2 *__restrict__ twx_g,
+                                                     const float2 *__restrict__ twy_g, const float *__restrict__ hann_g,
+                                                     const double *__restrict__ ix, const double *__restrict__ iy,
+                                                     double *__restrict__ gux, double *__restrict__ guy,
+                                                     double *__restrict__ gpk) {
     extern __shared__ float2 lds[];
-    const int n = P.wy * P.wx;
-    float2 *X = lds, *Y = lds + n;
+    const int wy = WIN ? WIN : P.wy, wx = WIN ? WIN : P.wx;
+    const int n = wy * wx;
+    const int tile = WIN ? WIN * T64 : n;  // LDS elements of one buffer
+    float2 *X = lds, *Y = lds + tile;
+    float2 *twx = Y + tile, *twy = WIN ? twx : twx + wx;
+    float *hy = (float *)(twy + wy), *hx = hy + wy;
+    for (int i = threadIdx.x; i < wx; i += blockDim.x) twx[i] = twx_g[i];
+    if (!WIN)
+        for (int i = threadIdx.x; i < wy; i += blockDim.x) twy[i] = twy_g[i];
+    for (int i = threadIdx.x; i < wy + wx; i += blockDim.x) hy[i] = hann_g[i];
     double *red = (double *)Y;  // block sums run while Y is free
-    float2 A[POC_MAX / 256];  // spectrum of the first window, element tid + 256 q
+    float2 A[POC_PER];          // spectrum of the first window, element tid + POC_NT q
     const int pt = blockIdx.x;
     const int gi = pt / P.nx, gj = pt - gi * P.nx;
     const int y = P.y0 + gi * P.step, x = P.x0 + gj * P.step;
-    const int hy = P.wy / 2, hx = P.wx / 2;
+    const int hy2 = wy / 2, hx2 = wx / 2;
     double dx = P.have_init ? ix[pt] : 0.0, dy = P.have_init ? iy[pt] : 0.0;
+    __syncthreads();
     // spectrum of the first window, kept in registers for every iteration
-    load_window(I1, P.W, y - hy, x - hx, P.wy, P.wx, X, red);
+    load_window(I1, P.W, y - hy2, x - hx2, wy, wx, WIN ? T64 : wx, hy, hx, X, red);
     {
-        float2 *r = fft2_lds(X, Y, P.wy, P.wx, P, twx, twy);
+        float2 *r;
+        if constexpr (WIN == 64) {
+            fft64x64(X, Y, twx);
+            r = X;
+        } else {
+            r = fft2_lds(X, Y, wy, wx, P, twx, twy);
+        }
 #pragma unroll
-        for (int q = 0; q < POC_MAX / 256; q++) {
-            const int i = threadIdx.x + 256 * q;
-            A[q] = i < n ? r[i] : make_float2(0.f, 0.f);
+        for (int q = 0; q < POC_PER; q++) {
+            const int i = threadIdx.x + POC_NT * q;
+            if constexpr (WIN == 64) A[q] = r[(i >> 6) * T64 + (i & 63)];
+            else A[q] = i < n ? r[i] : make_float2(0.f, 0.f);
         }
         __syncthreads();
     }
     int nlp = 0;  // low-pass mask size (lp.mean() = nlp / n)
     {
         int cy = 0, cx = 0;
-        for (int i = 0; i < P.wy; i++) cy += lp_pass(i, P.wy);
-        for (int i = 0; i < P.wx; i++) cx += lp_pass(i, P.wx);
+        for (int i = 0; i < wy; i++) cy += lp_pass(i, wy);
+        for (int i = 0; i < wx; i++) cx += lp_pass(i, wx);
         nlp = cy * cx;
     }
     const double lpm = (double)nlp / (double)n;
     double pk = 0.0;
     for (int t = 0; t < 3; t++) {
         const int ox = (int)rint(dx), oy = (int)rint(dy);
-        const int wy0 = y - hy + oy, wx0 = x - hx + ox;
-        if (wy0 < 0 || wx0 < 0 || wy0 + P.wy > P.H || wx0 + P.wx > P.W) {
+        const int wy0 = y - hy2 + oy, wx0 = x - hx2 + ox;
+        if (wy0 < 0 || wx0 < 0 || wy0 + wy > P.H || wx0 + wx > P.W) {
             pk = 0.0;
             break;
         }
-        load_window(I2, P.W, wy0, wx0, P.wy, P.wx, X, red);
-        float2 *B = fft2_lds(X, Y, P.wy, P.wx, P, twx, twy);
-        // normalised cross power spectrum, low-passed, conjugated for the inverse (layout [kx][ky])
+        load_window(I2, P.W, wy0, wx0, wy, wx, WIN ? T64 : wx, hy, hx, X, red);
+        float2 *B;
+        if constexpr (WIN == 64) {
+            fft64x64(X, Y, twx);
+            B = X;
+        } else {
+            B = fft2_lds(X, Y, wy, wx, P, twx, twy);
+        }
+        // normalised cross power spectrum, low-passed, conjugated for the inverse
 #pragma unroll
-        for (int q = 0; q < POC_MAX / 256; q++) {
-            const int i = threadIdx.x + 256 * q;
+        for (int q = 0; q < POC_PER; q++) {
+            const int i = threadIdx.x + POC_NT * q;
             if (i >= n) break;
-            const int kx = i / P.wy, ky = i - kx * P.wy;
-            const float2 b = B[i], a = A[q];
+            int kx, ky, e;
+            if constexpr (WIN == 64) {
+                ky = i >> 6;
+                kx = i & 63;
+                e = ky * T64 + kx;
+            } else {
+                kx = i / wy;
+                ky = i - kx * wy;
+                e = i;
+            }
+            const float2 b = B[e], a = A[q];
             float2 c = make_float2(b.x * a.x + b.y * a.y, b.y * a.x - b.x * a.y);
             const float m = sqrtf(c.x * c.x + c.y * c.y) + 1e-12f;
             c = make_float2(c.x / m, c.y / m);
-            if (!(lp_pass(kx, P.wx) && lp_pass(ky, P.wy))) c = make_float2(0.f, 0.f);
-            B[i] = cconj(c);
+            if (!(lp_pass(kx, wx) && lp_pass(ky, wy))) c = make_float2(0.f, 0.f);
+            B[e] = cconj(c);
         }
         __syncthreads();
-        // inverse: along ky (lines of length wy), transpose back to [ky][kx], along kx
-        float2 *o = (B == X) ? Y : X;
-        float2 *r1 = fft_lines_lds(B, o, P.wy, P.wx, P.facy, P.nfy, twy, threadIdx.x, blockDim.x);
-        float2 *o2 = (r1 == B) ? o : B;
-        for (int i = threadIdx.x; i < n; i += blockDim.x) {
-            const int kx = i / P.wy, yy = i - kx * P.wy;
-            o2[yy * P.wx + kx] = r1[i];
+        float2 *r2;
+        int pitch;
+        if constexpr (WIN == 64) {
+            fft64x64(X, Y, twx);  // forward transform of the conjugate = conjugate of the inverse
+            r2 = X;
+            pitch = T64;
+        } else {
+            // inverse: along ky (lines of length wy), transpose back to [ky][kx], along kx
+            float2 *o = (B == X) ? Y : X;
+            float2 *r1 = fft_lines_lds(B, o, wy, wx, P.facy, P.nfy, twy, threadIdx.x, blockDim.x);
+            float2 *o2 = (r1 == B) ? o : B;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) {
+                const int kx = i / wy, yy = i - kx * wy;
+                o2[yy * wx + kx] = r1[i];
+            }
+            __syncthreads();
+            r2 = fft_lines_lds(o2, r1, wx, wy, P.facx, P.nfx, twx, threadIdx.x, blockDim.x);
+            pitch = wx;
         }
-        __syncthreads();
-        float2 *r2 = fft_lines_lds(o2, r1, P.wx, P.wy, P.facx, P.nfx, twx, threadIdx.x, blockDim.x);
         // the buffer r2 does not occupy holds the argmax partials
         float *bval = (float *)((r2 == X) ? Y : X);
-        int *bidx = (int *)(bval + 256);
+        int *bidx = (int *)(bval + 64);
         // r = fftshift(real(ifft2)) / lpm; argmax in the shifted array (first maximum, NaN wins)
         const double sc = 1.0 / ((double)n * lpm);
-        const int shy = (P.wy + 1) / 2, shx = (P.wx + 1) / 2;
+        const int shy = (wy + 1) / 2, shx = (wx + 1) / 2;
+        auto at = [&](int sy, int sx) {  // shifted (sy, sx) -> unshifted element
+            int uy = sy + shy, ux = sx + shx;
+            uy -= uy >= wy ? wy : 0;
+            ux -= ux >= wx ? wx : 0;
+            return r2[uy * pitch + ux].x;
+        };
         float best = 0.f;
         int bi = -1;
         bool bnan = false;
         for (int s = threadIdx.x; s < n; s += blockDim.x) {
-            const int sy = s / P.wx, sx = s - sy * P.wx;
-            const int uy = (sy + shy) % P.wy, ux = (sx + shx) % P.wx;
-            const float v = (float)((double)r2[uy * P.wx + ux].x * sc);
+            int sy, sx;
+            if constexpr (WIN == 64) {
+                sy = s >> 6;
+                sx = s & 63;
+            } else {
+                sy = s / wx;
+                sx = s - sy * wx;
+            }
+            const float v = (float)((double)at(sy, sx) * sc);
             if (bnan) continue;
             if (isnan(v)) {
                 bnan = true;
@@ -252,14 +380,34 @@ __global__ void __launch_bounds__(256) k_poc_grid(const float *__restrict__ I1, 
                 bi = s;
             }
         }
-        bval[threadIdx.x] = bnan ? NAN : best;
-        bidx[threadIdx.x] = bi;
+        // first maximum over the workgroup (NaN wins, then the larger value, then the smaller index):
+        // wavefront shuffles, then the wavefront winners
+        for (int off = 32; off > 0; off >>= 1) {
+            const float ov = __shfl_xor(best, off);
+            const int oi = __shfl_xor(bi, off);
+            const bool on = __shfl_xor((int)bnan, off) != 0;
+            bool take;
+            if (oi < 0) take = false;
+            else if (bi < 0) take = true;
+            else if (bnan) take = on && oi < bi;
+            else if (on) take = true;
+            else take = ov > best || (ov == best && oi < bi);
+            if (take) {
+                best = ov;
+                bi = oi;
+                bnan = on;
+            }
+        }
+        if ((threadIdx.x & 63) == 0) {
+            bval[threadIdx.x >> 6] = bnan ? NAN : best;
+            bidx[threadIdx.x >> 6] = bi;
+        }
         __syncthreads();
         if (threadIdx.x == 0) {
             float bv = 0.f;
             int bb = -1;
             bool bn = false;
-            for (int k = 0; k < (int)blockDim.x; k++) {
+            for (int k = 0; k < (int)(blockDim.x >> 6); k++) {
                 const int ki = bidx[k];
                 if (ki < 0) continue;
                 const float kv = bval[k];
@@ -279,21 +427,18 @@ __global__ void __launch_bounds__(256) k_poc_grid(const float *__restrict__ I1, 
         }
         __syncthreads();
         const int bsel = bidx[0];
-        int py = bsel / P.wx, px = bsel - py * P.wx;
-        py = min(max(py, 1), P.wy - 2);
-        px = min(max(px, 1), P.wx - 2);
-        auto R = [&](int sy, int sx) {
-            const int uy = (sy + shy) % P.wy, ux = (sx + shx) % P.wx;
-            return (double)(float)((double)r2[uy * P.wx + ux].x * sc);
-        };
+        int py = bsel / wx, px = bsel - py * wx;
+        py = min(max(py, 1), wy - 2);
+        px = min(max(px, 1), wx - 2);
+        auto R = [&](int sy, int sx) { return (double)(float)((double)at(sy, sx) * sc); };
         const double c0 = R(py, px);
         const double sx = parabola(R(py, px - 1), c0, R(py, px + 1));
         const double sy = parabola(R(py - 1, px), c0, R(py + 1, px));
         pk = c0;
-        dx = ox + (px - P.wx / 2 + sx);
-        dy = oy + (py - P.wy / 2 + sy);
+        dx = ox + (px - wx / 2 + sx);
+        dy = oy + (py - wy / 2 + sy);
         __syncthreads();
-        if (fabs(px - P.wx / 2 + sx) < 0.5 && fabs(py - P.wy / 2 + sy) < 0.5) break;
+        if (fabs(px - wx / 2 + sx) < 0.5 && fabs(py - wy / 2 + sy) < 0.5) break;
     }
     if (threadIdx.x == 0) {
         gux[pt] = dx;
@@ -510,6 +655,7 @@ void coarse_release(CoarseWs *&ws) {
     (void)hipFree(ws->ix);
     (void)hipFree(ws->iy);
     (void)hipFree(ws->gflag);
+    (void)hipFree(ws->hann);
     delete ws;
     ws = nullptr;
 }
@@ -538,6 +684,7 @@ void coarse_dense(const float *i1, const float *i2, int w, int h, const ofx_conf
         }
         size_t cap = 0;
         ensure(ws->tmp, (size_t)w * ((h + 1) / 2), cap);
+        HIPCHECK(hipMalloc(&ws->hann, 2 * (size_t)POC_MAX * sizeof(float)));
     }
     // pyramid
     for (int l = 0; l < L; l++) {
@@ -623,9 +770,30 @@ void coarse_dense(const float *i1, const float *i2, int w, int h, const ofx_conf
             P.facy[i] = PY.fac[i];
         }
         P.have_init = have_init;
-        const size_t lds = 2 * (size_t)wy * wx * sizeof(float2);
-        hipLaunchKernelGGL(k_poc_grid, dim3((unsigned)np), dim3(256), lds, s, a, b, P, PX.tw, PY.tw, ws->ix, ws->iy,
-                           ws->gux, ws->guy, ws->gpk);
+        hipLaunchKernelGGL(k_hann, dim3((wy + wx + 255) / 256), dim3(256), 0, s, ws->hann, wy, wx);
+        LAUNCHCHECK();
+        if (wy == 64 && wx == 64) {
+            // 67.6 KiB: above the 64 KiB default, within the 160 KiB of a CDNA4 workgroup
+            const size_t lds = (2 * (size_t)64 * T64 + 64) * sizeof(float2) + 128 * sizeof(float);
+            static std::once_flag once;
+            std::call_once(once, [] {
+                (void)hipFuncSetAttribute((const void *)k_poc_grid<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          160 * 1024);
+            });
+            hipLaunchKernelGGL(k_poc_grid<64>, dim3((unsigned)np), dim3(POC_NT), lds, s, a, b, P, PX.tw, PY.tw,
+                               ws->hann, ws->ix, ws->iy, ws->gux, ws->guy, ws->gpk);
+        } else {
+            const size_t lds = (2 * (size_t)wy * wx + wx + wy) * sizeof(float2) + (size_t)(wy + wx) * sizeof(float);
+            if (lds > 65536) {
+                static std::once_flag once;
+                std::call_once(once, [] {
+                    (void)hipFuncSetAttribute((const void *)k_poc_grid<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                              160 * 1024);
+                });
+            }
+            hipLaunchKernelGGL(k_poc_grid<0>, dim3((unsigned)np), dim3(POC_NT), lds, s, a, b, P, PX.tw, PY.tw,
+                               ws->hann, ws->ix, ws->iy, ws->gux, ws->guy, ws->gpk);
+        }
         LAUNCHCHECK();
         hipLaunchKernelGGL(k_clean, dim3(1), dim3(1024), 0, s, ws->gux, ws->guy, ws->gpk, ws->ix, ws->iy, have_init,
                            ws->gflag, ny, nx, c.poc_peak_min, c.median_thresh);

@@ -169,13 +169,17 @@ def local_solve(sums, ux, uy, sigma, order):
     return o[0], o[1], o[2], o[3], slope  # ux, uy, energy, resid, slope
 
 
-def refine(i1, i2, ux, uy, cfg):
-    """Returns (ux, uy, conf, resid, energy) of phaseflow.refine."""
+def refine(i1, i2, ux, uy, cfg, shift=(0, 0)):
+    """Returns (ux, uy, conf, resid, energy) of phaseflow.refine. i2 may be any window of the second
+    frame, pixel (x, y) of i1 sitting at (x + shift[0], y + shift[1]) in it
+    (rust/crates/optflow-core/src/phaseflow.rs:1441-1456)."""
     i1, i2, ux, uy = _d(i1), _d(i2), _d(ux), _d(uy)
     h, w = i1.shape
     o = [np.empty((h, w)) for _ in range(5)]
     c = _cfg(cfg)
-    _check(lib().orc_refine(_p(i1), _p(i2), h, w, _p(ux), _p(uy), ctypes.byref(c), *[_p(a) for a in o]))
+    _check(lib().orc_refine_shifted(_p(i1), h, w, _p(i2), i2.shape[0], i2.shape[1], ctypes.c_long(shift[0]),
+                                    ctypes.c_long(shift[1]), _p(ux), _p(uy), ctypes.byref(c),
+                                    *[_p(a) for a in o]))
     return tuple(o)
 
 

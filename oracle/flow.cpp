@@ -307,10 +307,12 @@ double border_distance(double x, double y, int w, int h) {
 
 void gabor_bank_public(int hp, int wp, const orc_cfg &c, std::vector<double> &bank) { gabor_bank(hp, wp, c, bank); }
 
-// phaseflow.py:427-511 without the crack options
-void refine(const double *i1, const double *i2, int h, int w, const double *ux0, const double *uy0,
-            const orc_cfg &c, double *ux_out, double *uy_out, double *conf_out, double *resid_out,
-            double *energy_out) {
+// phaseflow.py:427-511 without the crack options; image 2 may be any window (h2 x w2) of the
+// second frame, pixel (x, y) of image 1 sitting at (x + shx, y + shy) in it before the field
+// displaces it (rust/crates/optflow-core/src/phaseflow.rs:1416-1439, 1441-1456).
+void refine_shifted(const double *i1, int h, int w, const double *i2, int h2, int w2, long shx, long shy,
+                    const double *ux0, const double *uy0, const orc_cfg &c, double *ux_out, double *uy_out,
+                    double *conf_out, double *resid_out, double *energy_out) {
     Cfg C(c);
     const int hp = h + 2 * PAD, wp = w + 2 * PAD;
     const size_t n = (size_t)h * w, np_ = (size_t)hp * wp;
@@ -322,8 +324,9 @@ void refine(const double *i1, const double *i2, int h, int w, const double *ux0,
     std::vector<double> ux(ux0, ux0 + n), uy(uy0, uy0 + n);
     std::vector<double> resid(n, M_PI), energy(n, 1.0), slope(4 * n, 0.0);
     std::vector<double> nux(n), nuy(n);
-    std::vector<double> c2(i2, i2 + n);
-    bspline_real(c2, h, w);
+    std::vector<double> c2(i2, i2 + (size_t)h2 * w2);
+    bspline_real(c2, h2, w2);
+    const double fsx = (double)shx, fsy = (double)shy;
     std::vector<double> sx(n), sy(n), warped(n), conf(n);
     std::vector<std::vector<cd>> resp(F);
     std::vector<double> sums(7 * n);
@@ -348,7 +351,7 @@ void refine(const double *i1, const double *i2, int h, int w, const double *ux0,
         for (int y = 0; y < h; y++)
             for (int x = 0; x < w; x++) {
                 size_t i = (size_t)y * w + x;
-                warped[i] = sample_at(c2.data(), h, w, x + sx[i], y + sy[i]);
+                warped[i] = sample_at(c2.data(), h2, w2, ((double)x + fsx) + sx[i], ((double)y + fsy) + sy[i]);
             }
         std::vector<cd> S2 = padded_spectrum(warped.data(), h, w);
         for (int f = 0; f < F; f++) {
@@ -414,6 +417,12 @@ void refine(const double *i1, const double *i2, int h, int w, const double *ux0,
         }
 }
 
+void refine(const double *i1, const double *i2, int h, int w, const double *ux0, const double *uy0,
+            const orc_cfg &c, double *ux_out, double *uy_out, double *conf_out, double *resid_out,
+            double *energy_out) {
+    refine_shifted(i1, h, w, i2, h, w, 0, 0, ux0, uy0, c, ux_out, uy_out, conf_out, resid_out, energy_out);
+}
+
 }  // namespace orc
 
 using namespace orc;
@@ -457,5 +466,20 @@ extern "C" int orc_refine(const double *i1, const double *i2, int h, int w, cons
         return 1;
     }
     refine(i1, i2, h, w, ux0, uy0, *cfg, ux, uy, conf, resid, energy);
+    return 0;
+}
+
+extern "C" int orc_refine_shifted(const double *i1, int h, int w, const double *i2, int h2, int w2, long shx,
+                                  long shy, const double *ux0, const double *uy0, const orc_cfg *cfg, double *ux,
+                                  double *uy, double *conf, double *resid, double *energy) {
+    if (h <= PAD || w <= PAD) {
+        set_error("refine: image too small for the reflect pad");
+        return 1;
+    }
+    if (h2 < 1 || w2 < 1) {
+        set_error("refine: empty image i2");
+        return 1;
+    }
+    refine_shifted(i1, h, w, i2, h2, w2, shx, shy, ux0, uy0, *cfg, ux, uy, conf, resid, energy);
     return 0;
 }

@@ -75,6 +75,9 @@ double orc_median(const double *v, long n);
 void orc_local_solve(const double *sums /* 7 x n */, const double *ux, const double *uy, int h,
                      int w, double sigma, int order, double *oux, double *ouy, double *energy,
                      double *resid, double *slope /* n x 4 */);
+int orc_refine_shifted(const double *i1, int h, int w, const double *i2, int h2, int w2, long shx,
+                       long shy, const double *ux0, const double *uy0, const orc_cfg *cfg, double *ux,
+                       double *uy, double *conf, double *resid, double *energy);
 int orc_refine(const double *i1, const double *i2, int h, int w, const double *ux0,
                const double *uy0, const orc_cfg *cfg, double *ux, double *uy, double *conf,
                double *resid, double *energy);

@@ -67,6 +67,27 @@ def test_refine_fixture(eng, order):
     assert e["resid"] <= 1e-3 and e["energy_rel"] <= 1e-4
 
 
+def test_refine_shifted_window(eng):
+    """FineEngine::refine_raw with image 2 a displaced window of the second frame (the tiled
+    driver's call, rust/crates/optflow-core/src/tile.rs:114-146, phaseflow.rs:1441-1456): GPU vs
+    oracle on the same window and shift."""
+    from optflow_soil_amd import synth
+    pat = synth.make_pattern(96 + 2 * 64, 112 + 2 * 64, seed=3)
+    f = synth.stretch_translate(0.01, (1.3, -0.8), (56.0, 48.0))
+    A = synth.as_float(synth.render(pat))
+    B = synth.as_float(synth.render(pat, f, noise=0.005, seed=1))
+    h, w = A.shape
+    win = np.ascontiguousarray(B[5:h - 2, 3:w - 2])
+    ix, iy = np.full((h, w), 1.0, np.float32), np.full((h, w), -1.0, np.float32)
+    cfg = Config()
+    ux, uy, en, res = eng.refine_raw(A, win, ix, iy, cfg, shift=(-3, -5))
+    oux, ouy, _, ores, oen = O.refine(f64(A), f64(win), f64(ix), f64(iy), cfg, shift=(-3, -5))
+    e = {"ux": rms(ux, oux), "uy": rms(uy, ouy), "resid": rms(res, ores), "energy_rel": rms(en, oen) / np.abs(oen).mean()}
+    print("refine shifted", " ".join(f"{k}={v:.3g}" for k, v in e.items()), flush=True)
+    assert e["ux"] <= DISP_TOL and e["uy"] <= DISP_TOL
+    assert e["resid"] <= 1e-3 and e["energy_rel"] <= 1e-4
+
+
 def test_pair_fixture(eng):
     a, b, roi = load("pair_a"), load("pair_b"), load("pair_roi") > 0.5
     r = eng.track_pair(f32(a), f32(b), Config(), mask=roi)

@@ -306,3 +306,32 @@ def test_sequence_mean_conf_and_reference_moves(name, forced):
     out = O.track_sequence(list(load(name + "_frames")), Config(ref_update_conf=2.0 if forced else 0.5))
     assert [o["timing"]["reference"] for o in out] == m["reference"]
     assert np.abs(np.array([o["timing"]["mean_conf"] for o in out]) - m["mean_conf"]).max() < 1e-6
+
+
+def test_refine_shifted_window_matches_full_frame():
+    """refine_raw with image 2 a displaced window of the second frame
+    (rust/crates/optflow-core/src/phaseflow.rs:1441-1456): cropping frame 2 to a window that holds
+    every sampled position and passing the crop's offset as the shift gives the interior field of
+    the whole-frame refine (the spline of the crop differs only within about 15 px of the crop's
+    border, z^15 < 3e-9). A wrong sign or axis of the shift moves every sample by twice the offset
+    and fails by pixels."""
+    from optflow_soil_amd import synth
+    pat = synth.make_pattern(96 + 2 * 64, 112 + 2 * 64, seed=3)
+    f = synth.stretch_translate(0.01, (1.3, -0.8), (56.0, 48.0))
+    A = synth.as_float(synth.render(pat)).astype(np.float64)
+    B = synth.as_float(synth.render(pat, f, noise=0.005, seed=1)).astype(np.float64)
+    h, w = A.shape
+    ux0, uy0 = np.full((h, w), 1.0), np.full((h, w), -1.0)
+    cfg = Config(outer_iters=3)
+    full = O.refine(A, B, ux0, uy0, cfg)
+    # window of frame 2 from (x0, y0) = (3, 5) to the far edge minus 2: pixel (x, y) of A sits at
+    # (x - 3, y - 5) in it
+    x0, y0 = 3, 5
+    win = np.ascontiguousarray(B[y0:h - 2, x0:w - 2])
+    part = O.refine(A, win, ux0, uy0, cfg, shift=(-x0, -y0))
+    inner = np.s_[40:-40, 40:-40]
+    assert np.abs(part[0][inner] - full[0][inner]).max() < 1e-4
+    assert np.abs(part[1][inner] - full[1][inner]).max() < 1e-4
+    # the same window with the shift's sign flipped lands elsewhere
+    bad = O.refine(A, win, ux0, uy0, cfg, shift=(x0, y0))
+    assert np.abs(bad[0][inner] - full[0][inner]).max() > 0.1
