@@ -52,7 +52,7 @@ typedef struct ofx_config {
     double bandwidth;         /* 0.5 */
     double dc_cut;            /* 2.0 */
     double sigma_w;           /* 4 */
-    int shape_order;          /* 1 (2 is refused with OFX_EUNSUPPORTED) */
+    int shape_order;          /* 1: affine, 2: quadratic shape function */
     double sigma_s;           /* 4 */
     int outer_iters;          /* 8 */
     int inner_iters;          /* 2 */
@@ -128,6 +128,11 @@ typedef struct ofx_result {
     double t_coarse, t_fine, t_strain; /* seconds (Timing, track.rs:148-161) */
     int reference;                     /* sequences: reference frame index */
     double mean_conf;                  /* sequences: mean confidence of the leg */
+    /* hybrid sequences: the timing shares of track/hybrid.rs:520-531 (over ROI pixels) */
+    int n_fb_max;                      /* most consecutive frames any pixel has been carried */
+    double fallback_frac;              /* valid by an accepted fallback */
+    double fb_gate_rejected;           /* fallback candidates the gate rejected */
+    double reacquired;                 /* re-acquired by the direct leg */
 } ofx_result;
 
 /* track_pair (rust/crates/optflow-core/src/track.rs:688-861; python/optflow_ref/track.py:115-146):
@@ -142,8 +147,10 @@ int ofx_track_pair_device(ofx_engine *e, const float *d_i1, const float *d_i2, i
                           const uint8_t *d_mask, const float *d_init_ux, const float *d_init_uy,
                           const ofx_config *cfg, ofx_result *d_out);
 
-/* track_sequence in seq_mode "chain" (track.rs:1112-1225; track.py:260-315): frames is n x h x w
- * (host), results has n - 1 entries for frames 1 .. n - 1. */
+/* track_sequence (track.rs:1112-1225; track.py:260-315) in seq_mode "chain", or "hybrid" when
+ * cfg->seq_hybrid (direct-first with the gated adjacent-frame fallback, track/hybrid.rs:262-540;
+ * track.py:383-505; results carry fb and the hybrid timing shares): frames is n x h x w (host),
+ * results has n - 1 entries for frames 1 .. n - 1. */
 int ofx_track_sequence(ofx_engine *e, const float *frames, int n, int w, int h,
                        const uint8_t *mask, const ofx_config *cfg, ofx_result *results);
 

@@ -45,7 +45,8 @@ class OfxResult(ctypes.Structure):
     _fields_ = [(k, _FP) for k in _RES_F32] + [
         ("valid", ctypes.POINTER(ctypes.c_uint8)), ("fb", ctypes.POINTER(ctypes.c_int32)),
         ("t_coarse", ctypes.c_double), ("t_fine", ctypes.c_double), ("t_strain", ctypes.c_double),
-        ("reference", ctypes.c_int), ("mean_conf", ctypes.c_double)]
+        ("reference", ctypes.c_int), ("mean_conf", ctypes.c_double), ("n_fb_max", ctypes.c_int),
+        ("fallback_frac", ctypes.c_double), ("fb_gate_rejected", ctypes.c_double), ("reacquired", ctypes.c_double)]
 
 
 _lib = None
@@ -109,9 +110,10 @@ class Result:
     raw_uy: np.ndarray
     valid: np.ndarray
     timing: dict = field(default_factory=dict)
+    fb: np.ndarray = None  # hybrid sequences: consecutive frames carried by the adjacent leg
 
 
-def _new_result(h, w):
+def _new_result(h, w, with_fb=False):
     arrs = {k: np.empty((h, w), np.float32) for k in _RES_F32}
     valid = np.empty((h, w), np.uint8)
     r = OfxResult()
@@ -119,6 +121,9 @@ def _new_result(h, w):
         setattr(r, k, arrs[k].ctypes.data_as(_FP))
     r.valid = valid.ctypes.data_as(ctypes.POINTER(ctypes.c_uint8))
     r.fb = None
+    if with_fb:
+        arrs["fb"] = np.zeros((h, w), np.int32)
+        r.fb = arrs["fb"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
     return r, arrs, valid
 
 
@@ -126,8 +131,12 @@ def _to_result(r, arrs, valid):
     strain = {"Exx": arrs["exx"], "Eyy": arrs["eyy"], "Exy": arrs["exy"], "gmax": arrs["gmax"], "vol": arrs["vol"]}
     timing = {"coarse": r.t_coarse, "fine": r.t_fine, "strain": r.t_strain, "reference": r.reference,
               "mean_conf": r.mean_conf, "device": "gpu"}
+    fb = arrs.get("fb")
+    if fb is not None:
+        timing.update(n_fb_max=r.n_fb_max, fallback_frac=r.fallback_frac, fb_gate_rejected=r.fb_gate_rejected,
+                      reacquired=r.reacquired)
     return Result(arrs["ux"], arrs["uy"], arrs["conf"], strain, arrs["raw_ux"], arrs["raw_uy"], valid.astype(bool),
-                  timing)
+                  timing, fb)
 
 
 class Engine:
@@ -222,7 +231,8 @@ class Engine:
         return {"coarse": r.t_coarse, "fine": r.t_fine, "strain": r.t_strain}
 
     def track_sequence(self, frames, cfg, mask=None):
-        """track.py:260-315 (chain): list of (k, Result) for k = 1 .. n - 1."""
+        """track.py:260-315 (seq_mode chain) and track.py:383-505 (hybrid; results carry fb and the
+        hybrid timing shares): list of (k, Result) for k = 1 .. n - 1."""
         fr = _f32(np.stack(frames))
         n, h, w = fr.shape
         if mask is not None and np.shape(mask) != (h, w):
@@ -232,8 +242,9 @@ class Engine:
             raise ValueError("ROI mask is empty")
         rs = (OfxResult * max(n - 1, 1))()
         keep = []
+        hybrid = cfg.seq_mode == "hybrid"
         for k in range(n - 1):
-            r, arrs, valid = _new_result(h, w)
+            r, arrs, valid = _new_result(h, w, with_fb=hybrid)
             rs[k] = r
             keep.append((arrs, valid))
         c = to_c(cfg)

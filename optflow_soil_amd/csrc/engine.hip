@@ -118,7 +118,7 @@ struct Workspace {
     DevBuf bi1, bi2, bmask, binx, biny;                 // staging of host inputs
     DevBuf bc2, bspec, btmp, btmp2, bresp, brefp;
     DevBuf bux, buy, bnux, bnuy, bsx, bsy, bcsx, bcsy, bwarped, bfx, bfy, bbt;
-    DevBuf bsums, bym, benergy, bresid, bslope, bmed, bmedscr;
+    DevBuf bsums, bym, benergy, bresid, bslope, bmed, bmedscr, bmom2;
     DevBuf bconf, bconf64, blow, bkept, bwithin, bok, bcold2, bcolrow, bcnt;
     DevBuf bparent, bsizes, blarge, bsrc, bnear, bmx, bmy, bytmp, bcolsum, bpftmp;
     DevBuf bstart_x, bstart_y;
@@ -129,6 +129,9 @@ struct Workspace {
     int taps_s_r = 0, taps_w_r = 0;
     // sequence state
     DevBuf bu0x, bu0y, bc0r, bcx, bcy, bconf0, bca, bprev_x, bprev_y, bprev_c, bleg_c, bmean, bframe_prev, bframe_ref;
+    // hybrid sequence state (track/hybrid.rs)
+    DevBuf banch, bgood_prev, bupx, bupy, bnfb, bpadj_x, bpadj_y, badj_x, badj_y, badj_c, blink, bflags, bax, bay,
+        bcax, bcay, bca_adj, bgate0, bgate, bhcnt, bgscr, bhlow, bhux, bhuy, bfb;
     CoarseWs *coarse = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     ~Workspace() {
@@ -182,6 +185,8 @@ double max_scale(const ofx_config &c) {
 }
 double auto_edge_guard(const ofx_config &c) { return 3.0 * c.lam * max_scale(c) / (2 * M_PI * c.bandwidth); }
 double edge_guard(const ofx_config &c) { return c.edge_guard >= 0 ? c.edge_guard : auto_edge_guard(c); }
+// config.py:253-255: half-width of the hybrid gate's window
+double fb_radius_px(const ofx_config &c) { return c.fb_radius > 0 ? c.fb_radius : 4.0 * c.sigma_w; }
 double edge_guard_min_area(const ofx_config &c) {
     if (c.edge_guard_min_area >= 0) return c.edge_guard_min_area;
     const double l = c.lam * max_scale(c);
@@ -214,8 +219,6 @@ void validate(const ofx_config *cfg) {
         throw Error(OFX_EINVAL,
                     "at most 64 filters (n_orient * len(scales)); sigma_w, sigma_s <= 256, strain_win <= 255 and "
                     "poc_win <= 1024 required");
-    if (c.shape_order == 2)
-        throw Error(OFX_EUNSUPPORTED, "shape_order 2 (quadratic shape function) is not implemented on the GPU yet");
 }
 
 void check_window_impl(int w, int h, const ofx_config &c) {
@@ -244,8 +247,9 @@ void set_taps(Workspace &ws, const ofx_config &c, hipStream_t s) {
     if (ws.taps_w_sigma != c.sigma_w) {
         std::vector<double> k = gauss_taps(c.sigma_w);
         const int r = (int)k.size() / 2, klen = 2 * r + 1;
-        std::vector<double> kt(3 * klen);
-        for (int p = 0; p < 3; p++)
+        // rows p = 0 .. 4: the taps times t^p (order 1 uses the first three)
+        std::vector<double> kt(5 * klen);
+        for (int p = 0; p < 5; p++)
             for (int t = -r; t <= r; t++) kt[p * klen + t + r] = k[t + r] * std::pow((double)t, p);
         double *d = ws.btaps_w.get<double>(kt.size());
         HIPCHECK(hipMemcpy(d, kt.data(), kt.size() * sizeof(double), hipMemcpyHostToDevice));
@@ -256,12 +260,11 @@ void set_taps(Workspace &ws, const ofx_config &c, hipStream_t s) {
     (void)s;
 }
 
-// The fine stage (phaseflow.py:427-511, phaseflow.rs:1514-1613) from the start field in
-// ws.bux / ws.buy (double). Leaves ux, uy, energy, resid, the energy median (ws.bmed[0]) and the
-// final confidence (ws.bconf as float, ws.bconf64 as double) in the workspace.
 // The fine stage (phaseflow.py:427-511, phaseflow.rs:1441-1613) on image 1 (w x h) against image 2
 // (w2 x h2), pixel (x, y) of image 1 sitting at (x + shx, y + shy) in image 2 before the field
-// displaces it. The start field is in ws.bux / ws.buy; the result replaces it.
+// displaces it. The start field is in ws.bux / ws.buy (double); the result replaces it, and energy,
+// resid, the energy median (ws.bmed[0]) and the final confidence (ws.bconf as float, ws.bconf64 as
+// double) are left in the workspace.
 void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h, int w2, int h2, long shx, long shy,
                    const ofx_config &c, hipStream_t s) {
     const int wp = w + 2 * PAD, hp = h + 2 * PAD;
@@ -347,6 +350,10 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
             ws.prof.mark(ST_ACCUMULATE, s);
             accumulate(ux, uy, sx, sy, csx, csy, resp, refp, sums, w, h, F, s);
             ws.prof.mark(ST_MOMENTS, s);
+            if (c.shape_order == 2)
+                moments_solve2(sums, B<double>(ws.bmom2, moments2_scratch_doubles(n)), ux, uy, nux, nuy, energy, resid,
+                               slope, w, h, (const double *)ws.btaps_w.p, ws.taps_w_r, s);
+            else
             moments_solve(sums, ym, ux, uy, nux, nuy, energy, resid, slope, w, h, (const double *)ws.btaps_w.p,
                           ws.taps_w_host.data(), ws.taps_w_r, s);
             ws.prof.mark(ST_SOLVE, s);
@@ -368,31 +375,37 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
     confidence(energy, resid, med, ux, uy, c.phase_tol, c.conf_min, Bw, w, h, nullptr, conf, conf64, s);
 }
 
-// Validity, guard and strain (track.py:56-108) of the field (ux, uy) with confidence conf (float).
-void finish_device(Workspace &ws, const double *ux, const double *uy, const float *conf, int w, int h,
-                   const uint8_t *mask, const ofx_config &c, ofx_result *o, hipStream_t s) {
+// Validity, guard and strain (track.py:56-108) of the field (ux, uy): low = conf < conf_min, or the
+// given low mask (the hybrid sequence's, hybrid.rs:495-503).
+void finish_device(Workspace &ws, const double *ux, const double *uy, const float *conf, const uint8_t *low_in,
+                   int w, int h, const uint8_t *mask, const ofx_config &c, ofx_result *o, hipStream_t s) {
     const size_t n = (size_t)w * h;
     uint8_t *low = B<uint8_t>(ws.blow, n), *near = B<uint8_t>(ws.bnear, n);
     const double guard = edge_guard(c);
     const int guard_on = guard > 0;
     ws.prof.mark(ST_GUARD, s);
+    if (low_in) HIPCHECK(hipMemcpyAsync(low, low_in, n, hipMemcpyDeviceToDevice, s));
+    else low_mask(conf, c.conf_min, n, low, s);
     if (guard_on) {
         uint8_t *large = B<uint8_t>(ws.blarge, n), *src = B<uint8_t>(ws.bsrc, n);
         int *parent = B<int>(ws.bparent, n);
         unsigned int *sizes = B<unsigned int>(ws.bsizes, n);
         int *cold2 = B<int>(ws.bcold2, n), *colrow = B<int>(ws.bcolrow, n);
-        low_mask(conf, c.conf_min, n, low, s);
         large_components(low, w, h, edge_guard_min_area(c), parent, sizes, large, s);
         guard_sources(large, mask, w, h, src, s);
         column_nearest(src, w, h, cold2, colrow, B<int>(ws.bcolsum, column_summary_ints(w, h)), s);
         within_distance(cold2, w, h, guard, near, s);
     }
     double *mx = B<double>(ws.bmx, n), *my = B<double>(ws.bmy, n), *yt = B<double>(ws.bytmp, 11 * n);
-    validity(conf, c.conf_min, mask, near, guard, guard_on, ux, uy, w, h, mx, my, s);
+    validity(low, mask, near, guard, guard_on, ux, uy, w, h, mx, my, s);
     ws.prof.mark(ST_STRAIN, s);
     strain(mx, my, w, h, c.strain_win, yt, o->ux, o->uy, o->valid, o->exx, o->eyy, o->exy, o->gmax, o->vol, s);
 }
 
+__global__ void k_fill(double *a, double v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
 __global__ void k_to_f32(const double *a, float *o, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) o[i] = (float)a[i];
@@ -445,7 +458,7 @@ void pair_device(Workspace &ws, const float *i1, const float *i2, int w, int h, 
     HIPCHECK(hipEventRecord(ws.ev[2], s));
     ux = (double *)ws.bux.p;
     uy = (double *)ws.buy.p;
-    finish_device(ws, ux, uy, (const float *)ws.bconf.p, w, h, mask, c, o, s);
+    finish_device(ws, ux, uy, (const float *)ws.bconf.p, nullptr, w, h, mask, c, o, s);
     to_f32(ux, o->raw_ux, n, s);
     to_f32(uy, o->raw_uy, n, s);
     if (o->conf) HIPCHECK(hipMemcpyAsync(o->conf, ws.bconf.p, n * sizeof(float), hipMemcpyDeviceToDevice, s));
@@ -503,6 +516,151 @@ int guarded(F &&f) {
     }
 }
 
+
+// seq_mode "hybrid" (rust/crates/optflow-core/src/track/hybrid.rs:262-540, track.py:383-505):
+// the direct leg of the chain plus, per pixel, the gated adjacent-frame fallback. The host steps
+// the frames and reads back the few counters that decide which legs and gates run.
+template <class Leg, class LegTo, class Mean>
+void sequence_hybrid(Workspace &ws, const float *frames, int nf, int w, int h, const uint8_t *dm, const ofx_config &c,
+                     ofx_result *results, hipStream_t s, Leg &run_leg, LegTo &run_leg_to, Mean &leg_mean) {
+    const size_t n = (size_t)w * h;
+    float *fref = (float *)ws.bframe_ref.p, *fprev = (float *)ws.bframe_prev.p, *fcur = (float *)ws.bi2.p;
+    double *u0x = (double *)ws.bu0x.p, *u0y = (double *)ws.bu0y.p, *c0r = (double *)ws.bc0r.p;
+    double *cx = (double *)ws.bcx.p, *cy = (double *)ws.bcy.p, *ca = (double *)ws.bca.p;
+    double *px = (double *)ws.bprev_x.p, *py = (double *)ws.bprev_y.p, *pc = (double *)ws.bprev_c.p;
+    double *lc = (double *)ws.bleg_c.p, *lx = (double *)ws.bstart_x.p, *ly = (double *)ws.bstart_y.p;
+    double *legc = B<double>(ws.bconf0, n);
+    uint8_t *anch = B<uint8_t>(ws.banch, n), *good_prev = B<uint8_t>(ws.bgood_prev, n);
+    double *upx = B<double>(ws.bupx, n), *upy = B<double>(ws.bupy, n);
+    int *nfb = B<int>(ws.bnfb, n);
+    double *pajx = B<double>(ws.bpadj_x, n), *pajy = B<double>(ws.bpadj_y, n);
+    double *ajx = B<double>(ws.badj_x, n), *ajy = B<double>(ws.badj_y, n), *ajc = B<double>(ws.badj_c, n);
+    double *link = B<double>(ws.blink, n);
+    uint8_t *flags = B<uint8_t>(ws.bflags, n);
+    double *ax = B<double>(ws.bax, n), *ay = B<double>(ws.bay, n);
+    double *cax = B<double>(ws.bcax, n), *cay = B<double>(ws.bcay, n), *caa = B<double>(ws.bca_adj, n);
+    uint8_t *gate0 = B<uint8_t>(ws.bgate0, n), *gate = B<uint8_t>(ws.bgate, n), *hlow = B<uint8_t>(ws.bhlow, n);
+    int *cnt = B<int>(ws.bhcnt, HYB_NCOUNTERS);
+    void *gscr = ws.bgscr.get<char>(gate_scratch_bytes(w, h));
+    double *hux = B<double>(ws.bhux, n), *huy = B<double>(ws.bhuy, n);
+    int32_t *dfb = B<int32_t>(ws.bfb, n);
+    {
+        std::vector<uint8_t> ones8(n, 1);
+        std::vector<double> zeros(n, 0.0);
+        std::vector<int> zi(n, 0);
+        HIPCHECK(hipMemcpyAsync(anch, ones8.data(), n, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(good_prev, ones8.data(), n, hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(upx, zeros.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(upy, zeros.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(caa, zeros.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(nfb, zi.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipStreamSynchronize(s));
+    }
+    auto counters = [&]() {
+        std::vector<int> hc(HYB_NCOUNTERS);
+        HIPCHECK(hipMemcpyAsync(hc.data(), cnt, HYB_NCOUNTERS * sizeof(int), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        return hc;
+    };
+    const bool gate_on = c.fb_min_support > 0;
+    const double radius = fb_radius_px(c);
+    int ref_idx = 0;
+    bool have_prev = false, have_prev_adj = false;
+    for (int k = 1; k < nf; k++) {
+        HIPCHECK(hipMemcpyAsync(fcur, frames + (size_t)k * n, n * sizeof(float), hipMemcpyHostToDevice, s));
+        auto tl = run_leg(fref, fcur, have_prev);
+        double cm = leg_mean();
+        if (cm < c.ref_update_conf && ref_idx != k - 1) {
+            conf_at(pc, u0x, u0y, w, h, ca, s);
+            hyb_refmove(good_prev, ca, c0r, anch, n, s);
+            HIPCHECK(hipMemcpyAsync(u0x, upx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipMemcpyAsync(u0y, upy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipMemcpyAsync(fref, fprev, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+            ref_idx = k - 1;
+            tl = run_leg(fref, fcur, false);
+            cm = leg_mean();
+        }
+        compose(u0x, u0y, lx, ly, w, h, cx, cy, s);
+        conf_at(lc, u0x, u0y, w, h, legc, s);
+        hyb_zero_counters(cnt, s);
+        hyb_classify(cx, cy, legc, c0r, anch, dm, upx, upy, nfb, c.conf_min, c.fb_max, ref_idx + 1 < k, link, flags, cnt,
+                     n, s);
+        bool have_adj = false;
+        if (counters()[HYB_ANY_NEED]) {
+            const double *rx = lx, *ry = ly, *rc = lc;
+            if (ref_idx != k - 1) {
+                auto ta = run_leg_to(fprev, fcur, have_prev_adj ? pajx : nullptr, have_prev_adj ? pajy : nullptr, ajx,
+                                     ajy, ajc);
+                tl.first += ta.first;
+                tl.second += ta.second;
+                rx = ajx;
+                ry = ajy;
+                rc = ajc;
+            }
+            compose(upx, upy, rx, ry, w, h, cax, cay, s);
+            conf_at(rc, upx, upy, w, h, caa, s);
+            hyb_carried(cax, cay, upx, upy, ax, ay, n, s);
+            // the adjacent leg's raw field starts the next adjacent leg
+            HIPCHECK(hipMemcpyAsync(pajx, rx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipMemcpyAsync(pajy, ry, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            have_adj = true;
+        } else {
+            HIPCHECK(hipMemcpyAsync(ax, upx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipMemcpyAsync(ay, upy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, caa, 0.0, n);
+            LAUNCHCHECK();
+        }
+        hyb_has_fb(flags, caa, c.conf_min, dm, cnt, n, s);
+        const uint8_t *g0 = nullptr;
+        if (gate_on && counters()[HYB_ANY_RE_NOFB]) {
+            hyb_gate(flags, dm, cx, cy, cx, cy, w, h, radius, c.fb_min_support, c.fb_tol, gscr, gate0, s);
+            g0 = gate0;
+        }
+        hyb_reacquire(flags, g0, cx, cy, ax, ay, legc, caa, c.fb_tol, c.conf_min, c.fb_max, c0r, anch, nfb, dm, cnt, n,
+                      s);
+        const uint8_t *g1 = nullptr;
+        if (gate_on && counters()[HYB_ANY_CAND]) {
+            hyb_gate(flags, dm, cx, cy, ax, ay, w, h, radius, c.fb_min_support, c.fb_tol, gscr, gate, s);
+            g1 = gate;
+        }
+        ofx_result *out = &results[k - 1];
+        Staged st = stage_outputs(ws, n, out);
+        hyb_output(flags, g1, cx, cy, ax, ay, link, caa, nfb, c.fb_max, dm, hux, huy, (float *)ws.bconf.p, hlow,
+                   dfb, cnt, n, s);
+        HIPCHECK(hipEventRecord(ws.ev[2], s));
+        finish_device(ws, hux, huy, (const float *)ws.bconf.p, hlow, w, h, dm, c, &st.d, s);
+        to_f32(hux, st.d.raw_ux, n, s);
+        to_f32(huy, st.d.raw_uy, n, s);
+        if (st.d.conf) HIPCHECK(hipMemcpyAsync(st.d.conf, ws.bconf.p, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        ws.prof.mark(ST_END, s);
+        HIPCHECK(hipEventRecord(ws.ev[3], s));
+        if (out->fb) HIPCHECK(hipMemcpyAsync(out->fb, dfb, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        download(st, out, n, s);
+        ws.prof.flush();
+        const std::vector<int> hc = counters();
+        const double nroi = (double)std::max(hc[HYB_N_ROI], 1);
+        out->t_coarse = tl.first;
+        out->t_fine = tl.second;
+        out->t_strain = elapsed(ws.ev[2], ws.ev[3]) * 1e-3;
+        out->reference = ref_idx;
+        out->mean_conf = cm;
+        out->n_fb_max = hc[HYB_NFB_MAX];
+        out->fallback_frac = hc[HYB_N_FB_OK] / nroi;
+        out->fb_gate_rejected = hc[HYB_N_REJECTED] / nroi;
+        out->reacquired = hc[HYB_N_RE] / nroi;
+        // state for frame k + 1
+        HIPCHECK(hipMemcpyAsync(upx, hux, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHECK(hipMemcpyAsync(upy, huy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        hyb_good_of(flags, good_prev, n, s);
+        have_prev_adj = have_adj;
+        HIPCHECK(hipMemcpyAsync(px, lx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHECK(hipMemcpyAsync(py, ly, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHECK(hipMemcpyAsync(pc, lc, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHECK(hipMemcpyAsync(fprev, fcur, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        have_prev = true;
+    }
+    HIPCHECK(hipStreamSynchronize(s));
+}
 }  // namespace
 
 }  // namespace ofx
@@ -757,7 +915,6 @@ int ofx_track_sequence(ofx_engine *e, const float *frames, int nf, int w, int h,
         validate(cfg);
         if (!frames || !results) throw Error(OFX_EINVAL, "frames or results is NULL");
         if (nf < 1) throw Error(OFX_EINVAL, "no frames");
-        if (cfg->seq_hybrid) throw Error(OFX_EUNSUPPORTED, "seq_mode hybrid is not implemented on the GPU yet");
         check_window_impl(w, h, *cfg);
         if (std::min(w, h) < cfg->poc_win)
             throw Error(OFX_EINVAL, "image smaller than poc_win (" + std::to_string(cfg->poc_win) + ")");
@@ -791,19 +948,27 @@ int ofx_track_sequence(ofx_engine *e, const float *frames, int nf, int w, int h,
         int ref_idx = 0;
         bool have_prev = false;
         ofx_result leg{};  // legs write nothing but the workspace's fields
-        auto run_leg = [&](const float *a, const float *b, bool with_init) {
+        // one leg a -> b from the start (ix, iy) or the coarse stage alone; its raw field and
+        // confidence go to (ox, oy, oc); returns the coarse and fine seconds
+        auto run_leg_to = [&](const float *a, const float *b, const double *ix, const double *iy, double *ox,
+                              double *oy, double *oc) {
             for (auto &ev : ws.ev)
                 if (!ev) HIPCHECK(hipEventCreate(&ev));
             HIPCHECK(hipEventRecord(ws.ev[0], s));
             ws.prof.mark(ST_COARSE, s);
             double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
-            coarse_dense(a, b, w, h, c, with_init ? px : nullptr, with_init ? py : nullptr, ux, uy, ws.coarse, s);
+            coarse_dense(a, b, w, h, c, ix, iy, ux, uy, ws.coarse, s);
             HIPCHECK(hipEventRecord(ws.ev[1], s));
             refine_device(ws, a, b, w, h, w, h, 0, 0, c, s);
             HIPCHECK(hipEventRecord(ws.ev[2], s));
-            HIPCHECK(hipMemcpyAsync(lx, ws.bux.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HIPCHECK(hipMemcpyAsync(ly, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HIPCHECK(hipMemcpyAsync(lc, ws.bconf64.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipMemcpyAsync(ox, ws.bux.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipMemcpyAsync(oy, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipMemcpyAsync(oc, ws.bconf64.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipEventSynchronize(ws.ev[2]));
+            return std::make_pair(elapsed(ws.ev[0], ws.ev[1]) * 1e-3, elapsed(ws.ev[1], ws.ev[2]) * 1e-3);
+        };
+        auto run_leg = [&](const float *a, const float *b, bool with_init) {
+            return run_leg_to(a, b, with_init ? px : nullptr, with_init ? py : nullptr, lx, ly, lc);
         };
         auto leg_mean = [&]() {
             mean_conf(lc, lx, ly, u0x, u0y, dm, Bw, w, h, macc, s);
@@ -812,9 +977,13 @@ int ofx_track_sequence(ofx_engine *e, const float *frames, int nf, int w, int h,
             HIPCHECK(hipStreamSynchronize(s));
             return acc[1] > 0 ? acc[0] / acc[1] : 0.0;
         };
+        if (c.seq_hybrid) {
+            sequence_hybrid(ws, frames, nf, w, h, dm, c, results, s, run_leg, run_leg_to, leg_mean);
+            return;
+        }
         for (int k = 1; k < nf; k++) {
             HIPCHECK(hipMemcpyAsync(fcur, frames + (size_t)k * n, n * sizeof(float), hipMemcpyHostToDevice, s));
-            run_leg(fref, fcur, have_prev);
+            auto tl = run_leg(fref, fcur, have_prev);
             double cm = leg_mean();
             if (cm < c.ref_update_conf && ref_idx != k - 1) {
                 // prev is the leg old reference -> k - 1: its confidence extends frame 0's link
@@ -825,7 +994,7 @@ int ofx_track_sequence(ofx_engine *e, const float *frames, int nf, int w, int h,
                 HIPCHECK(hipMemcpyAsync(u0y, cy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
                 HIPCHECK(hipMemcpyAsync(fref, fprev, n * sizeof(float), hipMemcpyDeviceToDevice, s));
                 ref_idx = k - 1;
-                run_leg(fref, fcur, false);
+                tl = run_leg(fref, fcur, false);
                 cm = leg_mean();
             }
             compose(u0x, u0y, lx, ly, w, h, cx, cy, s);
@@ -835,7 +1004,7 @@ int ofx_track_sequence(ofx_engine *e, const float *frames, int nf, int w, int h,
             LAUNCHCHECK();
             ofx_result *out = &results[k - 1];
             Staged st = stage_outputs(ws, n, out);
-            finish_device(ws, cx, cy, (const float *)ws.bconf.p, w, h, dm, c, &st.d, s);
+            finish_device(ws, cx, cy, (const float *)ws.bconf.p, nullptr, w, h, dm, c, &st.d, s);
             to_f32(cx, st.d.raw_ux, n, s);
             to_f32(cy, st.d.raw_uy, n, s);
             if (st.d.conf) HIPCHECK(hipMemcpyAsync(st.d.conf, ws.bconf.p, n * sizeof(float), hipMemcpyDeviceToDevice, s));
@@ -843,8 +1012,8 @@ int ofx_track_sequence(ofx_engine *e, const float *frames, int nf, int w, int h,
             HIPCHECK(hipEventRecord(ws.ev[3], s));
             download(st, out, n, s);
             ws.prof.flush();
-            out->t_coarse = elapsed(ws.ev[0], ws.ev[1]) * 1e-3;
-            out->t_fine = elapsed(ws.ev[1], ws.ev[2]) * 1e-3;
+            out->t_coarse = tl.first;
+            out->t_fine = tl.second;
             out->t_strain = elapsed(ws.ev[2], ws.ev[3]) * 1e-3;
             out->reference = ref_idx;
             out->mean_conf = cm;

@@ -95,13 +95,46 @@ __global__ void k_blur_y(const double *__restrict__ in, double *__restrict__ out
     out[(size_t)y * w + x] = s;
 }
 
-__global__ void k_blur_x(const double *__restrict__ in, float *__restrict__ out, int w, const BT bt, int r) {
-    const double *k = bt.k;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+// Register-blocked y pass for a compile-time radius R (sigma_s = 4 gives R = 16): one thread per
+// (column, BLUR_ROWS output rows); each of the BLUR_ROWS + 2R input rows is read once and added
+// into every output row it reaches, taps in the order of the sequential sum.
+constexpr int BLUR_ROWS = 16;
+
+template <int R>
+__global__ void __launch_bounds__(256) k_blur_y_rb(const double *__restrict__ in, double *__restrict__ out, int w,
+                                                   int h, const BT bt) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y0 = blockIdx.y * BLUR_ROWS;
     if (x >= w) return;
+    double acc[BLUR_ROWS];
+#pragma unroll
+    for (int q = 0; q < BLUR_ROWS; q++) acc[q] = 0;
+#pragma unroll
+    for (int j = 0; j < BLUR_ROWS + 2 * R; j++) {
+        const int yy = min(max(y0 - R + j, 0), h - 1);
+        const double v = in[(size_t)yy * w + x];
+#pragma unroll
+        for (int q = 0; q < BLUR_ROWS; q++) {
+            const int t = j - q;
+            if (t >= 0 && t <= 2 * R) acc[q] += bt.k[t] * v;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < BLUR_ROWS; q++)
+        if (y0 + q < h) out[(size_t)(y0 + q) * w + x] = acc[q];
+}
+
+// x pass from an LDS row segment of 256 + 2r values (replicated edges)
+__global__ void __launch_bounds__(256) k_blur_x_lds(const double *__restrict__ in, float *__restrict__ out, int w,
+                                                    const BT bt, int r) {
+    __shared__ double seg[256 + 2 * 64];
+    const int x0 = blockIdx.x * 256, y = blockIdx.y;
     const double *row = in + (size_t)y * w;
+    for (int c = threadIdx.x; c < 256 + 2 * r; c += blockDim.x) seg[c] = row[min(max(x0 - r + c, 0), w - 1)];
+    __syncthreads();
+    const int x = x0 + threadIdx.x;
+    if (x >= w) return;
     double s = 0;
-    for (int j = -r; j <= r; j++) s += k[j + r] * row[min(max(x + j, 0), w - 1)];
+    for (int j = 0; j <= 2 * r; j++) s += bt.k[j] * seg[threadIdx.x + j];
     out[(size_t)y * w + x] = (float)s;
 }
 
@@ -795,16 +828,199 @@ __global__ void k_energy_reset(const double *__restrict__ energy, const double *
     }
 }
 
+// ------------------------------------------------------------------ quadratic shape function
+// shape_order 2 (phaseflow.py:123-161): 12 unknowns (u, v and their first and second
+// derivatives), moments up to total power 4. A y pass writes 23 planes (the 7 sums times the y
+// powers each family needs), an x pass the 59 moments, and a solve pass the 12 x 12 Cholesky per
+// pixel. kt holds the window taps times t^p for p = 0 .. 4 (row p at p * (2r + 1)).
+constexpr int O2_YPLANES = 23, O2_MOMENTS = 59;
+
+// y-plane of (source, y power): sources 0..2 (gxx, gxy, gyy) powers 0..4, 3..4 (gxc, gyc) 0..2,
+// 5 (gcc), 6 (gw) power 0
+__host__ __device__ constexpr int o2_yplane(int src, int py) {
+    return src < 3 ? src * 5 + py : (src < 5 ? 15 + (src - 3) * 3 + py : 21 + (src - 5));
+}
+// moment of (source, x power, y power); the x powers of a y power run 0 .. (4 or 2) - py
+__host__ __device__ constexpr int o2_moment(int src, int px, int py) {
+    // sources 0..2: 15 moments each (px + py <= 4), ordered by py then px
+    // offset of (py) within a source with max total T: sum_{q < py} (T + 1 - q)
+    return src < 3 ? src * 15 + (py * 5 - py * (py - 1) / 2) + px
+                   : (src < 5 ? 45 + (src - 3) * 6 + (py * 3 - py * (py - 1) / 2) + px : 57 + (src - 5));
+}
+
+__global__ void __launch_bounds__(256) k_moments2_y(const double *__restrict__ sums, double *__restrict__ out, int w,
+                                                    int h, const double *__restrict__ kt, int r) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const size_t n = (size_t)w * h;
+    const int klen = 2 * r + 1;
+    double acc[O2_YPLANES];
+#pragma unroll
+    for (int q = 0; q < O2_YPLANES; q++) acc[q] = 0;
+    for (int j = -r; j <= r; j++) {
+        const int yy = min(max(y + j, 0), h - 1);
+        const size_t i = (size_t)yy * w + x;
+        double kp[5];
+#pragma unroll
+        for (int p = 0; p < 5; p++) kp[p] = kt[p * klen + j + r];
+#pragma unroll
+        for (int src = 0; src < 7; src++) {
+            const double v = sums[src * n + i];
+            const int np = src < 3 ? 5 : (src < 5 ? 3 : 1);
+#pragma unroll
+            for (int py = 0; py < 5; py++)
+                if (py < np) acc[o2_yplane(src, py)] += kp[py] * v;
+        }
+    }
+    const size_t i = (size_t)y * w + x;
+#pragma unroll
+    for (int q = 0; q < O2_YPLANES; q++) out[q * n + i] = acc[q];
+}
+
+__global__ void __launch_bounds__(128) k_moments2_x(const double *__restrict__ ym, double *__restrict__ mom, int w,
+                                                    int h, const double *__restrict__ kt, int r) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const size_t n = (size_t)w * h;
+    const int klen = 2 * r + 1;
+    double acc[O2_MOMENTS];
+#pragma unroll
+    for (int q = 0; q < O2_MOMENTS; q++) acc[q] = 0;
+    const double *row = ym + (size_t)y * w;
+    for (int j = -r; j <= r; j++) {
+        const int xx = min(max(x + j, 0), w - 1);
+        double kp[5];
+#pragma unroll
+        for (int p = 0; p < 5; p++) kp[p] = kt[p * klen + j + r];
+#pragma unroll
+        for (int src = 0; src < 7; src++) {
+            const int T = src < 3 ? 4 : (src < 5 ? 2 : 0);
+#pragma unroll
+            for (int py = 0; py <= 4; py++) {
+                if (py > T) continue;
+                const double v = row[o2_yplane(src, py) * n + xx];
+#pragma unroll
+                for (int px = 0; px <= 4; px++)
+                    if (px + py <= T) acc[o2_moment(src, px, py)] += kp[px] * v;
+            }
+        }
+    }
+    const size_t i = (size_t)y * w + x;
+#pragma unroll
+    for (int q = 0; q < O2_MOMENTS; q++) mom[q * n + i] = acc[q];
+}
+
+// The 12 x 12 system per pixel (unknown order of phaseflow.py:123-129: u, v, then for each power
+// (1,0), (0,1), (2,0), (1,1), (0,2) the u and v coefficients), as the 6 x 6 form: aperture test,
+// ridge 1e-12 tr / n, Cholesky, residual, slope = the four first derivatives.
+__global__ void __launch_bounds__(64) k_solve2(const double *__restrict__ mom, const double *__restrict__ ux_in,
+                                               const double *__restrict__ uy_in, double *__restrict__ ux_out,
+                                               double *__restrict__ uy_out, double *__restrict__ energy,
+                                               double *__restrict__ resid, double *__restrict__ slope, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    constexpr int N = 12;
+    constexpr int uc[N] = {0, 1, 0, 1, 0, 1, 0, 1, 0, 1, 0, 1};
+    constexpr int ua[N] = {0, 0, 1, 1, 0, 0, 2, 2, 1, 1, 0, 0};
+    constexpr int ub[N] = {0, 0, 0, 0, 1, 1, 0, 0, 1, 1, 2, 2};
+    double M[N][N], bv[N], b0[N];
+#pragma unroll
+    for (int a = 0; a < N; a++) {
+#pragma unroll
+        for (int b = 0; b <= a; b++) {
+            const int src = (uc[a] == 0 && uc[b] == 0) ? 0 : ((uc[a] == 1 && uc[b] == 1) ? 2 : 1);
+            M[a][b] = mom[(size_t)o2_moment(src, ua[a] + ua[b], ub[a] + ub[b]) * n + i];
+        }
+        bv[a] = mom[(size_t)o2_moment(3 + uc[a], ua[a], ub[a]) * n + i];
+    }
+    const double mcc = mom[(size_t)o2_moment(5, 0, 0) * n + i], mw = mom[(size_t)o2_moment(6, 0, 0) * n + i];
+    const double e = M[0][0] + M[1][1];
+    const double det2 = M[0][0] * M[1][1] - M[1][0] * M[1][0];
+    energy[i] = e;
+    bool ok = det2 > APERTURE_TOL * e * e;
+    if (!ok) {
+#pragma unroll
+        for (int a = 0; a < N; a++) {
+#pragma unroll
+            for (int b = 0; b <= a; b++) M[a][b] = (a == b) ? 1.0 : 0.0;
+            bv[a] = 0;
+        }
+    }
+    double tr = 0;
+#pragma unroll
+    for (int a = 0; a < N; a++) tr += M[a][a];
+#pragma unroll
+    for (int a = 0; a < N; a++) {
+        M[a][a] += RIDGE * tr / (double)N;
+        b0[a] = bv[a];
+    }
+#pragma unroll
+    for (int j = 0; j < N; j++) {
+        double d = M[j][j];
+#pragma unroll
+        for (int k = 0; k < j; k++) d -= M[j][k] * M[j][k];
+        ok = ok && isfinite(d) && d > 0;
+        const double l = sqrt(d);
+        M[j][j] = l;
+#pragma unroll
+        for (int a = j + 1; a < N; a++) {
+            double s = M[a][j];
+#pragma unroll
+            for (int k = 0; k < j; k++) s -= M[a][k] * M[j][k];
+            M[a][j] = s / l;
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < N; a++) {
+        double s = bv[a];
+#pragma unroll
+        for (int k = 0; k < a; k++) s -= M[a][k] * bv[k];
+        bv[a] = s / M[a][a];
+    }
+#pragma unroll
+    for (int a = N - 1; a >= 0; a--) {
+        double s = bv[a];
+#pragma unroll
+        for (int k = a + 1; k < N; k++) s -= M[k][a] * bv[k];
+        bv[a] = s / M[a][a];
+    }
+#pragma unroll
+    for (int a = 0; a < N; a++) ok = ok && isfinite(bv[a]);
+    double *sl = slope + 4 * i;
+    if (ok) {
+        double tb = 0;
+#pragma unroll
+        for (int a = 0; a < N; a++) tb += bv[a] * b0[a];
+        const double rr = mcc - tb;
+        resid[i] = sqrt((rr > 0 ? rr : (isnan(rr) ? rr : 0.0)) / (mw + 1e-30));
+        ux_out[i] = bv[0];
+        uy_out[i] = bv[1];
+        sl[0] = bv[2];
+        sl[1] = bv[3];
+        sl[2] = bv[4];
+        sl[3] = bv[5];
+    } else {
+        resid[i] = M_PI;
+        ux_out[i] = ux_in[i];
+        uy_out[i] = uy_in[i];
+        sl[0] = sl[1] = sl[2] = sl[3] = 0;
+    }
+}
+
 // ------------------------------------------------------------------ exact median (np.median)
-// Radix select over the order-preserving 64-bit keys of the doubles, 8 bits per pass, histograms
-// privatised in LDS. State: prefix (known high bits), rank still to find, NaN count.
+// Selection over the order-preserving 64-bit keys of the doubles: histograms of the top 12 and the
+// next 10 key bits (privatised in LDS), a compaction of the keys that share the selected 22-bit
+// prefix, and a one-workgroup select among those.
+constexpr int MS_B0 = 12, MS_B1 = 10;
+constexpr unsigned int MS_CAP = 8192;
+
 struct SelectState {
-    unsigned long long prefix;
-    unsigned long long rank;
+    unsigned int hist0[1 << MS_B0];  // digit 0: key bits 63 .. 52 (sign and exponent)
+    unsigned int hist1[1 << MS_B1];  // digit 1: key bits 51 .. 42, within the selected digit 0
     unsigned int nan_count;
-    unsigned int pad;
-    unsigned long long below_count;  // elements with key < the selected key
-    unsigned long long below_max;    // largest key below it
+    unsigned int cand;               // candidates: keys sharing the selected 22-bit prefix
+    unsigned long long below_max;    // largest key below that prefix (0: none)
+    unsigned long long cand_keys[MS_CAP];
 };
 
 __device__ __forceinline__ unsigned long long order_key(double v) {
@@ -816,93 +1032,222 @@ __device__ __forceinline__ double from_key(unsigned long long k) {
     return __longlong_as_double((long long)b);
 }
 
-__global__ void k_select_init(SelectState *st, unsigned int *hist, unsigned long long rank) {
-    st->prefix = 0;
-    st->rank = rank;
-    st->nan_count = 0;
-    st->below_count = 0;
-    st->below_max = 0;
-    for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+__global__ void k_msel_init(SelectState *st) {
+    for (int i = threadIdx.x; i < (1 << MS_B0); i += blockDim.x) st->hist0[i] = 0;
+    for (int i = threadIdx.x; i < (1 << MS_B1); i += blockDim.x) st->hist1[i] = 0;
+    if (threadIdx.x == 0) {
+        st->nan_count = 0;
+        st->cand = 0;
+        st->below_max = 0;
+    }
 }
 
-__global__ void __launch_bounds__(256) k_select_hist(const double *__restrict__ v, size_t n, int shift,
-                                                     SelectState *st, unsigned int *hist) {
-    __shared__ unsigned int lh[256];
-    lh[threadIdx.x] = 0;
+// The bin of a global histogram (nb bins, a multiple of the workgroup size, which is at most
+// 1024) holding rank `rank` and the rank left inside it, found by every thread of the workgroup:
+// per-thread partial sums in LDS, then one thread walks them. Every thread must call it.
+__device__ void find_bin(const unsigned int *hist, int nb, unsigned long long rank, int *bin_out,
+                         unsigned long long *rank_out) {
+    __shared__ unsigned long long part[1024];
+    __shared__ int sel_bin;
+    __shared__ unsigned long long sel_rank;
+    const int nt = blockDim.x, per = nb / nt;
+    unsigned long long p = 0;
+    for (int j = 0; j < per; j++) p += hist[threadIdx.x * per + j];
+    part[threadIdx.x] = p;
     __syncthreads();
-    const unsigned long long prefix = st->prefix;
-    const int hi = shift + 8;  // bits above the current digit must equal the prefix
+    if (threadIdx.x == 0) {
+        unsigned long long cum = 0;
+        int t = 0;
+        for (; t < nt - 1; t++) {
+            if (cum + part[t] > rank) break;
+            cum += part[t];
+        }
+        int b = t * per;
+        for (int j = 0; j < per - 1; j++, b++) {
+            const unsigned int c = hist[b];
+            if (cum + c > rank) break;
+            cum += c;
+        }
+        sel_bin = b;
+        sel_rank = rank - cum;
+    }
+    __syncthreads();
+    *bin_out = sel_bin;
+    *rank_out = sel_rank;
+}
+
+// digit 0 histogram (and the NaN count)
+__global__ void __launch_bounds__(256) k_msel_hist0(const double *__restrict__ v, size_t n, SelectState *st) {
+    __shared__ unsigned int lh[1 << MS_B0];
+    for (int i = threadIdx.x; i < (1 << MS_B0); i += blockDim.x) lh[i] = 0;
+    __syncthreads();
     unsigned int nan_local = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const double x = v[i];
-        if (shift == 56 && isnan(x)) nan_local++;
-        const unsigned long long k = order_key(x);
-        if (hi >= 64 || (k >> hi) == (prefix >> hi)) atomicAdd(&lh[(k >> shift) & 255], 1u);
+        nan_local += isnan(x) ? 1u : 0u;
+        atomicAdd(&lh[order_key(x) >> 52], 1u);
     }
     if (nan_local) atomicAdd(&st->nan_count, nan_local);
     __syncthreads();
-    if (lh[threadIdx.x]) atomicAdd(&hist[threadIdx.x], lh[threadIdx.x]);
+    for (int i = threadIdx.x; i < (1 << MS_B0); i += blockDim.x)
+        if (lh[i]) atomicAdd(&st->hist0[i], lh[i]);
 }
 
-__global__ void k_select_pick(SelectState *st, unsigned int *hist, int shift) {
-    if (threadIdx.x != 0) return;
-    unsigned long long rank = st->rank, cum = 0;
-    int d = 0;
-    for (; d < 256; d++) {
-        if (cum + hist[d] > rank) break;
-        cum += hist[d];
-    }
-    if (d == 256) d = 255;
-    st->prefix |= (unsigned long long)d << shift;
-    st->rank = rank - cum;
-    for (int i = 0; i < 256; i++) hist[i] = 0;
-}
-
-__global__ void __launch_bounds__(256) k_select_below(const double *__restrict__ v, size_t n, SelectState *st) {
-    const unsigned long long key = st->prefix;
-    unsigned long long cnt = 0, mx = 0;
+// digit 1 histogram of the keys in the selected digit-0 bin
+__global__ void __launch_bounds__(256) k_msel_hist1(const double *__restrict__ v, size_t n, unsigned long long k,
+                                                    SelectState *st) {
+    __shared__ unsigned int lh[1 << MS_B1];
+    int b0;
+    unsigned long long r0;
+    find_bin(st->hist0, 1 << MS_B0, k, &b0, &r0);
+    for (int i = threadIdx.x; i < (1 << MS_B1); i += blockDim.x) lh[i] = 0;
+    __syncthreads();
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const unsigned long long k = order_key(v[i]);
-        if (k < key) {
-            cnt++;
-            mx = k > mx ? k : mx;
+        const unsigned long long key = order_key(v[i]);
+        if ((int)(key >> 52) == b0) atomicAdd(&lh[(key >> 42) & ((1 << MS_B1) - 1)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < (1 << MS_B1); i += blockDim.x)
+        if (lh[i]) atomicAdd(&st->hist1[i], lh[i]);
+}
+
+// the keys sharing the selected 22-bit prefix go to the candidate list (at most MS_CAP kept, all
+// counted); the largest key below the prefix is kept for the even-count median
+__global__ void __launch_bounds__(256) k_msel_compact(const double *__restrict__ v, size_t n, unsigned long long k,
+                                                      SelectState *st) {
+    int b0, b1;
+    unsigned long long r0, r1;
+    find_bin(st->hist0, 1 << MS_B0, k, &b0, &r0);
+    find_bin(st->hist1, 1 << MS_B1, r0, &b1, &r1);
+    const unsigned long long prefix = ((unsigned long long)b0 << MS_B1) | (unsigned long long)b1;
+    unsigned long long mx = 0;
+    const int lane = threadIdx.x & 63;
+    for (size_t base = (size_t)blockIdx.x * blockDim.x; base < n; base += (size_t)gridDim.x * blockDim.x) {
+        const size_t i = base + threadIdx.x;
+        unsigned long long key = 0;
+        bool cand = false;
+        if (i < n) {
+            key = order_key(v[i]);
+            const unsigned long long p = key >> 42;
+            cand = p == prefix;
+            if (p < prefix && key > mx) mx = key;
+        }
+        const unsigned long long ball = __ballot(cand);
+        if (ball) {
+            unsigned int slot = 0;
+            if (lane == 0) slot = atomicAdd(&st->cand, (unsigned int)__popcll(ball));
+            slot = __shfl(slot, 0);
+            if (cand) {
+                const unsigned int at = slot + (unsigned int)__popcll(ball & ((1ull << lane) - 1ull));
+                if (at < MS_CAP) st->cand_keys[at] = key;
+            }
         }
     }
-    // wavefront reduction, then one atomic per workgroup
-    __shared__ unsigned long long rc[4], rm[4];
     for (int off = 32; off > 0; off >>= 1) {
-        cnt += __shfl_down(cnt, off);
-        unsigned long long o = __shfl_down(mx, off);
+        const unsigned long long o = __shfl_xor(mx, off);
+        mx = o > mx ? o : mx;
+    }
+    if (lane == 0 && mx) atomicMax(&st->below_max, mx);
+}
+
+// Final selection by one workgroup of 1024 threads: the key of rank r1 among the candidates (or,
+// if more than MS_CAP share the prefix, among the keys of the whole array with that prefix),
+// 7-bit digits over the 42 remaining bits with an LDS histogram; then the (k-1)-th for an even
+// count: the largest key below it if exactly r1 candidates lie below, else the key itself.
+__global__ void __launch_bounds__(1024) k_msel_final(const double *__restrict__ v, size_t n, unsigned long long k,
+                                                     int even, SelectState *st, double *out) {
+    __shared__ unsigned int lh[128];
+    __shared__ unsigned long long s_prefix, s_rank;
+    __shared__ unsigned long long red_c[16], red_m[16];
+    int b0, b1;
+    unsigned long long r0, r1;
+    __shared__ int s_b0, s_b1;
+    __shared__ unsigned long long s_r1;
+    find_bin(st->hist0, 1 << MS_B0, k, &b0, &r0);
+    find_bin(st->hist1, 1 << MS_B1, r0, &b1, &r1);
+    if (threadIdx.x == 0) {
+        s_b0 = b0;
+        s_b1 = b1;
+        s_r1 = r1;
+    }
+    __syncthreads();
+    if (st->nan_count) {
+        if (threadIdx.x == 0) out[0] = NAN;
+        return;
+    }
+    const unsigned long long prefix22 = ((unsigned long long)s_b0 << MS_B1) | (unsigned long long)s_b1;
+    const unsigned int m = st->cand;
+    const bool listed = m <= MS_CAP;
+    const size_t count = listed ? m : n;
+    auto key_at = [&](size_t i, bool &ok) {
+        unsigned long long key = listed ? st->cand_keys[i] : order_key(v[i]);
+        ok = (key >> 42) == prefix22;
+        return key;
+    };
+    if (threadIdx.x == 0) {
+        s_prefix = prefix22 << 42;
+        s_rank = s_r1;
+    }
+    __syncthreads();
+    for (int shift = 35; shift >= 0; shift -= 7) {
+        if (threadIdx.x < 128) lh[threadIdx.x] = 0;
+        __syncthreads();
+        const unsigned long long pre = s_prefix;
+        const int hi = shift + 7;
+        for (size_t i = threadIdx.x; i < count; i += blockDim.x) {
+            bool ok;
+            const unsigned long long key = key_at(i, ok);
+            if (ok && (key >> hi) == (pre >> hi)) atomicAdd(&lh[(key >> shift) & 127], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned long long cum = 0, rank = s_rank;
+            int d = 0;
+            for (; d < 127; d++) {
+                if (cum + lh[d] > rank) break;
+                cum += lh[d];
+            }
+            s_prefix = pre | ((unsigned long long)d << shift);
+            s_rank = rank - cum;
+        }
+        __syncthreads();
+    }
+    const unsigned long long K = s_prefix;
+    if (!even) {
+        if (threadIdx.x == 0) out[0] = from_key(K);
+        return;
+    }
+    // candidates below K: their count and largest key
+    unsigned long long c = 0, mx = 0;
+    for (size_t i = threadIdx.x; i < count; i += blockDim.x) {
+        bool ok;
+        const unsigned long long key = key_at(i, ok);
+        if (ok && key < K) {
+            c++;
+            mx = key > mx ? key : mx;
+        }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        c += __shfl_xor(c, off);
+        const unsigned long long o = __shfl_xor(mx, off);
         mx = o > mx ? o : mx;
     }
     if ((threadIdx.x & 63) == 0) {
-        rc[threadIdx.x >> 6] = cnt;
-        rm[threadIdx.x >> 6] = mx;
+        red_c[threadIdx.x >> 6] = c;
+        red_m[threadIdx.x >> 6] = mx;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        unsigned long long c = rc[0] + rc[1] + rc[2] + rc[3];
-        unsigned long long m = max(max(rm[0], rm[1]), max(rm[2], rm[3]));
-        if (c) {
-            atomicAdd(&st->below_count, c);
-            atomicMax(&st->below_max, m);
+        unsigned long long cc = 0, mm = 0;
+        for (int j = 0; j < (int)(blockDim.x >> 6); j++) {
+            cc += red_c[j];
+            mm = red_m[j] > mm ? red_m[j] : mm;
         }
+        const double upper = from_key(K);
+        double lower = upper;
+        if (cc == s_r1) lower = from_key(cc > 0 ? mm : st->below_max);
+        out[0] = (lower + upper) / 2.0;
     }
-}
-
-__global__ void k_select_finish(const SelectState *st, unsigned long long k, int even, double *out) {
-    if (threadIdx.x != 0) return;
-    if (st->nan_count) {
-        out[0] = NAN;
-        return;
-    }
-    const double upper = from_key(st->prefix);
-    if (!even) {
-        out[0] = upper;
-        return;
-    }
-    const double lower = (st->below_count >= k) ? from_key(st->below_max) : upper;
-    out[0] = (lower + upper) / 2.0;
 }
 
 // ------------------------------------------------------------------ confidence
@@ -953,8 +1298,12 @@ void blur(const double *in, double *tmp, float *out, int w, int h, const double 
     if (r > 64) throw Error(OFX_EUNSUPPORTED, "sigma_s above 16 px is not supported by the GPU blur");
     BT bt;
     for (int j = 0; j <= 2 * r; j++) bt.k[j] = taps_host[j];
-    hipLaunchKernelGGL(k_blur_y, grid2(w, h), dim3(256), 0, s, in, tmp, w, h, bt, r);
-    hipLaunchKernelGGL(k_blur_x, grid2(w, h), dim3(256), 0, s, tmp, out, w, bt, r);
+    if (r == 16)
+        hipLaunchKernelGGL(k_blur_y_rb<16>, dim3((w + 255) / 256, (h + BLUR_ROWS - 1) / BLUR_ROWS), dim3(256), 0, s, in,
+                           tmp, w, h, bt);
+    else
+        hipLaunchKernelGGL(k_blur_y, grid2(w, h), dim3(256), 0, s, in, tmp, w, h, bt, r);
+    hipLaunchKernelGGL(k_blur_x_lds, grid2(w, h), dim3(256), 0, s, tmp, out, w, bt, r);
     LAUNCHCHECK();
 }
 
@@ -1009,6 +1358,20 @@ void moments_solve(const double *sums, double *ym, const double *ux_in, const do
     LAUNCHCHECK();
 }
 
+size_t moments2_scratch_doubles(size_t n) { return (size_t)(O2_YPLANES + O2_MOMENTS) * n; }
+
+void moments_solve2(const double *sums, double *scratch, const double *ux_in, const double *uy_in, double *ux_out,
+                    double *uy_out, double *energy, double *resid, double *slope, int w, int h, const double *kt, int r,
+                    hipStream_t s) {
+    const size_t n = (size_t)w * h;
+    double *ym = scratch, *mom = scratch + (size_t)O2_YPLANES * n;
+    hipLaunchKernelGGL(k_moments2_y, grid2(w, h), dim3(256), 0, s, sums, ym, w, h, kt, r);
+    hipLaunchKernelGGL(k_moments2_x, dim3((w + 127) / 128, h), dim3(128), 0, s, (const double *)ym, mom, w, h, kt, r);
+    hipLaunchKernelGGL(k_solve2, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, (const double *)mom, ux_in, uy_in,
+                       ux_out, uy_out, energy, resid, slope, n);
+    LAUNCHCHECK();
+}
+
 void energy_reset(const double *energy, const double *med, const double *ux_in, const double *uy_in, double *ux,
                   double *uy, double *resid, double *slope, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_energy_reset, dim3((n + 255) / 256), dim3(256), 0, s, energy, med, ux_in, uy_in, ux, uy,
@@ -1016,21 +1379,21 @@ void energy_reset(const double *energy, const double *med, const double *ux_in, 
     LAUNCHCHECK();
 }
 
-size_t median_scratch_bytes() { return sizeof(SelectState) + 256 * sizeof(unsigned int); }
+size_t median_scratch_bytes() { return sizeof(SelectState); }
 
+// numpy.median of n values (NaN if any is NaN): the n/2-th order statistic by a 12-bit and a
+// 10-bit histogram pass over the order-preserving keys, a compaction of the keys sharing the
+// selected 22-bit prefix, and a one-workgroup select among them; for an even n the mean with the
+// (n/2 - 1)-th (core/select.rs:19-42).
 void median(const double *v, size_t n, void *scratch, double *out, hipStream_t s) {
     SelectState *st = (SelectState *)scratch;
-    unsigned int *hist = (unsigned int *)((char *)scratch + sizeof(SelectState));
     const unsigned long long k = n / 2;
-    hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, hist, k);
-    const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
-    for (int shift = 56; shift >= 0; shift -= 8) {
-        hipLaunchKernelGGL(k_select_hist, dim3(blocks), dim3(256), 0, s, v, n, shift, st, hist);
-        hipLaunchKernelGGL(k_select_pick, dim3(1), dim3(64), 0, s, st, hist, shift);
-    }
-    const int even = (n % 2 == 0) ? 1 : 0;
-    if (even) hipLaunchKernelGGL(k_select_below, dim3(blocks), dim3(256), 0, s, v, n, st);
-    hipLaunchKernelGGL(k_select_finish, dim3(1), dim3(64), 0, s, st, k, even, out);
+    const int blocks = (int)std::min<size_t>(512, (n + 255) / 256);
+    hipLaunchKernelGGL(k_msel_init, dim3(1), dim3(1024), 0, s, st);
+    hipLaunchKernelGGL(k_msel_hist0, dim3(blocks), dim3(256), 0, s, v, n, st);
+    hipLaunchKernelGGL(k_msel_hist1, dim3(blocks), dim3(256), 0, s, v, n, k, st);
+    hipLaunchKernelGGL(k_msel_compact, dim3(blocks), dim3(256), 0, s, v, n, k, st);
+    hipLaunchKernelGGL(k_msel_final, dim3(1), dim3(1024), 0, s, v, n, k, (n % 2 == 0) ? 1 : 0, st, out);
     LAUNCHCHECK();
 }
 

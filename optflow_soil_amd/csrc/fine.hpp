@@ -19,6 +19,12 @@ void accumulate(const double *ux, const double *uy, const float *sx, const float
 void moments_solve(const double *sums, double *ym, const double *ux_in, const double *uy_in, double *ux_out,
                    double *uy_out, double *energy, double *resid, double *slope, int w, int h, const double *kt,
                    const double *kt_host, int r, hipStream_t s);
+// shape_order 2: the same outputs from the 12 x 12 system; scratch of moments2_scratch_doubles(n)
+// doubles; kt holds the taps times t^0 .. t^4
+size_t moments2_scratch_doubles(size_t n);
+void moments_solve2(const double *sums, double *scratch, const double *ux_in, const double *uy_in, double *ux_out,
+                    double *uy_out, double *energy, double *resid, double *slope, int w, int h, const double *kt, int r,
+                    hipStream_t s);
 void energy_reset(const double *energy, const double *med, const double *ux_in, const double *uy_in, double *ux,
                   double *uy, double *resid, double *slope, size_t n, hipStream_t s);
 size_t median_scratch_bytes();
@@ -47,8 +53,8 @@ void large_components(const uint8_t *b, int w, int h, double min_area, int *pare
 // guard sources: large low components or outside the ROI
 void guard_sources(const uint8_t *large, const uint8_t *mask, int w, int h, uint8_t *src, hipStream_t s);
 // validity (track.py:56-108) and NaN-masked displacement as double planes for the strain
-void validity(const float *conf, double conf_min, const uint8_t *mask, const uint8_t *near_src, double guard,
-              int guard_on, const double *ux, const double *uy, int w, int h, double *mx, double *my, hipStream_t s);
+void validity(const uint8_t *low, const uint8_t *mask, const uint8_t *near_src, double guard, int guard_on,
+              const double *ux, const double *uy, int w, int h, double *mx, double *my, hipStream_t s);
 // strain (strain.py:61-112) from the NaN-masked field; writes the outputs of a result
 void strain(const double *mx, const double *my, int w, int h, int win, double *ytmp, float *ux_out, float *uy_out,
             uint8_t *valid, float *exx, float *eyy, float *exy, float *gmax, float *vol, hipStream_t s);
@@ -66,5 +72,38 @@ void compose(const double *ax, const double *ay, const double *bx, const double 
 void conf_at(const double *conf, const double *ux, const double *uy, int w, int h, double *out, hipStream_t s);
 void mean_conf(const double *conf, const double *raw_ux, const double *raw_uy, const double *ux, const double *uy,
                const uint8_t *mask, double B, int w, int h, double *acc /* [sum, count] */, hipStream_t s);
+
+// hybrid.hip: seq_mode "hybrid" (track/hybrid.rs) -- per-pixel flags and counters
+enum : uint8_t {
+    HYB_DIRECT_OK = 1, HYB_GOOD = 2, HYB_CAND_RE = 4, HYB_NEED = 8,
+    HYB_HAS_FB = 16, HYB_RE_NOFB = 32, HYB_RE = 64, HYB_FB_CAND = 128
+};
+enum {
+    HYB_ANY_NEED = 0, HYB_ANY_RE_NOFB, HYB_ANY_CAND, HYB_NFB_MAX, HYB_N_FB_OK, HYB_N_REJECTED, HYB_N_RE, HYB_N_ROI,
+    HYB_NCOUNTERS
+};
+void hyb_zero_counters(int *counters, hipStream_t s);
+void hyb_good_of(const uint8_t *flags, uint8_t *good, size_t n, hipStream_t s);
+void hyb_refmove(const uint8_t *good_prev, const double *cp, double *c0r, uint8_t *anch, size_t n, hipStream_t s);
+void hyb_classify(const double *cx, const double *cy, const double *leg_c, const double *c0r, const uint8_t *anch,
+                  const uint8_t *mask, const double *upx, const double *upy, const int *nfb, double conf_min,
+                  int fb_max, int reacq_ok, double *link, uint8_t *flags, int *counters, size_t n, hipStream_t s);
+void hyb_carried(const double *cax, const double *cay, const double *upx, const double *upy, double *ax, double *ay,
+                 size_t n, hipStream_t s);
+void hyb_has_fb(uint8_t *flags, const double *ca, double conf_min, const uint8_t *mask, int *counters, size_t n,
+                hipStream_t s);
+void hyb_reacquire(uint8_t *flags, const uint8_t *gate0, const double *cx, const double *cy, const double *ax,
+                   const double *ay, const double *leg_c, const double *ca, double fb_tol, double conf_min, int fb_max,
+                   double *c0r, uint8_t *anch, int *nfb, const uint8_t *mask, int *counters, size_t n, hipStream_t s);
+void hyb_output(const uint8_t *flags, const uint8_t *gate, const double *cx, const double *cy, const double *ax,
+                const double *ay, const double *link, const double *ca, const int *nfb, int fb_max,
+                const uint8_t *mask, double *ux, double *uy, float *conf, uint8_t *low, int32_t *fb, int *counters,
+                size_t n, hipStream_t s);
+// the local affine gate (hybrid.rs:147-260) of the direct-good ROI pixels (flags & HYB_GOOD,
+// mask) at targets (tx, ty); scratch of gate_scratch_bytes
+size_t gate_scratch_bytes(int w, int h);
+void hyb_gate(const uint8_t *flags, const uint8_t *mask, const double *cx, const double *cy, const double *tx,
+              const double *ty, int w, int h, double radius, long min_support, double tol, void *scratch,
+              uint8_t *ok, hipStream_t s);
 
 }  // namespace ofx

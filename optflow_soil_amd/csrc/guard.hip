@@ -266,14 +266,14 @@ __global__ void k_low(const float *__restrict__ conf, double conf_min, size_t n,
 
 // track.py:66-98: invalid = low conf, outside the ROI, within the guard of a source, or within the
 // guard of the image border
-__global__ void k_validity(const float *__restrict__ conf, double conf_min, const uint8_t *__restrict__ mask,
+__global__ void k_validity(const uint8_t *__restrict__ low, const uint8_t *__restrict__ mask,
                            const uint8_t *__restrict__ near_src, double guard, int guard_on,
                            const double *__restrict__ ux, const double *__restrict__ uy, int w, int h,
                            double *__restrict__ mx, double *__restrict__ my) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= w) return;
     const size_t i = (size_t)y * w + x;
-    bool bad = (double)conf[i] < conf_min || (mask && !mask[i]);
+    bool bad = low[i] || (mask && !mask[i]);
     if (guard_on) {
         bad = bad || near_src[i];
         const int bd = min(min(x, y), min(w - 1 - x, h - 1 - y));
@@ -490,10 +490,10 @@ void low_mask(const float *conf, double conf_min, size_t n, uint8_t *low, hipStr
     LAUNCHCHECK();
 }
 
-void validity(const float *conf, double conf_min, const uint8_t *mask, const uint8_t *near_src, double guard,
-              int guard_on, const double *ux, const double *uy, int w, int h, double *mx, double *my, hipStream_t s) {
-    hipLaunchKernelGGL(k_validity, grid2(w, h), dim3(256), 0, s, conf, conf_min, mask, near_src, guard, guard_on, ux,
-                       uy, w, h, mx, my);
+void validity(const uint8_t *low, const uint8_t *mask, const uint8_t *near_src, double guard, int guard_on,
+              const double *ux, const double *uy, int w, int h, double *mx, double *my, hipStream_t s) {
+    hipLaunchKernelGGL(k_validity, grid2(w, h), dim3(256), 0, s, low, mask, near_src, guard, guard_on, ux, uy, w, h, mx,
+                       my);
     LAUNCHCHECK();
 }
 

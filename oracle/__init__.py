@@ -44,7 +44,9 @@ class _Result(ctypes.Structure):
     _fields_ = [(k, ctypes.POINTER(ctypes.c_double)) for k in
                 ("ux", "uy", "conf", "raw_ux", "raw_uy", "exx", "eyy", "exy", "gmax", "vol")] + [
         ("fb", ctypes.POINTER(ctypes.c_int32)), ("t_coarse", ctypes.c_double), ("t_fine", ctypes.c_double),
-        ("t_strain", ctypes.c_double), ("reference", ctypes.c_int), ("mean_conf", ctypes.c_double)]
+        ("t_strain", ctypes.c_double), ("reference", ctypes.c_int), ("mean_conf", ctypes.c_double),
+        ("n_fb_max", ctypes.c_int), ("fallback_frac", ctypes.c_double), ("fb_gate_rejected", ctypes.c_double),
+        ("reacquired", ctypes.c_double)]
 
 
 def lib():
@@ -256,7 +258,8 @@ def _new_result(h, w):
     r = _Result()
     for k in _RES_KEYS:
         setattr(r, k, arrs[k].ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
-    r.fb = None
+    arrs["fb"] = np.zeros((h, w), np.int32)
+    r.fb = arrs["fb"].ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
     return r, arrs
 
 
@@ -264,7 +267,8 @@ def _as_dict(r, arrs):
     out = dict(arrs)
     out["Exx"], out["Eyy"], out["Exy"] = out.pop("exx"), out.pop("eyy"), out.pop("exy")
     out["timing"] = {"coarse": r.t_coarse, "fine": r.t_fine, "strain": r.t_strain, "reference": r.reference,
-                     "mean_conf": r.mean_conf}
+                     "mean_conf": r.mean_conf, "n_fb_max": r.n_fb_max, "fallback_frac": r.fallback_frac,
+                     "fb_gate_rejected": r.fb_gate_rejected, "reacquired": r.reacquired}
     return out
 
 
@@ -281,6 +285,25 @@ def track_pair(i1, i2, cfg, mask=None, init=None):
                                 _p(ix) if ix is not None else None, _p(iy) if iy is not None else None,
                                 ctypes.byref(c), ctypes.byref(r)))
     return _as_dict(r, arrs)
+
+
+def box_sum(a, r):
+    """Sum over the (2r + 1)^2 window, zero outside (rust/crates/optflow-core/src/track/hybrid.rs:67-112)."""
+    a = _d(a)
+    out = np.empty_like(a)
+    lib().orc_box_sum(_p(a), a.shape[0], a.shape[1], int(r), _p(out))
+    return out
+
+
+def local_affine_agreement(ux, uy, support, tx, ty, radius, min_support, tol):
+    """The hybrid gate (rust/crates/optflow-core/src/track/hybrid.rs:147-260)."""
+    ux, uy, tx, ty = _d(ux), _d(uy), _d(tx), _d(ty)
+    sup = np.ascontiguousarray(support, dtype=np.uint8)
+    ok = np.empty(ux.shape, np.uint8)
+    lib().orc_local_affine_agreement(_p(ux), _p(uy), _p(sup), _p(tx), _p(ty), ux.shape[0], ux.shape[1],
+                                     ctypes.c_double(radius), ctypes.c_long(int(min_support)), ctypes.c_double(tol),
+                                     _p(ok))
+    return ok.astype(bool)
 
 
 def track_sequence(frames, cfg, mask=None):

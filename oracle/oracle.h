@@ -108,8 +108,16 @@ typedef struct orc_result {
     double t_coarse, t_fine, t_strain;
     int reference;
     double mean_conf;
+    /* hybrid only: timing shares of track/hybrid.rs:520-531 */
+    int n_fb_max;
+    double fallback_frac, fb_gate_rejected, reacquired;
 } orc_result;
 
+/* hybrid helpers: zero-padded (2r+1)^2 window sums and the local affine gate (track/hybrid.rs) */
+void orc_box_sum(const double *a, int h, int w, int r, double *out);
+void orc_local_affine_agreement(const double *ux, const double *uy, const uint8_t *support, const double *tx,
+                                const double *ty, int h, int w, double radius, long min_support, double tol,
+                                uint8_t *ok);
 int orc_track_pair(const double *i1, const double *i2, int h, int w, const uint8_t *mask,
                    const double *init_ux, const double *init_uy, const orc_cfg *cfg,
                    orc_result *res);

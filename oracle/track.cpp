@@ -1,9 +1,11 @@
 // Pair and sequence pipeline of the oracle (test infrastructure only): coarse stage, fine stage,
-// guard and strain (python/optflow_ref/track.py:45-146), and the chained sequence with reference
-// updates, composition and link confidence (track.py:231-328).
+// guard and strain (python/optflow_ref/track.py:45-146), the chained sequence with reference
+// updates, composition and link confidence (track.py:231-328), and the hybrid sequence
+// (rust/crates/optflow-core/src/track/hybrid.rs, track.py:331-505).
 #include <omp.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 
 #include "common.hpp"
@@ -235,6 +237,330 @@ int sequence_chain(const double *frames, int nf, int h, int w, const uint8_t *ma
     return 0;
 }
 
+
+// ---------------------------------------------------------------- seq_mode "hybrid"
+// rust/crates/optflow-core/src/track/hybrid.rs (python/optflow_ref/track.py:331-505)
+
+// Sum over the (2r + 1)^2 window, zero outside the image, by running sums along rows then columns
+// (hybrid.rs:67-112, the order of its additions).
+std::vector<double> box_sum(const std::vector<double> &a, int w, int h, int r) {
+    std::vector<double> rows(a.size()), out(a.size());
+    for (int y = 0; y < h; y++) {
+        const double *in = a.data() + (size_t)y * w;
+        double *o = rows.data() + (size_t)y * w;
+        double s = 0.0;
+        for (int x = 0; x < std::min(r, w); x++) s += in[x];
+        for (int x = 0; x < w; x++) {
+            if (x + r < w) s += in[x + r];
+            if (x > r) s -= in[x - r - 1];
+            o[x] = s;
+        }
+    }
+    std::vector<double> acc(w, 0.0);
+    for (int y = 0; y < std::min(r, h); y++)
+        for (int x = 0; x < w; x++) acc[x] += rows[(size_t)y * w + x];
+    for (int y = 0; y < h; y++) {
+        if (y + r < h)
+            for (int x = 0; x < w; x++) acc[x] += rows[(size_t)(y + r) * w + x];
+        if (y > r)
+            for (int x = 0; x < w; x++) acc[x] -= rows[(size_t)(y - r - 1) * w + x];
+        std::copy(acc.begin(), acc.end(), out.begin() + (size_t)y * w);
+    }
+    return out;
+}
+
+// m c = b for two right-hand sides, Gaussian elimination with partial pivoting (the first row of
+// largest |pivot| wins ties as Rust's max_by keeps the last maximum); false if singular
+// (hybrid.rs:114-145).
+bool solve3(double m[3][3], double b[3][2], double x[3][2]) {
+    double a[3][5];
+    for (int i = 0; i < 3; i++) {
+        for (int j = 0; j < 3; j++) a[i][j] = m[i][j];
+        a[i][3] = b[i][0];
+        a[i][4] = b[i][1];
+    }
+    for (int c = 0; c < 3; c++) {
+        int p = c;
+        for (int i = c + 1; i < 3; i++)
+            if (!(std::fabs(a[i][c]) < std::fabs(a[p][c]))) p = i;  // max_by: the last of equal maxima
+        if (a[p][c] == 0.0 || !std::isfinite(a[p][c])) return false;
+        for (int j = 0; j < 5; j++) std::swap(a[c][j], a[p][j]);
+        for (int i = c + 1; i < 3; i++) {
+            const double f = a[i][c] / a[c][c];
+            for (int j = c; j < 5; j++) a[i][j] -= f * a[c][j];
+        }
+    }
+    for (int k = 0; k < 2; k++)
+        for (int i = 2; i >= 0; i--) {
+            double s = a[i][3 + k];
+            for (int j = i + 1; j < 3; j++) s -= a[i][j] * x[j][k];
+            x[i][k] = s / a[i][i];
+        }
+    return true;
+}
+
+// hybrid.rs:147-260 (_local_affine_agreement, track.py:331-380)
+std::vector<uint8_t> local_affine_agreement(int w, int h, const std::vector<double> &ux, const std::vector<double> &uy,
+                                            const std::vector<uint8_t> &support, const std::vector<double> &tx,
+                                            const std::vector<double> &ty, double radius, long min_support,
+                                            double tol) {
+    const size_t n = (size_t)w * h;
+    const int r = std::max<long>(1, (long)std::nearbyint(radius));  // round half to even
+    const double ridge = 1e-3 * (double)(r * r);
+    std::vector<double> ux0(n), uy0(n);
+    for (size_t i = 0; i < n; i++) {
+        ux0[i] = support[i] ? ux[i] : 0.0;
+        uy0[i] = support[i] ? uy[i] : 0.0;
+    }
+    std::vector<uint8_t> sel = support, ok_fit(n, 0);
+    std::vector<double> ax(n, NAN), ay(n, NAN), s1;
+    auto xs = [&](size_t i) { return (double)(i % w); };
+    auto ys = [&](size_t i) { return (double)(i / w); };
+    for (int pass = 0; pass < 2; pass++) {
+        std::vector<double> wt(n);
+        for (size_t i = 0; i < n; i++) wt[i] = sel[i] ? 1.0 : 0.0;
+        auto prod = [&](auto f) {
+            std::vector<double> v(n);
+            for (size_t i = 0; i < n; i++) v[i] = f(i);
+            return box_sum(v, w, h, r);
+        };
+        s1 = box_sum(wt, w, h, r);
+        const auto sx = prod([&](size_t i) { return wt[i] * xs(i); });
+        const auto sy = prod([&](size_t i) { return wt[i] * ys(i); });
+        const auto sxx = prod([&](size_t i) { return wt[i] * xs(i) * xs(i); });
+        const auto sxy = prod([&](size_t i) { return wt[i] * xs(i) * ys(i); });
+        const auto syy = prod([&](size_t i) { return wt[i] * ys(i) * ys(i); });
+        std::vector<double> rhs[6];
+        for (int k = 0; k < 2; k++) {
+            const auto &comp = k ? uy0 : ux0;
+            rhs[3 * k] = prod([&](size_t i) { return wt[i] * comp[i]; });
+            rhs[3 * k + 1] = prod([&](size_t i) { return wt[i] * xs(i) * comp[i]; });
+            rhs[3 * k + 2] = prod([&](size_t i) { return wt[i] * ys(i) * comp[i]; });
+        }
+#pragma omp parallel for schedule(static)
+        for (long li = 0; li < (long)n; li++) {
+            const size_t i = (size_t)li;
+            if (s1[i] < 3.0) {
+                ax[i] = ay[i] = NAN;
+                ok_fit[i] = 0;
+                continue;
+            }
+            const double x = xs(i), y = ys(i);
+            const double sxc = sx[i] - x * s1[i], syc = sy[i] - y * s1[i];
+            const double sxxc = sxx[i] - 2.0 * x * sx[i] + x * x * s1[i];
+            const double sxyc = sxy[i] - x * sy[i] - y * sx[i] + x * y * s1[i];
+            const double syyc = syy[i] - 2.0 * y * sy[i] + y * y * s1[i];
+            double m[3][3] = {{s1[i], sxc, syc}, {sxc, sxxc + ridge * s1[i], sxyc}, {syc, sxyc, syyc + ridge * s1[i]}};
+            double b[3][2], c[3][2];
+            for (int k = 0; k < 2; k++) {
+                const double su = rhs[3 * k][i];
+                b[0][k] = su;
+                b[1][k] = rhs[3 * k + 1][i] - x * su;
+                b[2][k] = rhs[3 * k + 2][i] - y * su;
+            }
+            if (solve3(m, b, c)) {
+                ax[i] = c[0][0];
+                ay[i] = c[0][1];
+            } else {
+                ax[i] = ay[i] = NAN;
+            }
+            ok_fit[i] = 1;
+        }
+        if (pass == 0)
+            for (size_t i = 0; i < n; i++)
+                sel[i] = support[i] && std::hypot(ux[i] - ax[i], uy[i] - ay[i]) <= 3.0 * tol;
+    }
+    std::vector<uint8_t> ok(n);
+    for (size_t i = 0; i < n; i++)
+        ok[i] = s1[i] + 0.5 >= (double)min_support && ok_fit[i] && std::hypot(tx[i] - ax[i], ty[i] - ay[i]) <= tol;
+    return ok;
+}
+
+// hybrid.rs:262-540 (track.py:383-505)
+int sequence_hybrid(const double *frames, int nf, int h, int w, const uint8_t *mask, const orc_cfg &c,
+                    orc_result *res) {
+    const size_t n = (size_t)h * w;
+    std::vector<uint8_t> roi(n, 1);
+    if (mask)
+        for (size_t i = 0; i < n; i++) roi[i] = mask[i] != 0;
+    size_t n_roi = 0;
+    for (size_t i = 0; i < n; i++) n_roi += roi[i];
+    const double *ref = frames, *prev = frames;
+    int ref_idx = 0;
+    std::vector<double> u0x(n, 0.0), u0y(n, 0.0), c0r(n, 1.0);
+    std::vector<uint8_t> anch(n, 1), good_prev(n, 1);
+    std::vector<double> upx(n, 0.0), upy(n, 0.0);
+    std::vector<long> nfb(n, 0);
+    Pair prev_res;
+    bool have_prev = false;
+    std::vector<double> prev_adj_x, prev_adj_y;
+    bool have_prev_adj = false;
+    const long cap = (long)c.fb_max + 1;
+    const double conf_min = c.conf_min, radius = Cfg(c).fb_radius_px();
+    const bool gate_on = c.fb_min_support > 0;
+    for (int k = 1; k < nf; k++) {
+        const double *fk = frames + (size_t)k * n;
+        Pair leg;
+        int rc = pair(ref, fk, h, w, nullptr, have_prev ? prev_res.raw_ux.data() : nullptr,
+                      have_prev ? prev_res.raw_uy.data() : nullptr, c, leg);
+        if (rc) return rc;
+        double cm = mean_conf(leg, u0x, u0y, mask, h, w, c);
+        if (cm < c.ref_update_conf && ref_idx != k - 1) {
+            // c0K is replaced (not min-accumulated) where the pixel was direct-good at k - 1, 0 elsewhere
+            const std::vector<double> cp = conf_at(prev_res.conf, u0x, u0y, h, w);
+            for (size_t i = 0; i < n; i++) c0r[i] = good_prev[i] ? cp[i] : 0.0;
+            anch = good_prev;
+            u0x = upx;
+            u0y = upy;
+            ref = prev;
+            ref_idx = k - 1;
+            leg = Pair();
+            rc = pair(ref, fk, h, w, nullptr, nullptr, nullptr, c, leg);
+            if (rc) return rc;
+            cm = mean_conf(leg, u0x, u0y, mask, h, w, c);
+        }
+        std::vector<double> cx, cy;
+        compose(u0x, u0y, leg.raw_ux, leg.raw_uy, h, w, cx, cy);
+        const std::vector<double> leg_c = conf_at(leg.conf, u0x, u0y, h, w);
+        std::vector<double> link(n);
+        std::vector<uint8_t> direct_ok(n), good(n), cand_re(n), need(n);
+        const bool reacq_ok = ref_idx + 1 < k;
+        bool any_need = false;
+        for (size_t i = 0; i < n; i++) {
+            link[i] = anch[i] ? std::min(c0r[i], leg_c[i]) : leg_c[i];
+            direct_ok[i] = std::isfinite(cx[i]) && std::isfinite(cy[i]) && link[i] >= conf_min;
+            good[i] = direct_ok[i] && anch[i];
+            cand_re[i] = direct_ok[i] && !anch[i] && roi[i] && reacq_ok;
+            need[i] = !good[i] && roi[i] && std::isfinite(upx[i]) && std::isfinite(upy[i]) && nfb[i] < (long)c.fb_max;
+            any_need |= need[i] != 0;
+        }
+        double t_coarse = leg.t_coarse, t_fine = leg.t_fine;
+        std::vector<double> ax, ay, ca(n, 0.0), araw_x, araw_y;
+        bool have_adj = false;
+        if (any_need) {
+            const std::vector<double> *rx, *ry, *rconf;
+            Pair adj;
+            if (ref_idx == k - 1) {
+                rx = &leg.raw_ux;
+                ry = &leg.raw_uy;
+                rconf = &leg.conf;
+            } else {
+                rc = pair(prev, fk, h, w, nullptr, have_prev_adj ? prev_adj_x.data() : nullptr,
+                          have_prev_adj ? prev_adj_y.data() : nullptr, c, adj);
+                if (rc) return rc;
+                t_coarse += adj.t_coarse;
+                t_fine += adj.t_fine;
+                rx = &adj.raw_ux;
+                ry = &adj.raw_uy;
+                rconf = &adj.conf;
+            }
+            std::vector<double> cax, cay;
+            compose(upx, upy, *rx, *ry, h, w, cax, cay);
+            ca = conf_at(*rconf, upx, upy, h, w);
+            ax.resize(n);
+            ay.resize(n);
+            for (size_t i = 0; i < n; i++) {
+                const bool fin = std::isfinite(cax[i]) && std::isfinite(cay[i]);
+                ax[i] = fin ? cax[i] : upx[i];
+                ay[i] = fin ? cay[i] : upy[i];
+            }
+            araw_x = *rx;
+            araw_y = *ry;
+            have_adj = true;
+        } else {
+            ax = upx;
+            ay = upy;
+        }
+        std::vector<uint8_t> has_fb(n), re_nofb(n), re(n), fb_cand(n), fb_ok(n);
+        bool any_re_nofb = false;
+        for (size_t i = 0; i < n; i++) {
+            has_fb[i] = need[i] && ca[i] >= conf_min;
+            re_nofb[i] = cand_re[i] && !has_fb[i];
+            any_re_nofb |= re_nofb[i] && roi[i];
+        }
+        auto masked = [&](const std::vector<double> &v) {
+            std::vector<double> o(n);
+            for (size_t i = 0; i < n; i++) o[i] = good[i] ? v[i] : NAN;
+            return o;
+        };
+        auto support = [&]() {
+            std::vector<uint8_t> s(n);
+            for (size_t i = 0; i < n; i++) s[i] = good[i] && roi[i];
+            return s;
+        };
+        if (gate_on && any_re_nofb) {
+            const auto gate0 = local_affine_agreement(w, h, masked(cx), masked(cy), support(), cx, cy, radius,
+                                                      c.fb_min_support, c.fb_tol);
+            for (size_t i = 0; i < n; i++) re_nofb[i] = re_nofb[i] && gate0[i];
+        }
+        for (size_t i = 0; i < n; i++) {
+            const bool agree = std::hypot(cx[i] - ax[i], cy[i] - ay[i]) <= c.fb_tol;
+            re[i] = (cand_re[i] && has_fb[i] && agree) || re_nofb[i];
+            if (re[i]) {
+                c0r[i] = leg_c[i];
+                anch[i] = 1;
+                good[i] = 1;
+            }
+        }
+        bool any_cand = false;
+        for (size_t i = 0; i < n; i++) {
+            nfb[i] = good[i] ? 0 : std::min(nfb[i] + 1, cap);
+            fb_cand[i] = !good[i] && need[i] && nfb[i] <= (long)c.fb_max && ca[i] >= conf_min;
+            any_cand |= fb_cand[i] && roi[i];
+        }
+        if (gate_on && any_cand) {
+            const auto gate = local_affine_agreement(w, h, masked(cx), masked(cy), support(), ax, ay, radius,
+                                                     c.fb_min_support, c.fb_tol);
+            for (size_t i = 0; i < n; i++) fb_ok[i] = fb_cand[i] && gate[i];
+        } else {
+            fb_ok = fb_cand;
+        }
+        std::vector<double> ux(n), uy(n), conf(n), conf_in(n);
+        long nmax = 0;
+        size_t n_fb_ok = 0, n_rej = 0, n_re = 0;
+        for (size_t i = 0; i < n; i++) {
+            ux[i] = good[i] ? cx[i] : ax[i];
+            uy[i] = good[i] ? cy[i] : ay[i];
+            conf[i] = good[i] ? link[i] : ca[i] * std::max(0.0, 1.0 - (double)nfb[i] / (double)cap);
+            // the low mask of finish: neither direct-good nor an accepted fallback
+            conf_in[i] = (good[i] || fb_ok[i]) ? 1.0 : -1.0;
+            nmax = std::max(nmax, nfb[i]);
+            if (roi[i]) {
+                n_fb_ok += fb_ok[i];
+                n_rej += fb_cand[i] && !fb_ok[i];
+                n_re += re[i];
+            }
+        }
+        Pair out;
+        finish(ux.data(), uy.data(), conf_in.data(), h, w, c, mask, out);
+        out.conf = conf;
+        out.raw_ux = ux;
+        out.raw_uy = uy;
+        out.t_coarse = t_coarse;
+        out.t_fine = t_fine;
+        out.t_strain = leg.t_strain;
+        store(out, n, &res[k - 1]);
+        if (res[k - 1].fb)
+            for (size_t i = 0; i < n; i++) res[k - 1].fb[i] = (int32_t)nfb[i];
+        res[k - 1].reference = ref_idx;
+        res[k - 1].mean_conf = cm;
+        res[k - 1].n_fb_max = (int)nmax;
+        res[k - 1].fallback_frac = (double)n_fb_ok / (double)n_roi;
+        res[k - 1].fb_gate_rejected = (double)n_rej / (double)n_roi;
+        res[k - 1].reacquired = (double)n_re / (double)n_roi;
+        upx = ux;
+        upy = uy;
+        good_prev = good;
+        prev_adj_x = araw_x;
+        prev_adj_y = araw_y;
+        have_prev_adj = have_adj;
+        prev = fk;
+        prev_res = leg;
+        have_prev = true;
+    }
+    return 0;
+}
+
 }  // namespace
 
 }  // namespace orc
@@ -256,8 +582,24 @@ extern "C" int orc_track_pair(const double *i1, const double *i2, int h, int w, 
 extern "C" int orc_track_sequence(const double *frames, int n, int h, int w, const uint8_t *mask,
                                   const orc_cfg *cfg, orc_result *res) {
     if (n < 1) { set_error("no frames"); return 1; }
-    if (cfg->seq_hybrid) { set_error("hybrid sequences are not in the oracle yet"); return 1; }
+    if (cfg->seq_hybrid) return sequence_hybrid(frames, n, h, w, mask, *cfg, res);
     return sequence_chain(frames, n, h, w, mask, *cfg, res);
+}
+
+extern "C" void orc_box_sum(const double *a, int h, int w, int r, double *out) {
+    std::vector<double> v(a, a + (size_t)h * w);
+    const auto o = box_sum(v, w, h, r);
+    std::copy(o.begin(), o.end(), out);
+}
+
+extern "C" void orc_local_affine_agreement(const double *ux, const double *uy, const uint8_t *support, const double *tx,
+                                           const double *ty, int h, int w, double radius, long min_support, double tol,
+                                           uint8_t *ok) {
+    const size_t n = (size_t)h * w;
+    const auto o = local_affine_agreement(w, h, std::vector<double>(ux, ux + n), std::vector<double>(uy, uy + n),
+                                          std::vector<uint8_t>(support, support + n), std::vector<double>(tx, tx + n),
+                                          std::vector<double>(ty, ty + n), radius, min_support, tol);
+    std::copy(o.begin(), o.end(), ok);
 }
 
 extern "C" void orc_set_threads(int n) {

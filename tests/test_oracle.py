@@ -335,3 +335,63 @@ def test_refine_shifted_window_matches_full_frame():
     # the same window with the shift's sign flipped lands elsewhere
     bad = O.refine(A, win, ux0, uy0, cfg, shift=(x0, y0))
     assert np.abs(bad[0][inner] - full[0][inner]).max() > 0.1
+
+
+def test_box_sum_is_the_zero_padded_window_sum():
+    """rust/crates/optflow-core/src/track/hybrid.rs:548-568."""
+    h, w = 7, 9
+    a = (((np.arange(h * w) * 37) % 11) - 4.0).reshape(h, w)
+    for r in (1, 2, 3, 20):
+        got = O.box_sum(a, r)
+        for y in range(h):
+            for x in range(w):
+                want = a[max(0, y - r):y + r + 1, max(0, x - r):x + r + 1].sum()
+                assert abs(got[y, x] - want) < 1e-9
+
+
+def test_local_affine_gate_compares_with_the_model_of_the_support():
+    """A plane of displacement on the support with a hole in the middle
+    (rust/crates/optflow-core/src/track/hybrid.rs:570-620): the model predicts the hole; a target
+    off the plane by more than tol fails, one within passes; too few support pixels fail."""
+    h = w = 40
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    px, py = 0.5 + 0.01 * xx - 0.02 * yy, -1.0 + 0.015 * yy
+    hole = (xx >= 15) & (xx < 25) & (yy >= 15) & (yy < 25)
+    sup = ~hole
+    ux, uy = np.where(sup, px, np.nan), np.where(sup, py, np.nan)
+
+    def gate(dx, radius, min_support):
+        return O.local_affine_agreement(ux, uy, sup, px + dx, py, radius, min_support, 0.5)[20, 20]
+
+    assert gate(0.0, 12.0, 20)
+    assert gate(0.4, 12.0, 20)
+    assert not gate(0.6, 12.0, 20)
+    assert not gate(0.0, 2.0, 20)
+    assert gate(0.0, 8.0, 20)
+    assert not gate(0.0, 8.0, 100000)
+
+
+@pytest.mark.parametrize("name,fb_max", [("A", 10), ("B", 2)])
+def test_sequence_hybrid_fixtures(name, fb_max):
+    """seq_mode "hybrid" against the reference's fixtures and timing shares
+    (rust/crates/optflow-core/tests/parity_track.rs:531-637): reference, n_fb_max, mean confidence,
+    the fallback / gate-rejected / re-acquired shares, the carried-frames map fb, ux (and conf for
+    variant A). Variant B (fb_max 2) uses the re-acquisition without a fallback value."""
+    frames = load("hyb_frames")
+    cfg = Config(seq_mode="hybrid", ref_update_conf=0.8, fb_max=fb_max)
+    out = O.track_sequence(list(frames), cfg)
+    want = meta()["hybrid"][name]
+    ux_py, fb_py = load(f"hyb{name}_ux"), load(f"hyb{name}_fb")
+    assert len(out) == len(want)
+    for k, (r, t) in enumerate(zip(out, want)):
+        tm = r["timing"]
+        assert tm["reference"] == t["reference"]
+        assert tm["n_fb_max"] == t["n_fb_max"]
+        assert abs(tm["mean_conf"] - t["mean_conf"]) < 1e-4
+        for key in ("fallback_frac", "fb_gate_rejected", "reacquired"):
+            assert abs(tm[key] - t[key]) < 2e-3, key
+        assert (r["fb"] == fb_py[k]).mean() >= 0.999
+        assert (np.isfinite(r["ux"]) == np.isfinite(ux_py[k])).mean() >= 0.995
+        assert rms(r["ux"], ux_py[k]) < 1e-3
+        if name == "A":
+            assert rms(r["conf"], load("hybA_conf")[k]) < 1e-3
