@@ -61,7 +61,7 @@ __device__ __forceinline__ float freq_step(int i, int n, float dw) {
 }
 
 // 1 - exp(-t) for t >= 0: a Taylor series below 1/4 (truncation t^8 / 8! < 4e-10 relative), where
-// the subtraction would cancel, and the hardware exponential above.
+// the subtraction would cancel, and expf (about 1 ulp) above.
 __device__ __forceinline__ float one_minus_exp_neg(float t) {
     const float ser =
         t * fmaf(-t * 0.5f,
@@ -70,7 +70,7 @@ __device__ __forceinline__ float one_minus_exp_neg(float t) {
                            1.f),
                       1.f),
                  1.f);
-    return t < 0.25f ? ser : 1.f - __expf(-t);
+    return t < 0.25f ? ser : 1.f - expf(-t);
 }
 
 // MODE_FWD_REFLECT: line y of the reflect-padded real image (length wp) -> out[y][x].
@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
                 const float fy = freq_step(y, n, dw);
                 const float hp = one_minus_exp_neg(fmaf(fy * fy, shp, hx));
                 const float ay = fy - w0y;
-                const float G = hp * __expf(-fmaf(ay * ay, i2s, gx));
+                const float G = hp * expf(-fmaf(ay * ay, i2s, gx));
                 float2 v = cscale(src[y], G);
                 if (mul == MUL_G_IFX) v = make_float2(-v.y * fx, v.x * fx);
                 else if (mul == MUL_G_IFY) v = make_float2(-v.y * fy, v.x * fy);

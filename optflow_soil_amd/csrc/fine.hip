@@ -4,7 +4,9 @@
 //
 // Reference: python/optflow_ref/phaseflow.py:100-511 (the specification) and its Rust port
 // rust/crates/optflow-core/src/phaseflow.rs (storage f32, arithmetic f64).
+#include <algorithm>
 #include <cmath>
+#include <cstdlib>
 
 #include "fine.hpp"
 #include "lines.hpp"
@@ -93,6 +95,16 @@ __global__ void k_blur_y(const double *__restrict__ in, double *__restrict__ out
         s += k[j + r] * in[(size_t)yy * w + x];
     }
     out[(size_t)y * w + x] = s;
+}
+
+__global__ void k_blur_x(const double *__restrict__ in, float *__restrict__ out, int w, const BT bt, int r) {
+    const double *k = bt.k;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= w) return;
+    const double *row = in + (size_t)y * w;
+    double s = 0;
+    for (int j = -r; j <= r; j++) s += k[j + r] * row[min(max(x + j, 0), w - 1)];
+    out[(size_t)y * w + x] = (float)s;
 }
 
 // Register-blocked y pass for a compile-time radius R (sigma_s = 4 gives R = 16): one thread per
@@ -1014,7 +1026,7 @@ __global__ void __launch_bounds__(64) k_solve2(const double *__restrict__ mom, c
 constexpr int MS_B0 = 12, MS_B1 = 10;
 constexpr unsigned int MS_CAP = 8192;
 
-struct SelectState {
+struct MedState {
     unsigned int hist0[1 << MS_B0];  // digit 0: key bits 63 .. 52 (sign and exponent)
     unsigned int hist1[1 << MS_B1];  // digit 1: key bits 51 .. 42, within the selected digit 0
     unsigned int nan_count;
@@ -1032,7 +1044,7 @@ __device__ __forceinline__ double from_key(unsigned long long k) {
     return __longlong_as_double((long long)b);
 }
 
-__global__ void k_msel_init(SelectState *st) {
+__global__ void k_msel_init(MedState *st) {
     for (int i = threadIdx.x; i < (1 << MS_B0); i += blockDim.x) st->hist0[i] = 0;
     for (int i = threadIdx.x; i < (1 << MS_B1); i += blockDim.x) st->hist1[i] = 0;
     if (threadIdx.x == 0) {
@@ -1077,7 +1089,7 @@ __device__ void find_bin(const unsigned int *hist, int nb, unsigned long long ra
 }
 
 // digit 0 histogram (and the NaN count)
-__global__ void __launch_bounds__(256) k_msel_hist0(const double *__restrict__ v, size_t n, SelectState *st) {
+__global__ void __launch_bounds__(256) k_msel_hist0(const double *__restrict__ v, size_t n, MedState *st) {
     __shared__ unsigned int lh[1 << MS_B0];
     for (int i = threadIdx.x; i < (1 << MS_B0); i += blockDim.x) lh[i] = 0;
     __syncthreads();
@@ -1095,7 +1107,7 @@ __global__ void __launch_bounds__(256) k_msel_hist0(const double *__restrict__ v
 
 // digit 1 histogram of the keys in the selected digit-0 bin
 __global__ void __launch_bounds__(256) k_msel_hist1(const double *__restrict__ v, size_t n, unsigned long long k,
-                                                    SelectState *st) {
+                                                    MedState *st) {
     __shared__ unsigned int lh[1 << MS_B1];
     int b0;
     unsigned long long r0;
@@ -1114,7 +1126,7 @@ __global__ void __launch_bounds__(256) k_msel_hist1(const double *__restrict__ v
 // the keys sharing the selected 22-bit prefix go to the candidate list (at most MS_CAP kept, all
 // counted); the largest key below the prefix is kept for the even-count median
 __global__ void __launch_bounds__(256) k_msel_compact(const double *__restrict__ v, size_t n, unsigned long long k,
-                                                      SelectState *st) {
+                                                      MedState *st) {
     int b0, b1;
     unsigned long long r0, r1;
     find_bin(st->hist0, 1 << MS_B0, k, &b0, &r0);
@@ -1155,7 +1167,7 @@ __global__ void __launch_bounds__(256) k_msel_compact(const double *__restrict__
 // 7-bit digits over the 42 remaining bits with an LDS histogram; then the (k-1)-th for an even
 // count: the largest key below it if exactly r1 candidates lie below, else the key itself.
 __global__ void __launch_bounds__(1024) k_msel_final(const double *__restrict__ v, size_t n, unsigned long long k,
-                                                     int even, SelectState *st, double *out) {
+                                                     int even, MedState *st, double *out) {
     __shared__ unsigned int lh[128];
     __shared__ unsigned long long s_prefix, s_rank;
     __shared__ unsigned long long red_c[16], red_m[16];
@@ -1250,6 +1262,106 @@ __global__ void __launch_bounds__(1024) k_msel_final(const double *__restrict__ 
     }
 }
 
+// Radix select, 8 bits per pass (the form every 2048^2 benchmark run completed with): histograms
+// privatised in LDS; state: prefix (known high bits), rank still to find, NaN count.
+struct SelectState {
+    unsigned long long prefix;
+    unsigned long long rank;
+    unsigned int nan_count;
+    unsigned int pad;
+    unsigned long long below_count;  // elements with key < the selected key
+    unsigned long long below_max;    // largest key below it
+};
+
+__global__ void k_select_init(SelectState *st, unsigned int *hist, unsigned long long rank) {
+    st->prefix = 0;
+    st->rank = rank;
+    st->nan_count = 0;
+    st->below_count = 0;
+    st->below_max = 0;
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_select_hist(const double *__restrict__ v, size_t n, int shift,
+                                                     SelectState *st, unsigned int *hist) {
+    __shared__ unsigned int lh[256];
+    lh[threadIdx.x] = 0;
+    __syncthreads();
+    const unsigned long long prefix = st->prefix;
+    const int hi = shift + 8;  // bits above the current digit must equal the prefix
+    unsigned int nan_local = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double x = v[i];
+        if (shift == 56 && isnan(x)) nan_local++;
+        const unsigned long long k = order_key(x);
+        if (hi >= 64 || (k >> hi) == (prefix >> hi)) atomicAdd(&lh[(k >> shift) & 255], 1u);
+    }
+    if (nan_local) atomicAdd(&st->nan_count, nan_local);
+    __syncthreads();
+    if (lh[threadIdx.x]) atomicAdd(&hist[threadIdx.x], lh[threadIdx.x]);
+}
+
+__global__ void k_select_pick(SelectState *st, unsigned int *hist, int shift) {
+    if (threadIdx.x != 0) return;
+    unsigned long long rank = st->rank, cum = 0;
+    int d = 0;
+    for (; d < 256; d++) {
+        if (cum + hist[d] > rank) break;
+        cum += hist[d];
+    }
+    if (d == 256) d = 255;
+    st->prefix |= (unsigned long long)d << shift;
+    st->rank = rank - cum;
+    for (int i = 0; i < 256; i++) hist[i] = 0;
+}
+
+__global__ void __launch_bounds__(256) k_select_below(const double *__restrict__ v, size_t n, SelectState *st) {
+    const unsigned long long key = st->prefix;
+    unsigned long long cnt = 0, mx = 0;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const unsigned long long k = order_key(v[i]);
+        if (k < key) {
+            cnt++;
+            mx = k > mx ? k : mx;
+        }
+    }
+    // wavefront reduction, then one atomic per workgroup
+    __shared__ unsigned long long rc[4], rm[4];
+    for (int off = 32; off > 0; off >>= 1) {
+        cnt += __shfl_down(cnt, off);
+        unsigned long long o = __shfl_down(mx, off);
+        mx = o > mx ? o : mx;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        rc[threadIdx.x >> 6] = cnt;
+        rm[threadIdx.x >> 6] = mx;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long c = rc[0] + rc[1] + rc[2] + rc[3];
+        unsigned long long m = max(max(rm[0], rm[1]), max(rm[2], rm[3]));
+        if (c) {
+            atomicAdd(&st->below_count, c);
+            atomicMax(&st->below_max, m);
+        }
+    }
+}
+
+__global__ void k_select_finish(const SelectState *st, unsigned long long k, int even, double *out) {
+    if (threadIdx.x != 0) return;
+    if (st->nan_count) {
+        out[0] = NAN;
+        return;
+    }
+    const double upper = from_key(st->prefix);
+    if (!even) {
+        out[0] = upper;
+        return;
+    }
+    const double lower = (st->below_count >= k) ? from_key(st->below_max) : upper;
+    out[0] = (lower + upper) / 2.0;
+}
+
 // ------------------------------------------------------------------ confidence
 // conf = min(1, E / (0.25 median(E) + 1e-30)) exp(-(resid / phase_tol)^2) (phaseflow.py:320-323).
 // low = conf < conf_min (before the border rule, for the pre-warp). With final != 0 also the
@@ -1283,6 +1395,19 @@ __global__ void k_confidence(const double *__restrict__ energy, const double *__
 // ------------------------------------------------------------------ host wrappers
 static dim3 grid2(int w, int h) { return dim3((w + 255) / 256, h); }
 
+// The kernels added after the last completed 2048^2 benchmark run (the histogram-compaction median
+// and the register-blocked blur) run only with OFX_STAGED_KERNELS=1: a benchmark process that ran
+// them stopped making progress and the cause is not found yet (their results match the earlier
+// kernels in every parity test).
+bool staged_kernels() {
+    static const bool on = [] {
+        const char *v = std::getenv("OFX_STAGED_KERNELS");
+        return v && v[0] == '1';
+    }();
+    return on;
+}
+
+
 void prefilter_real(float *c, float *tmp, int w, int h, hipStream_t s) {
     if (h > 1) {
         const dim3 g((w + 255) / 256, (h + PF_SEG - 1) / PF_SEG);
@@ -1298,12 +1423,17 @@ void blur(const double *in, double *tmp, float *out, int w, int h, const double 
     if (r > 64) throw Error(OFX_EUNSUPPORTED, "sigma_s above 16 px is not supported by the GPU blur");
     BT bt;
     for (int j = 0; j <= 2 * r; j++) bt.k[j] = taps_host[j];
-    if (r == 16)
-        hipLaunchKernelGGL(k_blur_y_rb<16>, dim3((w + 255) / 256, (h + BLUR_ROWS - 1) / BLUR_ROWS), dim3(256), 0, s, in,
-                           tmp, w, h, bt);
-    else
+    if (staged_kernels()) {
+        if (r == 16)
+            hipLaunchKernelGGL(k_blur_y_rb<16>, dim3((w + 255) / 256, (h + BLUR_ROWS - 1) / BLUR_ROWS), dim3(256), 0,
+                               s, in, tmp, w, h, bt);
+        else
+            hipLaunchKernelGGL(k_blur_y, grid2(w, h), dim3(256), 0, s, in, tmp, w, h, bt, r);
+        hipLaunchKernelGGL(k_blur_x_lds, grid2(w, h), dim3(256), 0, s, tmp, out, w, bt, r);
+    } else {
         hipLaunchKernelGGL(k_blur_y, grid2(w, h), dim3(256), 0, s, in, tmp, w, h, bt, r);
-    hipLaunchKernelGGL(k_blur_x_lds, grid2(w, h), dim3(256), 0, s, tmp, out, w, bt, r);
+        hipLaunchKernelGGL(k_blur_x, grid2(w, h), dim3(256), 0, s, tmp, out, w, bt, r);
+    }
     LAUNCHCHECK();
 }
 
@@ -1379,21 +1509,37 @@ void energy_reset(const double *energy, const double *med, const double *ux_in, 
     LAUNCHCHECK();
 }
 
-size_t median_scratch_bytes() { return sizeof(SelectState); }
+size_t median_scratch_bytes() { return std::max(sizeof(MedState), sizeof(SelectState) + 256 * sizeof(unsigned int)); }
 
-// numpy.median of n values (NaN if any is NaN): the n/2-th order statistic by a 12-bit and a
-// 10-bit histogram pass over the order-preserving keys, a compaction of the keys sharing the
-// selected 22-bit prefix, and a one-workgroup select among them; for an even n the mean with the
-// (n/2 - 1)-th (core/select.rs:19-42).
+
+// numpy.median of n values (NaN if any is NaN), for an even n the mean of the n/2 - 1 and n/2 order
+// statistics (core/select.rs:19-42).
 void median(const double *v, size_t n, void *scratch, double *out, hipStream_t s) {
-    SelectState *st = (SelectState *)scratch;
     const unsigned long long k = n / 2;
-    const int blocks = (int)std::min<size_t>(512, (n + 255) / 256);
-    hipLaunchKernelGGL(k_msel_init, dim3(1), dim3(1024), 0, s, st);
-    hipLaunchKernelGGL(k_msel_hist0, dim3(blocks), dim3(256), 0, s, v, n, st);
-    hipLaunchKernelGGL(k_msel_hist1, dim3(blocks), dim3(256), 0, s, v, n, k, st);
-    hipLaunchKernelGGL(k_msel_compact, dim3(blocks), dim3(256), 0, s, v, n, k, st);
-    hipLaunchKernelGGL(k_msel_final, dim3(1), dim3(1024), 0, s, v, n, k, (n % 2 == 0) ? 1 : 0, st, out);
+    const int even = (n % 2 == 0) ? 1 : 0;
+    if (staged_kernels()) {
+        // a 12-bit and a 10-bit histogram pass over the order-preserving keys, a compaction of the
+        // keys sharing the selected 22-bit prefix, and a one-workgroup select among them
+        MedState *st = (MedState *)scratch;
+        const int blocks = (int)std::min<size_t>(512, (n + 255) / 256);
+        hipLaunchKernelGGL(k_msel_init, dim3(1), dim3(1024), 0, s, st);
+        hipLaunchKernelGGL(k_msel_hist0, dim3(blocks), dim3(256), 0, s, v, n, st);
+        hipLaunchKernelGGL(k_msel_hist1, dim3(blocks), dim3(256), 0, s, v, n, k, st);
+        hipLaunchKernelGGL(k_msel_compact, dim3(blocks), dim3(256), 0, s, v, n, k, st);
+        hipLaunchKernelGGL(k_msel_final, dim3(1), dim3(1024), 0, s, v, n, k, even, st, out);
+        LAUNCHCHECK();
+        return;
+    }
+    SelectState *st = (SelectState *)scratch;
+    unsigned int *hist = (unsigned int *)((char *)scratch + sizeof(SelectState));
+    hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, hist, k);
+    const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
+    for (int shift = 56; shift >= 0; shift -= 8) {
+        hipLaunchKernelGGL(k_select_hist, dim3(blocks), dim3(256), 0, s, v, n, shift, st, hist);
+        hipLaunchKernelGGL(k_select_pick, dim3(1), dim3(64), 0, s, st, hist, shift);
+    }
+    if (even) hipLaunchKernelGGL(k_select_below, dim3(blocks), dim3(256), 0, s, v, n, st);
+    hipLaunchKernelGGL(k_select_finish, dim3(1), dim3(64), 0, s, st, k, even, out);
     LAUNCHCHECK();
 }
 

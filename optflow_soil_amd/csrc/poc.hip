@@ -21,6 +21,7 @@ struct CoarseWs {
     double *tmp = nullptr;         // blur scratch
     double *gux = nullptr, *guy = nullptr, *gpk = nullptr, *ix = nullptr, *iy = nullptr;
     int *gflag = nullptr;          // per point: valid, out flags
+    int *gwork = nullptr;          // clean: two lists of points to fill, then a counter
     float *hann = nullptr;         // Hann tables of the current level (wy + wx floats)
     size_t gcap = 0;
     size_t tmpcap = 0;
@@ -461,47 +462,62 @@ __device__ double median_small(double *v, int n) {
     return (n & 1) ? v[n / 2] : 0.5 * (v[n / 2 - 1] + v[n / 2]);
 }
 
-// flags: bit 0 valid (peak >= peak_min), bit 1 out (current pass), bit 2 out (next pass)
-__global__ void __launch_bounds__(1024) k_clean(double *ux, double *uy, double *peak, const double *fx,
-                                                const double *fy, int have_fb, int *flags, int ny, int nx,
-                                                double peak_min, double thresh) {
-    __shared__ int any_out, all_out;
+__global__ void k_zero_int(int *p) {
+    if (threadIdx.x == 0) p[0] = 0;
+}
+
+// flags: bit 0 valid (peak >= peak_min), bit 1 out (the point holds a usable value)
+// Outlier test (poc.py:95-120) over the whole grid, one thread per point: a valid point with at
+// least 3 valid 8-neighbours is an outlier if either component deviates from their median by more
+// than thresh (MAD + 0.1). counters[0] counts the points left out.
+__global__ void __launch_bounds__(256) k_clean_test(const double *__restrict__ ux, const double *__restrict__ uy,
+                                                    const double *__restrict__ peak, int *__restrict__ flags,
+                                                    int *__restrict__ counters, int ny, int nx, double peak_min,
+                                                    double thresh) {
     const int n = ny * nx;
-    const int OFF[8][2] = {{-1, -1}, {-1, 0}, {-1, 1}, {0, -1}, {0, 1}, {1, -1}, {1, 0}, {1, 1}};
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        const int v = peak[k] >= peak_min;
-        flags[k] = v | (v << 1);
-    }
-    __syncthreads();
-    // outlier test against the valid neighbours' medians (ux, uy unchanged during the test)
-    for (int k = threadIdx.x; k < n; k += blockDim.x) {
-        if (!(flags[k] & 1)) continue;
-        const int i = k / nx, j = k - i * nx;
-        int nb[8], c = 0;
-        for (int q = 0; q < 8; q++) {
-            const int a = i + OFF[q][0], b = j + OFF[q][1];
-            if (a >= 0 && a < ny && b >= 0 && b < nx && (flags[a * nx + b] & 1)) nb[c++] = a * nx + b;
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    bool out = false;
+    if (k < n) {
+        const bool valid = peak[k] >= peak_min;
+        out = valid;
+        if (valid) {
+            const int i = k / nx, j = k - i * nx;
+            int nb[8], c = 0;
+            for (int di = -1; di <= 1; di++)
+                for (int dj = -1; dj <= 1; dj++) {
+                    if (!di && !dj) continue;
+                    const int a = i + di, b = j + dj;
+                    if (a >= 0 && a < ny && b >= 0 && b < nx && peak[a * nx + b] >= peak_min) nb[c++] = a * nx + b;
+                }
+            if (c >= 3) {
+                for (int comp = 0; comp < 2; comp++) {
+                    const double *u = comp ? uy : ux;
+                    double vals[8], dev[8];
+                    for (int q = 0; q < c; q++) vals[q] = u[nb[q]];
+                    const double med = median_small(vals, c);
+                    for (int q = 0; q < c; q++) dev[q] = fabs(u[nb[q]] - med);
+                    const double res = median_small(dev, c);
+                    if (fabs(u[k] - med) / (res + 0.1) > thresh) out = false;
+                }
+            }
         }
-        if (c < 3) continue;
-        bool outlier = false;
-        for (int comp = 0; comp < 2; comp++) {
-            const double *u = comp ? uy : ux;
-            double vals[8], dev[8];
-            for (int q = 0; q < c; q++) vals[q] = u[nb[q]];
-            const double med = median_small(vals, c);
-            for (int q = 0; q < c; q++) dev[q] = fabs(u[nb[q]] - med);
-            const double res = median_small(dev, c);
-            if (fabs(u[k] - med) / (res + 0.1) > thresh) outlier = true;
-        }
-        if (outlier) atomicAnd(&flags[k], ~2);
+        flags[k] = (valid ? 1 : 0) | (out ? 2 : 0);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) any_out = 0;
-    __syncthreads();
-    for (int k = threadIdx.x; k < n; k += blockDim.x)
-        if (flags[k] & 2) any_out = 1;
-    __syncthreads();
-    if (!any_out) {
+    const unsigned long long b = __ballot(out);
+    if ((threadIdx.x & 63) == 0 && b) atomicAdd(&counters[0], (int)__popcll(b));
+}
+
+// Fill (poc.py:121-140) in one workgroup over the list of points still to fill: each pass gives
+// every listed point with out neighbours the median of their values, then drops it from the list;
+// at most 100 passes. Without any out point the grid falls back to the guess (or 0) with peak 0.
+// Finally the peak of every invalid point is 0.
+__global__ void __launch_bounds__(1024) k_clean_fill(double *ux, double *uy, double *peak, const double *fx,
+                                                     const double *fy, int have_fb, int *flags, int *work,
+                                                     const int *counters, int ny, int nx) {
+    __shared__ int nwork, nnext;
+    // work: two lists of up to ny nx points (this pass, next pass)
+    const int n = ny * nx;
+    if (counters[0] == 0) {
         for (int k = threadIdx.x; k < n; k += blockDim.x) {
             ux[k] = have_fb ? fx[k] : 0.0;
             uy[k] = have_fb ? fy[k] : 0.0;
@@ -509,42 +525,49 @@ __global__ void __launch_bounds__(1024) k_clean(double *ux, double *uy, double *
         }
         return;
     }
-    for (int pass = 0; pass < 100; pass++) {
-        if (threadIdx.x == 0) all_out = 1;
-        __syncthreads();
-        for (int k = threadIdx.x; k < n; k += blockDim.x)
-            if (!(flags[k] & 2)) all_out = 0;
-        __syncthreads();
-        if (all_out) break;
-        for (int k = threadIdx.x; k < n; k += blockDim.x) {
-            int f = flags[k];
-            if (f & 2) {
-                f |= 4;
-            } else {
-                const int i = k / nx, j = k - i * nx;
-                double vx[8], vy[8];
-                int c = 0;
-                for (int q = 0; q < 8; q++) {
-                    const int a = i + OFF[q][0], b = j + OFF[q][1];
+    int *cur = work, *nxt = work + n;
+    if (threadIdx.x == 0) nwork = 0;
+    __syncthreads();
+    for (int k = threadIdx.x; k < n; k += blockDim.x)
+        if (!(flags[k] & 2)) cur[atomicAdd(&nwork, 1)] = k;
+    __syncthreads();
+    for (int pass = 0; pass < 100 && nwork > 0; pass++) {
+        // values from the points out at the start of the pass; a filled point is marked with bit 2
+        for (int t = threadIdx.x; t < nwork; t += blockDim.x) {
+            const int k = cur[t];
+            const int i = k / nx, j = k - i * nx;
+            double vx[8], vy[8];
+            int c = 0;
+            for (int di = -1; di <= 1; di++)
+                for (int dj = -1; dj <= 1; dj++) {
+                    if (!di && !dj) continue;
+                    const int a = i + di, b = j + dj;
                     if (a >= 0 && a < ny && b >= 0 && b < nx && (flags[a * nx + b] & 2)) {
                         vx[c] = ux[a * nx + b];
                         vy[c] = uy[a * nx + b];
                         c++;
                     }
                 }
-                if (c) {
-                    ux[k] = median_small(vx, c);
-                    uy[k] = median_small(vy, c);
-                    f |= 4;
-                }
+            if (c) {
+                ux[k] = median_small(vx, c);
+                uy[k] = median_small(vy, c);
+                flags[k] |= 4;
             }
-            flags[k] = f;
+        }
+        if (threadIdx.x == 0) nnext = 0;
+        __syncthreads();
+        // filled points become out; the others go to the next list (in any order: a point's fill
+        // depends only on the state at the start of a pass)
+        for (int t = threadIdx.x; t < nwork; t += blockDim.x) {
+            const int k = cur[t];
+            if (flags[k] & 4) flags[k] = (flags[k] & 1) | 2;
+            else nxt[atomicAdd(&nnext, 1)] = k;
         }
         __syncthreads();
-        for (int k = threadIdx.x; k < n; k += blockDim.x) {
-            const int f = flags[k];
-            flags[k] = (f & 1) | ((f >> 2) & 1) << 1;
-        }
+        int *tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+        if (threadIdx.x == 0) nwork = nnext;
         __syncthreads();
     }
     for (int k = threadIdx.x; k < n; k += blockDim.x)
@@ -655,6 +678,7 @@ void coarse_release(CoarseWs *&ws) {
     (void)hipFree(ws->ix);
     (void)hipFree(ws->iy);
     (void)hipFree(ws->gflag);
+    (void)hipFree(ws->gwork);
     (void)hipFree(ws->hann);
     delete ws;
     ws = nullptr;
@@ -712,13 +736,14 @@ void coarse_dense(const float *i1, const float *i2, int w, int h, const ofx_conf
     }
     if (!ws->gux || ws->gcap < npmax) {
         (void)hipFree(ws->gux); (void)hipFree(ws->guy); (void)hipFree(ws->gpk);
-        (void)hipFree(ws->ix); (void)hipFree(ws->iy); (void)hipFree(ws->gflag);
+        (void)hipFree(ws->ix); (void)hipFree(ws->iy); (void)hipFree(ws->gflag); (void)hipFree(ws->gwork);
         HIPCHECK(hipMalloc(&ws->gux, npmax * sizeof(double) * 2));
         HIPCHECK(hipMalloc(&ws->guy, npmax * sizeof(double) * 2));
         HIPCHECK(hipMalloc(&ws->gpk, npmax * sizeof(double)));
         HIPCHECK(hipMalloc(&ws->ix, npmax * sizeof(double)));
         HIPCHECK(hipMalloc(&ws->iy, npmax * sizeof(double)));
         HIPCHECK(hipMalloc(&ws->gflag, npmax * sizeof(int)));
+        HIPCHECK(hipMalloc(&ws->gwork, (2 * npmax + 64) * sizeof(int)));
         ws->gcap = npmax;
     }
     GridGeom prev{};
@@ -795,8 +820,13 @@ void coarse_dense(const float *i1, const float *i2, int w, int h, const ofx_conf
                                ws->hann, ws->ix, ws->iy, ws->gux, ws->guy, ws->gpk);
         }
         LAUNCHCHECK();
-        hipLaunchKernelGGL(k_clean, dim3(1), dim3(1024), 0, s, ws->gux, ws->guy, ws->gpk, ws->ix, ws->iy, have_init,
-                           ws->gflag, ny, nx, c.poc_peak_min, c.median_thresh);
+        int *cnt = ws->gwork + 2 * ws->gcap;
+        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, cnt);
+        hipLaunchKernelGGL(k_clean_test, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, (const double *)ws->gux,
+                           (const double *)ws->guy, (const double *)ws->gpk, ws->gflag, cnt, ny, nx, c.poc_peak_min,
+                           c.median_thresh);
+        hipLaunchKernelGGL(k_clean_fill, dim3(1), dim3(1024), 0, s, ws->gux, ws->guy, ws->gpk, ws->ix, ws->iy, have_init,
+                           ws->gflag, ws->gwork, (const int *)cnt, ny, nx);
         LAUNCHCHECK();
         prev = cur;
         have_prev = true;
