@@ -116,6 +116,14 @@ int ofx_refine_raw(ofx_engine *e, const float *i1, int w, int h, const float *i2
                    const ofx_config *cfg, double *out_ux, double *out_uy, double *out_energy,
                    double *out_resid);
 
+/* The coarse stage alone: coarse_flow (rust/crates/optflow-core/src/poc.rs:694-766,
+ * python/optflow_ref/poc.py:216-239) on the pyramid of i1, i2 (w x h each), then the dense start
+ * field of track_pair (track.rs:496-554, track.py:131-137): the grid interpolated to every
+ * pixel. init_ux/init_uy: dense guess or NULL (both or neither). Outputs f64, w x h. Host
+ * pointers. */
+int ofx_coarse_flow(ofx_engine *e, const float *i1, const float *i2, int w, int h, const float *init_ux,
+                    const float *init_uy, const ofx_config *cfg, double *out_ux, double *out_uy);
+
 /* Outputs of one pair or sequence frame: the reference's TrackResult fields and .npy outputs
  * (rust/crates/optflow-core/src/track.rs:157-209, cli/io.rs:558-600). Any pointer may be NULL
  * (not written). ux, uy: NaN where invalid; valid: 1 where ux is finite; raw_*: unmasked

@@ -20,7 +20,8 @@ OFX_OK, OFX_EINVAL, OFX_ETOOLARGE, OFX_EDEVICE, OFX_ENODEVICE, OFX_EBUSY, OFX_EU
 
 # every symbol include/ofx.h declares
 SYMBOLS = ("ofx_config_default", "ofx_create", "ofx_destroy", "ofx_last_error", "ofx_lanes", "ofx_returns_slope",
-           "ofx_check_window", "ofx_release_caches", "ofx_refine_raw", "ofx_track_pair", "ofx_track_pair_device",
+           "ofx_check_window", "ofx_release_caches", "ofx_refine_raw", "ofx_coarse_flow", "ofx_track_pair",
+           "ofx_track_pair_device",
            "ofx_track_sequence", "ofx_profile", "ofx_profile_read", "ofx_profile_stage")
 
 
@@ -198,6 +199,21 @@ class Engine:
                                     ctypes.c_int64(int(shift[0])), ctypes.c_int64(int(shift[1])), _p(ux), _p(uy),
                                     ctypes.byref(c), *[_p(a) for a in o]))
         return tuple(o)
+
+    def coarse_flow(self, I1, I2, cfg, init=None):
+        """The coarse stage (poc.py:216-239) and the dense start field of track.py:131-137: the
+        (ux, uy) field, float64, that the fine stage of a pair starts from."""
+        I1, I2 = _f32(I1), _f32(I2)
+        if I1.shape != I2.shape:
+            raise ValueError(f"image sizes differ: {I1.shape} vs {I2.shape}")
+        h, w = I1.shape
+        ix = iy = None
+        if init is not None:
+            ix, iy = _f32(init[0]), _f32(init[1])
+        ox, oy = np.empty((h, w)), np.empty((h, w))
+        _check(lib().ofx_coarse_flow(self._h, _p(I1), _p(I2), w, h, _p(ix), _p(iy), ctypes.byref(to_c(cfg)),
+                                     _p(ox), _p(oy)))
+        return ox, oy
 
     def track_pair(self, I1, I2, cfg, mask=None, init=None) -> Result:
         """track.py:115-146: displacement of I1's pixels into I2."""

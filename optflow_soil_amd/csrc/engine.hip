@@ -835,6 +835,40 @@ int ofx_refine_raw(ofx_engine *e, const float *i1, int w, int h, const float *i2
     });
 }
 
+int ofx_coarse_flow(ofx_engine *e, const float *i1, const float *i2, int w, int h, const float *init_ux,
+                    const float *init_uy, const ofx_config *cfg, double *out_ux, double *out_uy) {
+    return guarded([&] {
+        validate(cfg);
+        if (!i1 || !i2 || !out_ux || !out_uy) throw Error(OFX_EINVAL, "input or output pointer is NULL");
+        if ((init_ux == nullptr) != (init_uy == nullptr)) throw Error(OFX_EINVAL, "init needs both components");
+        check_window_impl(w, h, *cfg);
+        if (std::min(w, h) < cfg->poc_win)
+            throw Error(OFX_EINVAL, "image smaller than poc_win (" + std::to_string(cfg->poc_win) + ")");
+        LaneGuard g(e);
+        Workspace &ws = *g.lane->ws;
+        hipStream_t s = g.lane->stream;
+        const size_t n = (size_t)w * h;
+        float *d1 = B<float>(ws.bi1, n), *d2 = B<float>(ws.bi2, n);
+        HIPCHECK(hipMemcpyAsync(d1, i1, n * sizeof(float), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(d2, i2, n * sizeof(float), hipMemcpyHostToDevice, s));
+        double *ix = nullptr, *iy = nullptr;
+        if (init_ux) {
+            float *fx = B<float>(ws.binx, n), *fy = B<float>(ws.biny, n);
+            HIPCHECK(hipMemcpyAsync(fx, init_ux, n * sizeof(float), hipMemcpyHostToDevice, s));
+            HIPCHECK(hipMemcpyAsync(fy, init_uy, n * sizeof(float), hipMemcpyHostToDevice, s));
+            ix = B<double>(ws.bstart_x, n);
+            iy = B<double>(ws.bstart_y, n);
+            to_f64(fx, ix, n, s);
+            to_f64(fy, iy, n, s);
+        }
+        double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
+        coarse_dense(d1, d2, w, h, *cfg, ix, iy, ux, uy, ws.coarse, s);
+        HIPCHECK(hipMemcpyAsync(out_ux, ux, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipMemcpyAsync(out_uy, uy, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+    });
+}
+
 int ofx_track_pair_device(ofx_engine *e, const float *d_i1, const float *d_i2, int w, int h, const uint8_t *d_mask,
                           const float *d_init_ux, const float *d_init_uy, const ofx_config *cfg, ofx_result *d_out) {
     return guarded([&] {

@@ -138,18 +138,22 @@ __global__ void k_prewarp_fill(const double *__restrict__ ux, const double *__re
     long long best = -1;
     int bx = -1;
     const int maxdx = max(x, w - 1 - x);
+    // the smallest squared distance, and among equal ones the smallest column (scipy's rule); a
+    // tie can come from any |dx| up to the distance itself, so equal totals found later still
+    // compete by column
     for (int dx = 0; dx <= maxdx; dx++) {
         const long long dd = (long long)dx * dx;
         if (best >= 0 && dd > best) break;
-        // smaller column first, so that a tie keeps it
         const int c0 = x - dx, c1 = x + dx;
         if (c0 >= 0) {
             const int g = row[c0];
-            if (g >= 0 && (best < 0 || dd + g < best)) { best = dd + g; bx = c0; }
+            const long long t = dd + g;
+            if (g >= 0 && (best < 0 || t < best || (t == best && c0 < bx))) { best = t; bx = c0; }
         }
         if (dx > 0 && c1 < w) {
             const int g = row[c1];
-            if (g >= 0 && (best < 0 || dd + g < best)) { best = dd + g; bx = c1; }
+            const long long t = dd + g;
+            if (g >= 0 && (best < 0 || t < best || (t == best && c1 < bx))) { best = t; bx = c1; }
         }
     }
     if (bx < 0) {  // unreachable while n_ok > 0

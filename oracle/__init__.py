@@ -218,6 +218,19 @@ def coarse_flow(i1, i2, cfg, init=None):
     return tuple(o)
 
 
+def coarse_dense(i1, i2, cfg, init=None):
+    """The dense start field (ux, uy) of track.py:131-137: coarse_flow's grid at every pixel."""
+    i1, i2 = _d(i1), _d(i2)
+    h, w = i1.shape
+    ix = iy = None
+    if init is not None:
+        ix, iy = _d(init[0]), _d(init[1])
+    ox, oy = np.empty((h, w)), np.empty((h, w))
+    lib().orc_coarse_dense(_p(i1), _p(i2), h, w, ctypes.byref(_cfg(cfg)), _p(ix) if ix is not None else None,
+                           _p(iy) if iy is not None else None, _p(ox), _p(oy))
+    return ox, oy
+
+
 def edt(inp, return_indices=False):
     """scipy.ndimage.distance_transform_edt(inp) (and the flat nearest-zero index)."""
     b = np.ascontiguousarray(inp, dtype=np.uint8)

@@ -91,6 +91,9 @@ void orc_grid_dims(int h, int w, const orc_cfg *cfg, int *nx, int *ny);
 int orc_coarse_flow(const double *i1, const double *i2, int h, int w, const orc_cfg *cfg,
                     const double *init_ux, const double *init_uy, double *gux, double *guy,
                     double *peak);
+/* the dense start field of track.py:131-137 (coarse_flow's grid at every pixel), h x w each */
+void orc_coarse_dense(const double *i1, const double *i2, int h, int w, const orc_cfg *cfg,
+                      const double *init_ux, const double *init_uy, double *ux, double *uy);
 
 /* ---- distance transform and components (scipy.ndimage semantics) ---- */
 void orc_edt(const uint8_t *input, int h, int w, double *dist, int32_t *nearest);

@@ -396,6 +396,11 @@ extern "C" void orc_grid_dims(int h, int w, const orc_cfg *cfg, int *nx, int *ny
     *ny = (int)grid_axis(h, cfg->poc_win, cfg->grid_step).size();
 }
 
+extern "C" void orc_coarse_dense(const double *i1, const double *i2, int h, int w, const orc_cfg *cfg,
+                                 const double *init_ux, const double *init_uy, double *ux, double *uy) {
+    coarse_dense(i1, i2, h, w, *cfg, init_ux, init_uy, ux, uy, nullptr, nullptr, nullptr);
+}
+
 extern "C" int orc_coarse_flow(const double *i1, const double *i2, int h, int w, const orc_cfg *cfg,
                                const double *init_ux, const double *init_uy, double *gux, double *guy,
                                double *peak) {
