@@ -162,6 +162,20 @@ int ofx_track_pair_device(ofx_engine *e, const float *d_i1, const float *d_i2, i
 int ofx_track_sequence(ofx_engine *e, const float *frames, int n, int w, int h,
                        const uint8_t *mask, const ofx_config *cfg, ofx_result *results);
 
+/* The streaming form of track_sequence, one frame at a time (the Sequence iterator of
+ * track.rs:1112-1225; track.py:260-315 and track/hybrid.rs:262-540): ofx_seq_begin takes frame 0
+ * (w x h, host) and the optional ROI, and holds one lane of the engine until ofx_seq_end;
+ * ofx_seq_step takes frame k = 1, 2, ... and writes its result on the frame-0 grid, as
+ * ofx_track_sequence does for results[k - 1]. ofx_seq_step_device takes a device frame and device
+ * output arrays (as ofx_track_pair_device). Errors as ofx_track_sequence; a failed step leaves the
+ * sequence unusable, and ofx_seq_end must still be called. */
+typedef struct ofx_sequence ofx_sequence;
+int ofx_seq_begin(ofx_engine *e, const float *frame0, int w, int h, const uint8_t *mask, const ofx_config *cfg,
+                  ofx_sequence **out);
+int ofx_seq_step(ofx_sequence *seq, const float *frame, ofx_result *out);
+int ofx_seq_step_device(ofx_sequence *seq, const float *d_frame, ofx_result *d_out);
+void ofx_seq_end(ofx_sequence *seq);
+
 #ifdef __cplusplus
 }
 #endif

@@ -21,7 +21,7 @@ OFX_OK, OFX_EINVAL, OFX_ETOOLARGE, OFX_EDEVICE, OFX_ENODEVICE, OFX_EBUSY, OFX_EU
 # every symbol include/ofx.h declares
 SYMBOLS = ("ofx_config_default", "ofx_create", "ofx_destroy", "ofx_last_error", "ofx_lanes", "ofx_returns_slope",
            "ofx_check_window", "ofx_release_caches", "ofx_refine_raw", "ofx_coarse_flow", "ofx_track_pair",
-           "ofx_track_pair_device",
+           "ofx_track_pair_device", "ofx_seq_begin", "ofx_seq_step", "ofx_seq_step_device", "ofx_seq_end",
            "ofx_track_sequence", "ofx_profile", "ofx_profile_read", "ofx_profile_stage")
 
 
@@ -266,6 +266,67 @@ class Engine:
         c = to_c(cfg)
         _check(lib().ofx_track_sequence(self._h, _p(fr), n, w, h, _p(m), ctypes.byref(c), rs))
         return [(k + 1, _to_result(rs[k], *keep[k])) for k in range(n - 1)]
+
+    def sequence(self, frame0, cfg, mask=None):
+        """Opens a streaming sequence on frame 0 (the Sequence iterator of track.rs:1112-1225): a
+        Sequence whose step(frame) returns the Result of the next frame. It holds one lane until
+        closed."""
+        return Sequence(self, frame0, cfg, mask)
+
+    def iter_sequence(self, frames, cfg, mask=None):
+        """track.py:260-315 / 383-505 as a generator over any iterable of frames: yields (k, Result)
+        for k = 1, 2, ... as each frame arrives, one frame in memory at a time."""
+        it = iter(frames)
+        try:
+            f0 = next(it)
+        except StopIteration:
+            raise ValueError("no frames") from None
+        with self.sequence(f0, cfg, mask) as q:
+            for k, f in enumerate(it, start=1):
+                yield k, q.step(f)
+
+
+class Sequence:
+    """A sequence in progress on one lane of an Engine (ofx_seq_begin / ofx_seq_step / ofx_seq_end)."""
+
+    def __init__(self, engine, frame0, cfg, mask=None):
+        f0 = _f32(frame0)
+        h, w = f0.shape
+        if mask is not None and np.shape(mask) != (h, w):
+            raise ValueError(f"ROI mask shape {np.shape(mask)} differs from the frame shape {(h, w)}")
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        self._engine = engine  # keeps the engine alive while the sequence holds its lane
+        self._shape = (h, w)
+        self._hybrid = cfg.seq_mode == "hybrid"
+        self._h = ctypes.c_void_p()
+        _check(lib().ofx_seq_begin(engine._h, _p(f0), w, h, _p(m), ctypes.byref(to_c(cfg)), ctypes.byref(self._h)))
+
+    def step(self, frame) -> Result:
+        if not self._h:
+            raise RuntimeError("sequence is closed")
+        f = _f32(frame)
+        if f.shape != self._shape:
+            raise ValueError(f"frame shape {f.shape} differs from frame 0's {self._shape}")
+        r, arrs, valid = _new_result(*self._shape, with_fb=self._hybrid)
+        _check(lib().ofx_seq_step(self._h, _p(f), ctypes.byref(r)))
+        return _to_result(r, arrs, valid)
+
+    def close(self):
+        if self._h:
+            lib().ofx_seq_end(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 _default = None

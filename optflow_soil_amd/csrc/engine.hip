@@ -235,6 +235,19 @@ T *B(DevBuf &b, size_t n) {
     return b.get<T>(n);
 }
 
+// the checks of track.py:270-279 (and track.rs:655-663) before a sequence starts
+void check_sequence_args(int w, int h, const uint8_t *mask, const ofx_config &c) {
+    check_window_impl(w, h, c);
+    if (std::min(w, h) < c.poc_win)
+        throw Error(OFX_EINVAL, "image smaller than poc_win (" + std::to_string(c.poc_win) + ")");
+    if (mask) {
+        const size_t n = (size_t)w * h;
+        bool any = false;
+        for (size_t i = 0; i < n && !any; i++) any = mask[i] != 0;
+        if (!any) throw Error(OFX_EINVAL, "ROI mask is empty");
+    }
+}
+
 void set_taps(Workspace &ws, const ofx_config &c, hipStream_t s) {
     if (ws.taps_s_sigma != c.sigma_s) {
         std::vector<double> k = gauss_taps(c.sigma_s);
@@ -517,58 +530,215 @@ int guarded(F &&f) {
 }
 
 
-// seq_mode "hybrid" (rust/crates/optflow-core/src/track/hybrid.rs:262-540, track.py:383-505):
-// the direct leg of the chain plus, per pixel, the gated adjacent-frame fallback. The host steps
-// the frames and reads back the few counters that decide which legs and gates run.
-template <class Leg, class LegTo, class Mean>
-void sequence_hybrid(Workspace &ws, const float *frames, int nf, int w, int h, const uint8_t *dm, const ofx_config &c,
-                     ofx_result *results, hipStream_t s, Leg &run_leg, LegTo &run_leg_to, Mean &leg_mean) {
-    const size_t n = (size_t)w * h;
-    float *fref = (float *)ws.bframe_ref.p, *fprev = (float *)ws.bframe_prev.p, *fcur = (float *)ws.bi2.p;
-    double *u0x = (double *)ws.bu0x.p, *u0y = (double *)ws.bu0y.p, *c0r = (double *)ws.bc0r.p;
-    double *cx = (double *)ws.bcx.p, *cy = (double *)ws.bcy.p, *ca = (double *)ws.bca.p;
-    double *px = (double *)ws.bprev_x.p, *py = (double *)ws.bprev_y.p, *pc = (double *)ws.bprev_c.p;
-    double *lc = (double *)ws.bleg_c.p, *lx = (double *)ws.bstart_x.p, *ly = (double *)ws.bstart_y.p;
-    double *legc = B<double>(ws.bconf0, n);
-    uint8_t *anch = B<uint8_t>(ws.banch, n), *good_prev = B<uint8_t>(ws.bgood_prev, n);
-    double *upx = B<double>(ws.bupx, n), *upy = B<double>(ws.bupy, n);
-    int *nfb = B<int>(ws.bnfb, n);
-    double *pajx = B<double>(ws.bpadj_x, n), *pajy = B<double>(ws.bpadj_y, n);
-    double *ajx = B<double>(ws.badj_x, n), *ajy = B<double>(ws.badj_y, n), *ajc = B<double>(ws.badj_c, n);
-    double *link = B<double>(ws.blink, n);
-    uint8_t *flags = B<uint8_t>(ws.bflags, n);
-    double *ax = B<double>(ws.bax, n), *ay = B<double>(ws.bay, n);
-    double *cax = B<double>(ws.bcax, n), *cay = B<double>(ws.bcay, n), *caa = B<double>(ws.bca_adj, n);
-    uint8_t *gate0 = B<uint8_t>(ws.bgate0, n), *gate = B<uint8_t>(ws.bgate, n), *hlow = B<uint8_t>(ws.bhlow, n);
-    int *cnt = B<int>(ws.bhcnt, HYB_NCOUNTERS);
-    void *gscr = ws.bgscr.get<char>(gate_scratch_bytes(w, h));
-    double *hux = B<double>(ws.bhux, n), *huy = B<double>(ws.bhuy, n);
-    int32_t *dfb = B<int32_t>(ws.bfb, n);
-    {
-        std::vector<uint8_t> ones8(n, 1);
-        std::vector<double> zeros(n, 0.0);
-        std::vector<int> zi(n, 0);
-        HIPCHECK(hipMemcpyAsync(anch, ones8.data(), n, hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(good_prev, ones8.data(), n, hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(upx, zeros.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(upy, zeros.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(caa, zeros.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(nfb, zi.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipStreamSynchronize(s));
-    }
-    auto counters = [&]() {
-        std::vector<int> hc(HYB_NCOUNTERS);
-        HIPCHECK(hipMemcpyAsync(hc.data(), cnt, HYB_NCOUNTERS * sizeof(int), hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
-        return hc;
-    };
-    const bool gate_on = c.fb_min_support > 0;
-    const double radius = fb_radius_px(c);
+// A sequence on one lane (rust/crates/optflow-core/src/track.rs:1112-1225 Sequence::new and
+// step; track.py:260-315 for seq_mode "chain", track/hybrid.rs:262-540 and track.py:383-505 for
+// "hybrid"). Its state lives in the lane's workspace: the reference frame, the previous frame,
+// u_0r (frame 0 -> reference), the frame-0 link confidence c0r, the previous leg's raw field and
+// confidence, and for hybrid the per-pixel anchor, carried field, fallback count and adjacent-leg
+// start. The lane stays held from begin to the last step, so nothing else touches that state.
+struct Seq {
+    ofx_engine *eng;
+    Lane *lane;
+    Workspace &ws;
+    hipStream_t s;
+    ofx_config c;
+    int w, h;
+    size_t n;
+    uint8_t *dm = nullptr;
+    double Bw;
+    int k = 0;  // index of the last frame seen
     int ref_idx = 0;
     bool have_prev = false, have_prev_adj = false;
-    for (int k = 1; k < nf; k++) {
-        HIPCHECK(hipMemcpyAsync(fcur, frames + (size_t)k * n, n * sizeof(float), hipMemcpyHostToDevice, s));
-        auto tl = run_leg(fref, fcur, have_prev);
+
+    Seq(ofx_engine *e, Lane *l, const ofx_config &cfg, int w_, int h_)
+        : eng(e), lane(l), ws(*l->ws), s(l->stream), c(cfg), w(w_), h(h_), n((size_t)w_ * h_),
+          Bw(edge_guard(cfg) > 0 ? 4.0 * cfg.sigma_w : 0.0) {}
+
+    // one leg a -> b from the start (ix, iy) or the coarse stage alone; its raw field and
+    // confidence go to (ox, oy, oc); returns the coarse and fine seconds
+    std::pair<double, double> run_leg_to(const float *a, const float *b, const double *ix, const double *iy, double *ox,
+                                         double *oy, double *oc) {
+        for (auto &ev : ws.ev)
+            if (!ev) HIPCHECK(hipEventCreate(&ev));
+        HIPCHECK(hipEventRecord(ws.ev[0], s));
+        ws.prof.mark(ST_COARSE, s);
+        double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
+        coarse_dense(a, b, w, h, c, ix, iy, ux, uy, ws.coarse, s);
+        HIPCHECK(hipEventRecord(ws.ev[1], s));
+        refine_device(ws, a, b, w, h, w, h, 0, 0, c, s);
+        HIPCHECK(hipEventRecord(ws.ev[2], s));
+        HIPCHECK(hipMemcpyAsync(ox, ws.bux.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHECK(hipMemcpyAsync(oy, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHECK(hipMemcpyAsync(oc, ws.bconf64.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHECK(hipEventSynchronize(ws.ev[2]));
+        return std::make_pair(elapsed(ws.ev[0], ws.ev[1]) * 1e-3, elapsed(ws.ev[1], ws.ev[2]) * 1e-3);
+    }
+    // the direct leg reference -> current into (lx, ly, lc), from the previous leg's raw field
+    std::pair<double, double> run_leg(bool with_init) {
+        return run_leg_to((const float *)ws.bframe_ref.p, (const float *)ws.bi2.p,
+                          with_init ? (const double *)ws.bprev_x.p : nullptr,
+                          with_init ? (const double *)ws.bprev_y.p : nullptr, (double *)ws.bstart_x.p,
+                          (double *)ws.bstart_y.p, (double *)ws.bleg_c.p);
+    }
+    // mean link confidence of the direct leg over the reportable ROI pixels (track.rs:986-1023)
+    double leg_mean() {
+        double *macc = B<double>(ws.bmean, 2 + 2 * (size_t)h + n);
+        mean_conf((const double *)ws.bleg_c.p, (const double *)ws.bstart_x.p, (const double *)ws.bstart_y.p,
+                  (const double *)ws.bu0x.p, (const double *)ws.bu0y.p, dm, Bw, w, h, macc, s);
+        double acc[2];
+        HIPCHECK(hipMemcpyAsync(acc, macc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        return acc[1] > 0 ? acc[0] / acc[1] : 0.0;
+    }
+
+    void begin(const float *frame0, bool device_frame, const uint8_t *mask) {
+        if (mask) {
+            dm = B<uint8_t>(ws.bmask, n);
+            HIPCHECK(hipMemcpyAsync(dm, mask, n, hipMemcpyHostToDevice, s));
+        }
+        float *fref = B<float>(ws.bframe_ref, n), *fprev = B<float>(ws.bframe_prev, n);
+        (void)B<float>(ws.bi2, n);
+        const hipMemcpyKind kind = device_frame ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        HIPCHECK(hipMemcpyAsync(fref, frame0, n * sizeof(float), kind, s));
+        HIPCHECK(hipMemcpyAsync(fprev, frame0, n * sizeof(float), kind, s));
+        double *u0x = B<double>(ws.bu0x, n), *u0y = B<double>(ws.bu0y, n), *c0r = B<double>(ws.bc0r, n);
+        for (DevBuf *b : {&ws.bcx, &ws.bcy, &ws.bconf0, &ws.bca, &ws.bprev_x, &ws.bprev_y, &ws.bprev_c, &ws.bleg_c,
+                          &ws.bstart_x, &ws.bstart_y})
+            (void)B<double>(*b, n);
+        hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, u0x, 0.0, n);
+        hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, u0y, 0.0, n);
+        hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, c0r, 1.0, n);
+        LAUNCHCHECK();
+        if (c.seq_hybrid) {
+            uint8_t *anch = B<uint8_t>(ws.banch, n), *good_prev = B<uint8_t>(ws.bgood_prev, n);
+            double *upx = B<double>(ws.bupx, n), *upy = B<double>(ws.bupy, n), *caa = B<double>(ws.bca_adj, n);
+            int *nfb = B<int>(ws.bnfb, n);
+            std::vector<uint8_t> ones8(n, 1);
+            std::vector<int> zi(n, 0);
+            HIPCHECK(hipMemcpyAsync(anch, ones8.data(), n, hipMemcpyHostToDevice, s));
+            HIPCHECK(hipMemcpyAsync(good_prev, ones8.data(), n, hipMemcpyHostToDevice, s));
+            HIPCHECK(hipMemcpyAsync(nfb, zi.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, upx, 0.0, n);
+            hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, upy, 0.0, n);
+            hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, caa, 0.0, n);
+            LAUNCHCHECK();
+            HIPCHECK(hipStreamSynchronize(s));  // the host vectors above go out of scope
+        }
+        HIPCHECK(hipStreamSynchronize(s));
+        k = 0;
+        ref_idx = 0;
+        have_prev = have_prev_adj = false;
+    }
+
+    void step(const float *frame, bool device_frame, ofx_result *out, bool device_out) {
+        k++;
+        HIPCHECK(hipMemcpyAsync(ws.bi2.p, frame, n * sizeof(float),
+                                device_frame ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+        if (c.seq_hybrid)
+            step_hybrid(out, device_out);
+        else
+            step_chain(out, device_out);
+    }
+
+    // the result planes: staged in the workspace and downloaded for a host result, written in
+    // place for a device one
+    Staged outputs(ofx_result *out, bool device_out) {
+        if (device_out) {
+            Staged st;
+            st.d = *out;
+            return st;
+        }
+        return stage_outputs(ws, n, out);
+    }
+    void deliver(const Staged &st, ofx_result *out, bool device_out) {
+        if (device_out)
+            HIPCHECK(hipStreamSynchronize(s));
+        else
+            download(st, out, n, s);
+    }
+
+    void step_chain(ofx_result *out, bool device_out) {
+        float *fref = (float *)ws.bframe_ref.p, *fprev = (float *)ws.bframe_prev.p, *fcur = (float *)ws.bi2.p;
+        double *u0x = (double *)ws.bu0x.p, *u0y = (double *)ws.bu0y.p, *c0r = (double *)ws.bc0r.p;
+        double *cx = (double *)ws.bcx.p, *cy = (double *)ws.bcy.p, *conf0 = (double *)ws.bconf0.p;
+        double *ca = (double *)ws.bca.p, *lc = (double *)ws.bleg_c.p;
+        double *px = (double *)ws.bprev_x.p, *py = (double *)ws.bprev_y.p, *pc = (double *)ws.bprev_c.p;
+        double *lx = (double *)ws.bstart_x.p, *ly = (double *)ws.bstart_y.p;
+        auto tl = run_leg(have_prev);
+        double cm = leg_mean();
+        if (cm < c.ref_update_conf && ref_idx != k - 1) {
+            // prev is the leg old reference -> k - 1: its confidence extends frame 0's link
+            conf_at(pc, u0x, u0y, w, h, ca, s);
+            hipLaunchKernelGGL(k_min_inplace, dim3((n + 255) / 256), dim3(256), 0, s, c0r, (const double *)ca, n);
+            compose(u0x, u0y, px, py, w, h, cx, cy, s);
+            HIPCHECK(hipMemcpyAsync(u0x, cx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipMemcpyAsync(u0y, cy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            HIPCHECK(hipMemcpyAsync(fref, fprev, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+            ref_idx = k - 1;
+            tl = run_leg(false);
+            cm = leg_mean();
+        }
+        compose(u0x, u0y, lx, ly, w, h, cx, cy, s);
+        conf_at(lc, u0x, u0y, w, h, ca, s);
+        hipLaunchKernelGGL(k_min_pair, dim3((n + 255) / 256), dim3(256), 0, s, (const double *)c0r, (const double *)ca,
+                           conf0, (float *)ws.bconf.p, n);
+        LAUNCHCHECK();
+        Staged st = outputs(out, device_out);
+        finish_device(ws, cx, cy, (const float *)ws.bconf.p, nullptr, w, h, dm, c, &st.d, s);
+        to_f32(cx, st.d.raw_ux, n, s);
+        to_f32(cy, st.d.raw_uy, n, s);
+        if (st.d.conf) HIPCHECK(hipMemcpyAsync(st.d.conf, ws.bconf.p, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        ws.prof.mark(ST_END, s);
+        HIPCHECK(hipEventRecord(ws.ev[3], s));
+        deliver(st, out, device_out);
+        ws.prof.flush();
+        HIPCHECK(hipEventSynchronize(ws.ev[3]));
+        out->t_coarse = tl.first;
+        out->t_fine = tl.second;
+        out->t_strain = elapsed(ws.ev[2], ws.ev[3]) * 1e-3;
+        out->reference = ref_idx;
+        out->mean_conf = cm;
+        // this leg becomes prev
+        HIPCHECK(hipMemcpyAsync(px, lx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHECK(hipMemcpyAsync(py, ly, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHECK(hipMemcpyAsync(pc, lc, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        HIPCHECK(hipMemcpyAsync(fprev, fcur, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        have_prev = true;
+        HIPCHECK(hipStreamSynchronize(s));
+    }
+
+    // seq_mode "hybrid": the direct leg of the chain plus, per pixel, the gated adjacent-frame
+    // fallback. The host reads back the few counters that decide which legs and gates run.
+    void step_hybrid(ofx_result *out, bool device_out) {
+        float *fref = (float *)ws.bframe_ref.p, *fprev = (float *)ws.bframe_prev.p, *fcur = (float *)ws.bi2.p;
+        double *u0x = (double *)ws.bu0x.p, *u0y = (double *)ws.bu0y.p, *c0r = (double *)ws.bc0r.p;
+        double *cx = (double *)ws.bcx.p, *cy = (double *)ws.bcy.p, *ca = (double *)ws.bca.p;
+        double *px = (double *)ws.bprev_x.p, *py = (double *)ws.bprev_y.p, *pc = (double *)ws.bprev_c.p;
+        double *lc = (double *)ws.bleg_c.p, *lx = (double *)ws.bstart_x.p, *ly = (double *)ws.bstart_y.p;
+        double *legc = B<double>(ws.bconf0, n);
+        uint8_t *anch = B<uint8_t>(ws.banch, n), *good_prev = B<uint8_t>(ws.bgood_prev, n);
+        double *upx = B<double>(ws.bupx, n), *upy = B<double>(ws.bupy, n);
+        int *nfb = B<int>(ws.bnfb, n);
+        double *pajx = B<double>(ws.bpadj_x, n), *pajy = B<double>(ws.bpadj_y, n);
+        double *ajx = B<double>(ws.badj_x, n), *ajy = B<double>(ws.badj_y, n), *ajc = B<double>(ws.badj_c, n);
+        double *link = B<double>(ws.blink, n);
+        uint8_t *flags = B<uint8_t>(ws.bflags, n);
+        double *ax = B<double>(ws.bax, n), *ay = B<double>(ws.bay, n);
+        double *cax = B<double>(ws.bcax, n), *cay = B<double>(ws.bcay, n), *caa = B<double>(ws.bca_adj, n);
+        uint8_t *gate0 = B<uint8_t>(ws.bgate0, n), *gate = B<uint8_t>(ws.bgate, n), *hlow = B<uint8_t>(ws.bhlow, n);
+        int *cnt = B<int>(ws.bhcnt, HYB_NCOUNTERS);
+        void *gscr = ws.bgscr.get<char>(gate_scratch_bytes(w, h));
+        double *hux = B<double>(ws.bhux, n), *huy = B<double>(ws.bhuy, n);
+        int32_t *dfb = B<int32_t>(ws.bfb, n);
+        auto counters = [&]() {
+            std::vector<int> hc(HYB_NCOUNTERS);
+            HIPCHECK(hipMemcpyAsync(hc.data(), cnt, HYB_NCOUNTERS * sizeof(int), hipMemcpyDeviceToHost, s));
+            HIPCHECK(hipStreamSynchronize(s));
+            return hc;
+        };
+        const bool gate_on = c.fb_min_support > 0;
+        const double radius = fb_radius_px(c);
+        auto tl = run_leg(have_prev);
         double cm = leg_mean();
         if (cm < c.ref_update_conf && ref_idx != k - 1) {
             conf_at(pc, u0x, u0y, w, h, ca, s);
@@ -577,7 +747,7 @@ void sequence_hybrid(Workspace &ws, const float *frames, int nf, int w, int h, c
             HIPCHECK(hipMemcpyAsync(u0y, upy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
             HIPCHECK(hipMemcpyAsync(fref, fprev, n * sizeof(float), hipMemcpyDeviceToDevice, s));
             ref_idx = k - 1;
-            tl = run_leg(fref, fcur, false);
+            tl = run_leg(false);
             cm = leg_mean();
         }
         compose(u0x, u0y, lx, ly, w, h, cx, cy, s);
@@ -623,8 +793,7 @@ void sequence_hybrid(Workspace &ws, const float *frames, int nf, int w, int h, c
             hyb_gate(flags, dm, cx, cy, ax, ay, w, h, radius, c.fb_min_support, c.fb_tol, gscr, gate, s);
             g1 = gate;
         }
-        ofx_result *out = &results[k - 1];
-        Staged st = stage_outputs(ws, n, out);
+        Staged st = outputs(out, device_out);
         hyb_output(flags, g1, cx, cy, ax, ay, link, caa, nfb, c.fb_max, dm, hux, huy, (float *)ws.bconf.p, hlow,
                    dfb, cnt, n, s);
         HIPCHECK(hipEventRecord(ws.ev[2], s));
@@ -634,8 +803,10 @@ void sequence_hybrid(Workspace &ws, const float *frames, int nf, int w, int h, c
         if (st.d.conf) HIPCHECK(hipMemcpyAsync(st.d.conf, ws.bconf.p, n * sizeof(float), hipMemcpyDeviceToDevice, s));
         ws.prof.mark(ST_END, s);
         HIPCHECK(hipEventRecord(ws.ev[3], s));
-        if (out->fb) HIPCHECK(hipMemcpyAsync(out->fb, dfb, n * sizeof(int32_t), hipMemcpyDeviceToHost, s));
-        download(st, out, n, s);
+        if (out->fb)
+            HIPCHECK(hipMemcpyAsync(out->fb, dfb, n * sizeof(int32_t),
+                                    device_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+        deliver(st, out, device_out);
         ws.prof.flush();
         const std::vector<int> hc = counters();
         const double nroi = (double)std::max(hc[HYB_N_ROI], 1);
@@ -658,9 +829,9 @@ void sequence_hybrid(Workspace &ws, const float *frames, int nf, int w, int h, c
         HIPCHECK(hipMemcpyAsync(pc, lc, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         HIPCHECK(hipMemcpyAsync(fprev, fcur, n * sizeof(float), hipMemcpyDeviceToDevice, s));
         have_prev = true;
+        HIPCHECK(hipStreamSynchronize(s));
     }
-    HIPCHECK(hipStreamSynchronize(s));
-}
+};
 }  // namespace
 
 }  // namespace ofx
@@ -942,125 +1113,75 @@ int ofx_track_pair(ofx_engine *e, const float *i1, const float *i2, int w, int h
     });
 }
 
-// track.py:260-315 (seq_mode chain) with the legs, composition and link confidence on the GPU.
+// track.py:260-315 / track/hybrid.rs:262-540 over a whole frame stack: begin, one step per
+// frame, end, on one lane.
 int ofx_track_sequence(ofx_engine *e, const float *frames, int nf, int w, int h, const uint8_t *mask,
                        const ofx_config *cfg, ofx_result *results) {
     return guarded([&] {
         validate(cfg);
         if (!frames || !results) throw Error(OFX_EINVAL, "frames or results is NULL");
         if (nf < 1) throw Error(OFX_EINVAL, "no frames");
-        check_window_impl(w, h, *cfg);
-        if (std::min(w, h) < cfg->poc_win)
-            throw Error(OFX_EINVAL, "image smaller than poc_win (" + std::to_string(cfg->poc_win) + ")");
-        const size_t n = (size_t)w * h;
+        check_sequence_args(w, h, mask, *cfg);
         LaneGuard g(e);
-        Workspace &ws = *g.lane->ws;
-        hipStream_t s = g.lane->stream;
-        const ofx_config &c = *cfg;
-        uint8_t *dm = nullptr;
-        if (mask) {
-            dm = B<uint8_t>(ws.bmask, n);
-            HIPCHECK(hipMemcpyAsync(dm, mask, n, hipMemcpyHostToDevice, s));
-        }
-        // frame buffers: reference, previous, current
-        float *fref = B<float>(ws.bframe_ref, n), *fprev = B<float>(ws.bframe_prev, n), *fcur = B<float>(ws.bi2, n);
-        double *u0x = B<double>(ws.bu0x, n), *u0y = B<double>(ws.bu0y, n), *c0r = B<double>(ws.bc0r, n);
-        double *cx = B<double>(ws.bcx, n), *cy = B<double>(ws.bcy, n), *conf0 = B<double>(ws.bconf0, n);
-        double *ca = B<double>(ws.bca, n);
-        double *px = B<double>(ws.bprev_x, n), *py = B<double>(ws.bprev_y, n), *pc = B<double>(ws.bprev_c, n);
-        double *lc = B<double>(ws.bleg_c, n);
-        double *macc = B<double>(ws.bmean, 2 + 2 * (size_t)h + n);
-        double *lx = B<double>(ws.bstart_x, n), *ly = B<double>(ws.bstart_y, n);  // leg raw field
-        HIPCHECK(hipMemcpyAsync(fref, frames, n * sizeof(float), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(fprev, frames, n * sizeof(float), hipMemcpyHostToDevice, s));
-        std::vector<double> zeros(n, 0.0), ones(n, 1.0);
-        HIPCHECK(hipMemcpyAsync(u0x, zeros.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(u0y, zeros.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(c0r, ones.data(), n * sizeof(double), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipStreamSynchronize(s));
-        const double Bw = edge_guard(c) > 0 ? 4.0 * c.sigma_w : 0.0;
-        int ref_idx = 0;
-        bool have_prev = false;
-        ofx_result leg{};  // legs write nothing but the workspace's fields
-        // one leg a -> b from the start (ix, iy) or the coarse stage alone; its raw field and
-        // confidence go to (ox, oy, oc); returns the coarse and fine seconds
-        auto run_leg_to = [&](const float *a, const float *b, const double *ix, const double *iy, double *ox,
-                              double *oy, double *oc) {
-            for (auto &ev : ws.ev)
-                if (!ev) HIPCHECK(hipEventCreate(&ev));
-            HIPCHECK(hipEventRecord(ws.ev[0], s));
-            ws.prof.mark(ST_COARSE, s);
-            double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
-            coarse_dense(a, b, w, h, c, ix, iy, ux, uy, ws.coarse, s);
-            HIPCHECK(hipEventRecord(ws.ev[1], s));
-            refine_device(ws, a, b, w, h, w, h, 0, 0, c, s);
-            HIPCHECK(hipEventRecord(ws.ev[2], s));
-            HIPCHECK(hipMemcpyAsync(ox, ws.bux.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HIPCHECK(hipMemcpyAsync(oy, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HIPCHECK(hipMemcpyAsync(oc, ws.bconf64.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HIPCHECK(hipEventSynchronize(ws.ev[2]));
-            return std::make_pair(elapsed(ws.ev[0], ws.ev[1]) * 1e-3, elapsed(ws.ev[1], ws.ev[2]) * 1e-3);
-        };
-        auto run_leg = [&](const float *a, const float *b, bool with_init) {
-            return run_leg_to(a, b, with_init ? px : nullptr, with_init ? py : nullptr, lx, ly, lc);
-        };
-        auto leg_mean = [&]() {
-            mean_conf(lc, lx, ly, u0x, u0y, dm, Bw, w, h, macc, s);
-            double acc[2];
-            HIPCHECK(hipMemcpyAsync(acc, macc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-            HIPCHECK(hipStreamSynchronize(s));
-            return acc[1] > 0 ? acc[0] / acc[1] : 0.0;
-        };
-        if (c.seq_hybrid) {
-            sequence_hybrid(ws, frames, nf, w, h, dm, c, results, s, run_leg, run_leg_to, leg_mean);
-            return;
-        }
-        for (int k = 1; k < nf; k++) {
-            HIPCHECK(hipMemcpyAsync(fcur, frames + (size_t)k * n, n * sizeof(float), hipMemcpyHostToDevice, s));
-            auto tl = run_leg(fref, fcur, have_prev);
-            double cm = leg_mean();
-            if (cm < c.ref_update_conf && ref_idx != k - 1) {
-                // prev is the leg old reference -> k - 1: its confidence extends frame 0's link
-                conf_at(pc, u0x, u0y, w, h, ca, s);
-                hipLaunchKernelGGL(k_min_inplace, dim3((n + 255) / 256), dim3(256), 0, s, c0r, (const double *)ca, n);
-                compose(u0x, u0y, px, py, w, h, cx, cy, s);
-                HIPCHECK(hipMemcpyAsync(u0x, cx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-                HIPCHECK(hipMemcpyAsync(u0y, cy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-                HIPCHECK(hipMemcpyAsync(fref, fprev, n * sizeof(float), hipMemcpyDeviceToDevice, s));
-                ref_idx = k - 1;
-                tl = run_leg(fref, fcur, false);
-                cm = leg_mean();
-            }
-            compose(u0x, u0y, lx, ly, w, h, cx, cy, s);
-            conf_at(lc, u0x, u0y, w, h, ca, s);
-            hipLaunchKernelGGL(k_min_pair, dim3((n + 255) / 256), dim3(256), 0, s, (const double *)c0r,
-                               (const double *)ca, conf0, (float *)ws.bconf.p, n);
-            LAUNCHCHECK();
-            ofx_result *out = &results[k - 1];
-            Staged st = stage_outputs(ws, n, out);
-            finish_device(ws, cx, cy, (const float *)ws.bconf.p, nullptr, w, h, dm, c, &st.d, s);
-            to_f32(cx, st.d.raw_ux, n, s);
-            to_f32(cy, st.d.raw_uy, n, s);
-            if (st.d.conf) HIPCHECK(hipMemcpyAsync(st.d.conf, ws.bconf.p, n * sizeof(float), hipMemcpyDeviceToDevice, s));
-            ws.prof.mark(ST_END, s);
-            HIPCHECK(hipEventRecord(ws.ev[3], s));
-            download(st, out, n, s);
-            ws.prof.flush();
-            out->t_coarse = tl.first;
-            out->t_fine = tl.second;
-            out->t_strain = elapsed(ws.ev[2], ws.ev[3]) * 1e-3;
-            out->reference = ref_idx;
-            out->mean_conf = cm;
-            // this leg becomes prev
-            HIPCHECK(hipMemcpyAsync(px, lx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HIPCHECK(hipMemcpyAsync(py, ly, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HIPCHECK(hipMemcpyAsync(pc, lc, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HIPCHECK(hipMemcpyAsync(fprev, fcur, n * sizeof(float), hipMemcpyDeviceToDevice, s));
-            have_prev = true;
-        }
-        HIPCHECK(hipStreamSynchronize(s));
-        (void)leg;
+        Seq q(e, g.lane, *cfg, w, h);
+        q.begin(frames, false, mask);
+        for (int k = 1; k < nf; k++) q.step(frames + (size_t)k * q.n, false, &results[k - 1], false);
     });
+}
+
+// The streaming form: the sequence holds its lane from ofx_seq_begin to ofx_seq_end.
+struct ofx_sequence {
+    ofx_engine *e = nullptr;
+    Lane *lane = nullptr;
+    Seq *q = nullptr;
+};
+
+int ofx_seq_begin(ofx_engine *e, const float *frame0, int w, int h, const uint8_t *mask, const ofx_config *cfg,
+                  ofx_sequence **out) {
+    return guarded([&] {
+        validate(cfg);
+        if (!frame0 || !out) throw Error(OFX_EINVAL, "frame0 or out is NULL");
+        *out = nullptr;
+        check_sequence_args(w, h, mask, *cfg);
+        LaneGuard g(e);
+        std::unique_ptr<Seq> q(new Seq(e, g.lane, *cfg, w, h));
+        q->begin(frame0, false, mask);
+        ofx_sequence *sq = new ofx_sequence;
+        sq->e = e;
+        sq->lane = g.lane;
+        sq->q = q.release();
+        g.lane = nullptr;  // the sequence keeps the lane
+        *out = sq;
+    });
+}
+
+int ofx_seq_step(ofx_sequence *sq, const float *frame, ofx_result *out) {
+    return guarded([&] {
+        if (!sq || !sq->q) throw Error(OFX_EINVAL, "sequence is NULL or ended");
+        if (!frame || !out) throw Error(OFX_EINVAL, "frame or out is NULL");
+        HIPCHECK(hipSetDevice(sq->e->device));
+        sq->q->step(frame, false, out, false);
+    });
+}
+
+int ofx_seq_step_device(ofx_sequence *sq, const float *d_frame, ofx_result *d_out) {
+    return guarded([&] {
+        if (!sq || !sq->q) throw Error(OFX_EINVAL, "sequence is NULL or ended");
+        if (!d_frame || !d_out) throw Error(OFX_EINVAL, "frame or out is NULL");
+        HIPCHECK(hipSetDevice(sq->e->device));
+        sq->q->step(d_frame, true, d_out, true);
+    });
+}
+
+void ofx_seq_end(ofx_sequence *sq) {
+    if (!sq) return;
+    if (sq->q) {
+        (void)hipSetDevice(sq->e->device);
+        (void)hipStreamSynchronize(sq->lane->stream);
+        delete sq->q;
+    }
+    if (sq->lane) sq->lane->busy.store(false);
+    delete sq;
 }
 
 }  // extern "C"

@@ -116,3 +116,13 @@ def test_crop_plan_and_embed():
     import pytest
     with pytest.raises(ValueError, match="ROI mask is empty"):
         io.Crop.plan(np.zeros((100, 100), bool), cfg)
+
+
+def test_sequence_handle_errors_need_no_device():
+    """A NULL or ended sequence is an OFX_EINVAL with a message, not a crash; ofx_seq_end(NULL) is
+    a no-op (these paths touch no device)."""
+    lib = ofx.lib()
+    assert lib.ofx_seq_step(None, None, None) == ofx.OFX_EINVAL
+    assert b"sequence is NULL" in lib.ofx_last_error()
+    assert lib.ofx_seq_step_device(None, None, None) == ofx.OFX_EINVAL
+    lib.ofx_seq_end(None)

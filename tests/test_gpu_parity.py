@@ -224,3 +224,27 @@ def test_sequence_hybrid_fixtures(eng, name, fb_max):
         assert e_fix < DISP_TOL and e_orc < DISP_TOL
         if name == "A":
             assert rms(r.conf.astype(np.float64), load("hybA_conf")[k - 1]) < CONF_TOL
+
+
+@pytest.mark.parametrize("mode", ["chain", "hybrid"])
+def test_streaming_sequence_matches_the_frame_stack(eng, mode):
+    """ofx_seq_begin / ofx_seq_step (one frame per call, the Sequence iterator of
+    track.rs:1112-1225) run the same kernels in the same order as ofx_track_sequence over the
+    whole stack, so every output plane and timing share is bit-identical."""
+    frames = [f32(f) for f in (load("seq_frames") if mode == "chain" else load("hyb_frames"))]
+    cfg = Config(ref_update_conf=2.0) if mode == "chain" else Config(seq_mode="hybrid", ref_update_conf=0.8)
+    whole = eng.track_sequence(frames, cfg)
+    print(f"streaming {mode}: {len(frames)} frames", flush=True)
+    streamed = list(eng.iter_sequence(iter(frames), cfg))
+    assert [k for k, _ in streamed] == [k for k, _ in whole]
+    for (k, a), (_, b) in zip(whole, streamed):
+        for name in ("ux", "uy", "conf", "raw_ux", "raw_uy"):
+            np.testing.assert_array_equal(getattr(a, name), getattr(b, name), err_msg=f"frame {k} {name}")
+        for name in a.strain:
+            np.testing.assert_array_equal(a.strain[name], b.strain[name], err_msg=f"frame {k} {name}")
+        for key in ("reference", "mean_conf", "n_fb_max", "fallback_frac", "fb_gate_rejected", "reacquired"):
+            assert a.timing.get(key) == b.timing.get(key), (k, key)
+        if mode == "hybrid":
+            np.testing.assert_array_equal(a.fb, b.fb)
+    # the sequence released its lane: a pair call on the same engine runs
+    eng.track_pair(frames[0], frames[1], Config())
