@@ -162,6 +162,24 @@ int ofx_track_pair_device(ofx_engine *e, const float *d_i1, const float *d_i2, i
 int ofx_track_sequence(ofx_engine *e, const float *frames, int n, int w, int h,
                        const uint8_t *mask, const ofx_config *cfg, ofx_result *results);
 
+/* Temporal averaging (rust/crates/optflow-cli/src/tavg.rs:118-207; python/optflow_ref/tavg.py:30-96):
+ * out (w x h) is the mean of the nf frames (nf x h x w, host) around frames[centre], summed in f64
+ * in the order centre, then the others. align 0: the plain mean; 1: every other frame is first
+ * aligned to the centre by the robust affine fit to the coarse POC grid (tavg_model "affine";
+ * only grid points inside mask, if given); 2: by the dense coarse field (tavg_model "dense").
+ * A frame whose motion cannot be estimated is left out, or enters unaligned with
+ * keep_unaligned. *n_used (may be NULL) is the number of frames in the mean. */
+int ofx_tavg_window(ofx_engine *e, const float *frames, int nf, int centre, int w, int h, int align,
+                    const uint8_t *mask, int keep_unaligned, const ofx_config *cfg, float *out, int *n_used);
+
+/* Frame conversion (rust/crates/optflow-cli/src/io.rs:44-52, :259-281; python/optflow_ref/io.py:15-54):
+ * pixels (h x w x channels, channels 1, 3 or 4, 8 or 16 bit, host) -> grey levels in [0, 1]
+ * (v / 255 or v / 65535; colour 0.299 R + 0.587 G + 0.114 B), box downscaled by factor (the
+ * remainder cropped), f32 (w / factor x h / factor), bit-identical to the reference's f64 values
+ * rounded to f32. */
+int ofx_frame_to_gray(ofx_engine *e, const void *pixels, int bits, int channels, int w, int h, int factor,
+                      float *out);
+
 /* The streaming form of track_sequence, one frame at a time (the Sequence iterator of
  * track.rs:1112-1225; track.py:260-315 and track/hybrid.rs:262-540): ofx_seq_begin takes frame 0
  * (w x h, host) and the optional ROI, and holds one lane of the engine until ofx_seq_end;

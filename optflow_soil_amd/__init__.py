@@ -22,6 +22,7 @@ OFX_OK, OFX_EINVAL, OFX_ETOOLARGE, OFX_EDEVICE, OFX_ENODEVICE, OFX_EBUSY, OFX_EU
 SYMBOLS = ("ofx_config_default", "ofx_create", "ofx_destroy", "ofx_last_error", "ofx_lanes", "ofx_returns_slope",
            "ofx_check_window", "ofx_release_caches", "ofx_refine_raw", "ofx_coarse_flow", "ofx_track_pair",
            "ofx_track_pair_device", "ofx_seq_begin", "ofx_seq_step", "ofx_seq_step_device", "ofx_seq_end",
+           "ofx_tavg_window", "ofx_frame_to_gray",
            "ofx_track_sequence", "ofx_profile", "ofx_profile_read", "ofx_profile_stage")
 
 
@@ -266,6 +267,42 @@ class Engine:
         c = to_c(cfg)
         _check(lib().ofx_track_sequence(self._h, _p(fr), n, w, h, _p(m), ctypes.byref(c), rs))
         return [(k + 1, _to_result(rs[k], *keep[k])) for k in range(n - 1)]
+
+    def tavg_window(self, frames, centre, model=None, cfg=None, mask=None, keep_unaligned=False):
+        """tavg.rs:168-207 / tavg.py:78-96 on the GPU: (the mean of the window as float32, the
+        number of frames in it). model None: plain; "affine" / "dense": every other frame aligned
+        to frames[centre] first (tavg.rs:118-163)."""
+        fr = _f32(np.stack(frames))
+        nf, h, w = fr.shape
+        align = {None: 0, "affine": 1, "dense": 2}[model]
+        m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+        if m is not None and m.shape != (h, w):
+            raise ValueError(f"ROI mask shape {m.shape} differs from the frame shape {(h, w)}")
+        out = np.empty((h, w), np.float32)
+        used = ctypes.c_int()
+        c = to_c(cfg or Config())
+        _check(lib().ofx_tavg_window(self._h, _p(fr), nf, int(centre), w, h, align, _p(m), int(bool(keep_unaligned)),
+                                     ctypes.byref(c), _p(out), ctypes.byref(used)))
+        return out, used.value
+
+    def frame_to_gray(self, pixels, factor=1):
+        """io.rs:44-52, :259-281 on the GPU: uint8 / uint16 pixels (H, W) or (H, W, 1|3|4) ->
+        float32 grey levels in [0, 1], box downscaled by the integer factor."""
+        a = np.ascontiguousarray(pixels)
+        if a.dtype not in (np.uint8, np.uint16):
+            if np.issubdtype(a.dtype, np.floating) and a.ndim == 2 and factor == 1:
+                return a.astype(np.float32)
+            raise ValueError(f"unsupported pixel type {a.dtype}")
+        if a.ndim not in (2, 3):
+            raise ValueError(f"unsupported image shape {a.shape}")
+        h, w = a.shape[:2]
+        ch = 1 if a.ndim == 2 else a.shape[2]
+        if factor < 1:
+            raise ValueError(f"scale must be >= 1, got {factor}")
+        out = np.empty((h // factor, w // factor), np.float32)
+        _check(lib().ofx_frame_to_gray(self._h, a.ctypes.data_as(ctypes.c_void_p), 8 * a.itemsize, ch, w, h,
+                                       int(factor), _p(out)))
+        return out
 
     def sequence(self, frame0, cfg, mask=None):
         """Opens a streaming sequence on frame 0 (the Sequence iterator of track.rs:1112-1225): a

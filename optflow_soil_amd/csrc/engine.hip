@@ -122,6 +122,7 @@ struct Workspace {
     DevBuf bconf, bconf64, blow, bkept, bwithin, bok, bcold2, bcolrow, bcnt;
     DevBuf bparent, bsizes, blarge, bsrc, bnear, bmx, bmy, bytmp, bcolsum, bpftmp;
     DevBuf bstart_x, bstart_y;
+    DevBuf btav_acc, btav_a, btav_fx, btav_fy, btav_c, btav_tmp, bpix;  // temporal averaging, frame input
     DevBuf bout;                                       // staging of host outputs (f32 x 10 + u8)
     DevBuf btaps_s, btaps_w;
     double taps_s_sigma = -1, taps_w_sigma = -1;
@@ -1036,6 +1037,115 @@ int ofx_coarse_flow(ofx_engine *e, const float *i1, const float *i2, int w, int 
         coarse_dense(d1, d2, w, h, *cfg, ix, iy, ux, uy, ws.coarse, s);
         HIPCHECK(hipMemcpyAsync(out_ux, ux, n * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipMemcpyAsync(out_uy, uy, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+    });
+}
+
+// tavg.rs:118-163 on the device: `other` (device, w x h) resampled onto the grid of `centre` by
+// the affine fit to the coarse grid (align 1, fit on the host like the reference's) or the dense
+// coarse field (align 2), into `out`; false when no motion can be estimated.
+bool align_device(Workspace &ws, const float *centre, const float *other, int w, int h, int align,
+                  const uint8_t *mask, const ofx_config &c, float *out, hipStream_t s) {
+    if (std::min(w, h) < c.poc_win) return false;  // no POC grid fits (tavg.rs:290-296)
+    const size_t n = (size_t)w * h;
+    double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
+    float *fx = B<float>(ws.btav_fx, n), *fy = B<float>(ws.btav_fy, n);
+    coarse_dense(centre, other, w, h, c, nullptr, nullptr, ux, uy, ws.coarse, s);
+    if (align == 2) {
+        f64_to_f32(ux, fx, n, s);
+        f64_to_f32(uy, fy, n, s);
+    } else {
+        std::vector<int> gxs, gys;
+        std::vector<double> gux, guy, gpk, xs, ys, vx, vy;
+        coarse_grid(ws.coarse, w, h, c, gxs, gys, gux, guy, gpk, s);
+        for (size_t iy = 0; iy < gys.size(); iy++)
+            for (size_t ix = 0; ix < gxs.size(); ix++) {
+                const size_t j = iy * gxs.size() + ix;
+                // the reference's grid is f32 (poc.rs Grid)
+                if ((float)gpk[j] > 0.0f && (!mask || mask[(size_t)gys[iy] * w + gxs[ix]])) {
+                    xs.push_back(gxs[ix]);
+                    ys.push_back(gys[iy]);
+                    vx.push_back((double)(float)gux[j]);
+                    vy.push_back((double)(float)guy[j]);
+                }
+            }
+        double cx[3], cy[3];
+        if (xs.size() < 6 || !fit_affine(xs, ys, vx, vy, cx, cy)) return false;
+        affine_field(cx, cy, w, h, fx, fy, s);
+    }
+    float *coef = B<float>(ws.btav_c, n), *tmp = B<float>(ws.btav_tmp, n);
+    HIPCHECK(hipMemcpyAsync(coef, other, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    prefilter_real(coef, tmp, w, h, s);
+    warp(coef, w, h, 0, 0, fx, fy, out, w, h, s);
+    return true;
+}
+
+int ofx_tavg_window(ofx_engine *e, const float *frames, int nf, int centre, int w, int h, int align,
+                    const uint8_t *mask, int keep_unaligned, const ofx_config *cfg, float *out, int *n_used) {
+    return guarded([&] {
+        if (!frames || !out) throw Error(OFX_EINVAL, "frames or out is NULL");
+        if (nf < 1 || centre < 0 || centre >= nf) throw Error(OFX_EINVAL, "centre outside the window");
+        if (w < 1 || h < 1) throw Error(OFX_EINVAL, "empty window");
+        if (align < 0 || align > 2) throw Error(OFX_EINVAL, "align must be 0 (none), 1 (affine) or 2 (dense)");
+        if (align) {
+            if (!cfg) throw Error(OFX_EINVAL, "aligning needs a configuration");
+            validate(cfg);
+            if (std::min(w, h) >= cfg->poc_win) check_window_impl(w, h, *cfg);
+        }
+        LaneGuard g(e);
+        Workspace &ws = *g.lane->ws;
+        hipStream_t s = g.lane->stream;
+        const size_t n = (size_t)w * h;
+        float *dc = B<float>(ws.bi1, n), *dother = B<float>(ws.bi2, n), *da = B<float>(ws.btav_a, n);
+        double *acc = B<double>(ws.btav_acc, n);
+        HIPCHECK(hipMemcpyAsync(dc, frames + (size_t)centre * n, n * sizeof(float), hipMemcpyHostToDevice, s));
+        mean_add(acc, dc, n, true, s);
+        int used = 1;
+        for (int i = 0; i < nf; i++) {
+            if (i == centre) continue;
+            HIPCHECK(hipMemcpyAsync(dother, frames + (size_t)i * n, n * sizeof(float), hipMemcpyHostToDevice, s));
+            if (!align) {
+                mean_add(acc, dother, n, false, s);
+                used++;
+            } else if (align_device(ws, dc, dother, w, h, align, mask, *cfg, da, s)) {
+                mean_add(acc, da, n, false, s);
+                used++;
+            } else if (keep_unaligned) {
+                mean_add(acc, dother, n, false, s);
+                used++;
+            }
+        }
+        if (used == 1)
+            HIPCHECK(hipMemcpyAsync(out, dc, n * sizeof(float), hipMemcpyDeviceToHost, s));
+        else {
+            float *dout = B<float>(ws.btav_fx, n);
+            mean_finish(acc, (double)used, dout, n, s);
+            HIPCHECK(hipMemcpyAsync(out, dout, n * sizeof(float), hipMemcpyDeviceToHost, s));
+        }
+        HIPCHECK(hipStreamSynchronize(s));
+        if (n_used) *n_used = used;
+    });
+}
+
+int ofx_frame_to_gray(ofx_engine *e, const void *pixels, int bits, int channels, int w, int h, int factor,
+                      float *out) {
+    return guarded([&] {
+        if (!pixels || !out) throw Error(OFX_EINVAL, "pixels or out is NULL");
+        if (bits != 8 && bits != 16) throw Error(OFX_EINVAL, "pixels must be 8 or 16 bit");
+        if (channels != 1 && channels != 3 && channels != 4) throw Error(OFX_EINVAL, "channels must be 1, 3 or 4");
+        if (w < 1 || h < 1) throw Error(OFX_EINVAL, "empty frame");
+        if (factor < 1) throw Error(OFX_EINVAL, "scale must be >= 1, got " + std::to_string(factor));
+        LaneGuard g(e);
+        Workspace &ws = *g.lane->ws;
+        hipStream_t s = g.lane->stream;
+        const size_t bytes = (size_t)w * h * channels * (bits / 8);
+        const int ow = w / factor, oh = h / factor;
+        if (ow < 1 || oh < 1) return;
+        char *dp = B<char>(ws.bpix, bytes);
+        float *dout = B<float>(ws.btav_fx, (size_t)ow * oh);
+        HIPCHECK(hipMemcpyAsync(dp, pixels, bytes, hipMemcpyHostToDevice, s));
+        frame_to_gray(dp, bits, channels, w, h, factor, dout, s);
+        HIPCHECK(hipMemcpyAsync(out, dout, (size_t)ow * oh * sizeof(float), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipStreamSynchronize(s));
     });
 }

@@ -65,6 +65,21 @@ struct CoarseWs;
 void coarse_dense(const float *i1, const float *i2, int w, int h, const ofx_config &c, const double *init_ux,
                   const double *init_uy, double *ux, double *uy, CoarseWs *&ws, hipStream_t s);
 void coarse_release(CoarseWs *&ws);
+void coarse_grid(const CoarseWs *ws, int w, int h, const ofx_config &c, std::vector<int> &xs, std::vector<int> &ys,
+                 std::vector<double> &ux, std::vector<double> &uy, std::vector<double> &peak, hipStream_t s);
+
+// tavg.hip: temporal averaging and frame conversion (tavg.rs, io.rs)
+// acc (f64, n) = frame (first) or acc += frame (in the order of the calls)
+void mean_add(double *acc, const float *frame, size_t n, bool first, hipStream_t s);
+void mean_finish(const double *acc, double count, float *out, size_t n, hipStream_t s);
+// ux = (float)(cx0 + cx1 x + cx2 y), likewise uy, every product and sum rounded as written
+void affine_field(const double cx[3], const double cy[3], int w, int h, float *ux, float *uy, hipStream_t s);
+void f64_to_f32(const double *in, float *out, size_t n, hipStream_t s);
+// pixels (h x w x channels, 8 or 16 bit) -> grey levels, box downscaled by factor
+void frame_to_gray(const void *pixels, int bits, int channels, int w, int h, int factor, float *out, hipStream_t s);
+// tavg.rs:87-110: the robust affine fit (host)
+bool fit_affine(const std::vector<double> &xs, const std::vector<double> &ys, const std::vector<double> &ux,
+                const std::vector<double> &uy, double cx[3], double cy[3]);
 
 // seq.hip: sequence helpers (track.py:231-328)
 void compose(const double *ax, const double *ay, const double *bx, const double *by, int w, int h, double *ox,

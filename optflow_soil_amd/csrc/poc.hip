@@ -667,6 +667,28 @@ void ensure(T *&p, size_t n, size_t &cap) {
 
 }  // namespace
 
+// The level-0 grid of the last coarse_dense call with this workspace: positions and (ux, uy,
+// peak) per point, ny x nx, on the host.
+void coarse_grid(const CoarseWs *ws, int w, int h, const ofx_config &c, std::vector<int> &xs, std::vector<int> &ys,
+                 std::vector<double> &ux, std::vector<double> &uy, std::vector<double> &peak, hipStream_t s) {
+    if (!ws || !ws->gux) throw Error(OFX_EINVAL, "no coarse grid computed");
+    int x0, nx, y0, ny;
+    grid_axis(w, c.poc_win, c.grid_step, x0, nx);
+    grid_axis(h, c.poc_win, c.grid_step, y0, ny);
+    xs.resize(nx);
+    ys.resize(ny);
+    for (int i = 0; i < nx; i++) xs[i] = x0 + i * c.grid_step;
+    for (int j = 0; j < ny; j++) ys[j] = y0 + j * c.grid_step;
+    const size_t np = (size_t)nx * ny;
+    ux.resize(np);
+    uy.resize(np);
+    peak.resize(np);
+    HIPCHECK(hipMemcpyAsync(ux.data(), ws->gux, np * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(uy.data(), ws->guy, np * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipMemcpyAsync(peak.data(), ws->gpk, np * sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHECK(hipStreamSynchronize(s));
+}
+
 void coarse_release(CoarseWs *&ws) {
     if (!ws) return;
     for (auto p : ws->pa) (void)hipFree(p);

@@ -231,6 +231,52 @@ def coarse_dense(i1, i2, cfg, init=None):
     return ox, oy
 
 
+def fit_affine(xs, ys, ux, uy):
+    """tavg.rs:87-110: ((c0, c1, c2) for ux, (c0, c1, c2) for uy) of the robust affine fit, or None."""
+    xs, ys, ux, uy = _d(xs), _d(ys), _d(ux), _d(uy)
+    cx, cy = np.empty(3), np.empty(3)
+    ok = lib().orc_fit_affine(_p(xs), _p(ys), _p(ux), _p(uy), ctypes.c_long(len(xs)), _p(cx), _p(cy))
+    return (cx, cy) if ok else None
+
+
+TAVG_MODES = {None: 0, "affine": 1, "dense": 2}
+
+
+def align_to(centre, other, cfg, mask=None, model="affine"):
+    """tavg.rs:118-163: `other` resampled onto the grid of `centre`, or None."""
+    c, o = _d(centre), _d(other)
+    out = np.empty_like(c)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    ok = lib().orc_align_to(_p(c), _p(o), c.shape[0], c.shape[1], TAVG_MODES[model], _p(m) if m is not None else None,
+                            ctypes.byref(_cfg(cfg)), _p(out))
+    return out if ok else None
+
+
+def average_window(frames, centre, model=None, cfg=None, mask=None, keep_unaligned=False):
+    """tavg.rs:168-207: the mean of the window (model None: plain; "affine" / "dense": every other
+    frame aligned to frames[centre] first). Returns (mean, number of frames in it)."""
+    fr = _d(np.stack(frames))
+    nf, h, w = fr.shape
+    out = np.empty((h, w))
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+    from optflow_soil_amd.config import Config
+    used = lib().orc_tavg_window(_p(fr), nf, centre, h, w, TAVG_MODES[model], _p(m) if m is not None else None,
+                                 int(keep_unaligned), ctypes.byref(_cfg(cfg or Config())), _p(out))
+    return out, used
+
+
+def to_gray(pixels, factor=1):
+    """io.rs:44-52, :259-281: uint8 / uint16 pixels (h, w) or (h, w, channels) -> f64 grey levels,
+    box downscaled by factor."""
+    a = np.ascontiguousarray(pixels)
+    bits = {np.dtype(np.uint8): 8, np.dtype(np.uint16): 16}[a.dtype]
+    h, w = a.shape[:2]
+    ch = 1 if a.ndim == 2 else a.shape[2]
+    out = np.empty((h // factor, w // factor)) if factor > 1 else np.empty((h, w))
+    lib().orc_to_gray(a.ctypes.data_as(ctypes.c_void_p), bits, ch, h, w, factor, _p(out))
+    return out
+
+
 def edt(inp, return_indices=False):
     """scipy.ndimage.distance_transform_edt(inp) (and the flat nearest-zero index)."""
     b = np.ascontiguousarray(inp, dtype=np.uint8)

@@ -46,6 +46,13 @@ double median(std::vector<double> v);
 void edt(const uint8_t *input, int h, int w, double *dist, int32_t *nearest);
 void large_components(const uint8_t *b, int h, int w, double min_area, uint8_t *out);
 
+// coarse stage (poc.cpp): grid positions along one axis; the dense start of track.py:131-137
+// (gx_out, gy_out: h x w, or NULL) and the level-0 grid (ny x nx, or NULL)
+std::vector<int> grid_positions(int n, int win, int step);
+void coarse_dense(const double *I1, const double *I2, int H, int W, const orc_cfg &c, const double *init_x,
+                  const double *init_y, double *gx_out, double *gy_out, std::vector<double> *grid_ux,
+                  std::vector<double> *grid_uy, std::vector<double> *grid_peak);
+
 struct Cfg {
     const orc_cfg &c;
     explicit Cfg(const orc_cfg &cc) : c(cc) {}

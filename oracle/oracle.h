@@ -95,6 +95,15 @@ int orc_coarse_flow(const double *i1, const double *i2, int h, int w, const orc_
 void orc_coarse_dense(const double *i1, const double *i2, int h, int w, const orc_cfg *cfg,
                       const double *init_ux, const double *init_uy, double *ux, double *uy);
 
+/* ---- temporal averaging and frame conversion (tavg.rs, io.rs) ---- */
+int orc_fit_affine(const double *xs, const double *ys, const double *ux, const double *uy, long n,
+                   double *cx, double *cy);
+int orc_align_to(const double *centre, const double *other, int h, int w, int mode, const uint8_t *mask,
+                 const orc_cfg *cfg, double *out);
+int orc_tavg_window(const double *frames, int nf, int centre, int h, int w, int mode, const uint8_t *mask,
+                    int keep_unaligned, const orc_cfg *cfg, double *out);
+void orc_to_gray(const void *pixels, int bits, int channels, int h, int w, int factor, double *out);
+
 /* ---- distance transform and components (scipy.ndimage semantics) ---- */
 void orc_edt(const uint8_t *input, int h, int w, double *dist, int32_t *nearest);
 void orc_large_components(const uint8_t *b, int h, int w, double min_area, uint8_t *out);

@@ -268,6 +268,9 @@ std::vector<double> downsample(const std::vector<double> &img, int h, int w, int
 
 }  // namespace
 
+// the grid positions along one axis (public form of grid_axis for the other oracle files)
+std::vector<int> grid_positions(int n, int win, int step) { return grid_axis(n, win, step); }
+
 // poc.py:172-187
 int level_window(int h, int w, int win, int &wy, int &wx) {
     int lim_h = pow2_floor(2 * h / 3), lim_w = pow2_floor(2 * w / 3);
