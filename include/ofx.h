@@ -150,6 +150,22 @@ int ofx_track_pair(ofx_engine *e, const float *i1, const float *i2, int w, int h
                    const uint8_t *mask, const float *init_ux, const float *init_uy,
                    const ofx_config *cfg, ofx_result *out);
 
+/* One pair of a batch: the host arguments of ofx_track_pair. */
+typedef struct ofx_pair {
+    const float *i1, *i2;
+    int w, h;
+    const uint8_t *mask;             /* ROI or NULL */
+    const float *init_ux, *init_uy;  /* dense start guess or NULL */
+} ofx_pair;
+
+/* Independent pairs of any sizes (many uneven pairs at once; no reference counterpart, the
+ * reference runs them one by one through track_pair, track.rs:688-861): the pairs are spread over
+ * the engine's free lanes, each lane running one pair at a time on its own stream, largest
+ * first. results[i] is what ofx_track_pair writes for pairs[i], bit for bit. Every pair is
+ * checked before any work; a pair that fails at run time does not stop the others, and the call
+ * returns the first failure with the pair's index in the message. */
+int ofx_track_pairs(ofx_engine *e, int n, const ofx_pair *pairs, const ofx_config *cfg, ofx_result *results);
+
 /* The same with every image, mask, guess and output array in device memory (HBM). */
 int ofx_track_pair_device(ofx_engine *e, const float *d_i1, const float *d_i2, int w, int h,
                           const uint8_t *d_mask, const float *d_init_ux, const float *d_init_uy,

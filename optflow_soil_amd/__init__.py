@@ -21,7 +21,7 @@ OFX_OK, OFX_EINVAL, OFX_ETOOLARGE, OFX_EDEVICE, OFX_ENODEVICE, OFX_EBUSY, OFX_EU
 # every symbol include/ofx.h declares
 SYMBOLS = ("ofx_config_default", "ofx_create", "ofx_destroy", "ofx_last_error", "ofx_lanes", "ofx_returns_slope",
            "ofx_check_window", "ofx_release_caches", "ofx_refine_raw", "ofx_coarse_flow", "ofx_track_pair",
-           "ofx_track_pair_device", "ofx_seq_begin", "ofx_seq_step", "ofx_seq_step_device", "ofx_seq_end",
+           "ofx_track_pair_device", "ofx_track_pairs", "ofx_seq_begin", "ofx_seq_step", "ofx_seq_step_device", "ofx_seq_end",
            "ofx_tavg_window", "ofx_frame_to_gray",
            "ofx_track_sequence", "ofx_profile", "ofx_profile_read", "ofx_profile_stage")
 
@@ -113,6 +113,11 @@ class Result:
     valid: np.ndarray
     timing: dict = field(default_factory=dict)
     fb: np.ndarray = None  # hybrid sequences: consecutive frames carried by the adjacent leg
+
+
+class OfxPair(ctypes.Structure):
+    _fields_ = [("i1", ctypes.c_void_p), ("i2", ctypes.c_void_p), ("w", ctypes.c_int), ("h", ctypes.c_int),
+                ("mask", ctypes.c_void_p), ("init_ux", ctypes.c_void_p), ("init_uy", ctypes.c_void_p)]
 
 
 def _new_result(h, w, with_fb=False):
@@ -233,6 +238,34 @@ class Engine:
         _check(lib().ofx_track_pair(self._h, _p(I1), _p(I2), w, h, _p(m), _p(ix), _p(iy), ctypes.byref(c),
                                     ctypes.byref(r)))
         return _to_result(r, arrs, valid)
+
+    def track_pairs(self, pairs, cfg):
+        """Independent pairs [(I1, I2), (I1, I2, mask), (I1, I2, mask, init), ...] of any sizes,
+        spread over the engine's lanes (ofx_track_pairs); returns a Result per pair, each
+        bit-identical to track_pair on it."""
+        n = len(pairs)
+        cp = (OfxPair * max(n, 1))()
+        rs = (OfxResult * max(n, 1))()
+        keep = []
+        for k, p in enumerate(pairs):
+            I1, I2 = _f32(p[0]), _f32(p[1])
+            if I1.shape != I2.shape:
+                raise ValueError(f"pair {k}: image sizes differ: {I1.shape} vs {I2.shape}")
+            h, w = I1.shape
+            mask = p[2] if len(p) > 2 else None
+            init = p[3] if len(p) > 3 else None
+            m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint8)
+            if m is not None and m.shape != (h, w):
+                raise ValueError(f"pair {k}: ROI mask shape {m.shape} differs from the image shape {(h, w)}")
+            ix = iy = None
+            if init is not None:
+                ix, iy = _f32(init[0]), _f32(init[1])
+            r, arrs, valid = _new_result(h, w)
+            cp[k] = OfxPair(_p(I1), _p(I2), w, h, _p(m), _p(ix), _p(iy))
+            rs[k] = r
+            keep.append((I1, I2, m, ix, iy, arrs, valid))
+        _check(lib().ofx_track_pairs(self._h, n, cp, ctypes.byref(to_c(cfg)), rs))
+        return [_to_result(rs[k], keep[k][5], keep[k][6]) for k in range(n)]
 
     def track_pair_device(self, d_i1, d_i2, w, h, cfg, d_out, d_mask=0, d_init=(0, 0)):
         """Device-pointer form (integers from e.g. torch's data_ptr()); d_out maps result names to

@@ -7,6 +7,7 @@
 // rust/crates/optflow-core/src/track.rs:1112-1225 (chained sequence),
 // rust/crates/optflow-gpu/src/engine.rs:633-1071 (lanes, check_window, errors).
 #include <atomic>
+#include <thread>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -514,6 +515,56 @@ void download(const Staged &st, ofx_result *host, size_t n, hipStream_t s) {
         if (hst[k]) HIPCHECK(hipMemcpyAsync(hst[k], d[k], n * sizeof(float), hipMemcpyDeviceToHost, s));
     if (host->valid) HIPCHECK(hipMemcpyAsync(host->valid, st.d.valid, n, hipMemcpyDeviceToHost, s));
     HIPCHECK(hipStreamSynchronize(s));
+}
+
+// the argument checks of track_pair (track.rs:655-663, track.py:115-125)
+void check_pair_args(const float *i1, const float *i2, int w, int h, const uint8_t *mask, const float *init_ux,
+                     const float *init_uy, const ofx_config &c, const ofx_result *out) {
+    if (!i1 || !i2 || !out) throw Error(OFX_EINVAL, "input or output pointer is NULL");
+    if ((init_ux == nullptr) != (init_uy == nullptr)) throw Error(OFX_EINVAL, "init needs both components");
+    check_window_impl(w, h, c);
+    if (std::min(w, h) < c.poc_win) throw Error(OFX_EINVAL, "image smaller than poc_win (" + std::to_string(c.poc_win) + ")");
+    if (mask) {
+        const size_t n = (size_t)w * h;
+        bool any = false;
+        for (size_t i = 0; i < n && !any; i++) any = mask[i] != 0;
+        if (!any) throw Error(OFX_EINVAL, "ROI mask is empty");
+    }
+}
+
+// One pair from host inputs to host outputs on a lane the caller holds.
+void track_pair_on_lane(Lane &lane, const float *i1, const float *i2, int w, int h, const uint8_t *mask,
+                        const float *init_ux, const float *init_uy, const ofx_config &c, ofx_result *out) {
+    Workspace &ws = *lane.ws;
+    hipStream_t s = lane.stream;
+    const size_t n = (size_t)w * h;
+    float *d1 = B<float>(ws.bi1, n), *d2 = B<float>(ws.bi2, n);
+    HIPCHECK(hipMemcpyAsync(d1, i1, n * sizeof(float), hipMemcpyHostToDevice, s));
+    HIPCHECK(hipMemcpyAsync(d2, i2, n * sizeof(float), hipMemcpyHostToDevice, s));
+    uint8_t *dm = nullptr;
+    if (mask) {
+        dm = B<uint8_t>(ws.bmask, n);
+        HIPCHECK(hipMemcpyAsync(dm, mask, n, hipMemcpyHostToDevice, s));
+    }
+    double *ix = nullptr, *iy = nullptr;
+    if (init_ux) {
+        float *fx = B<float>(ws.binx, n), *fy = B<float>(ws.biny, n);
+        HIPCHECK(hipMemcpyAsync(fx, init_ux, n * sizeof(float), hipMemcpyHostToDevice, s));
+        HIPCHECK(hipMemcpyAsync(fy, init_uy, n * sizeof(float), hipMemcpyHostToDevice, s));
+        ix = B<double>(ws.bstart_x, n);
+        iy = B<double>(ws.bstart_y, n);
+        to_f64(fx, ix, n, s);
+        to_f64(fy, iy, n, s);
+    }
+    Staged st = stage_outputs(ws, n, out);
+    pair_device(ws, d1, d2, w, h, dm, ix, iy, c, &st.d, s);
+    record_times(ws, &st.d);
+    download(st, out, n, s);
+    out->t_coarse = st.d.t_coarse;
+    out->t_fine = st.d.t_fine;
+    out->t_strain = st.d.t_strain;
+    out->reference = 0;
+    out->mean_conf = 0;
 }
 
 template <class F>
@@ -1179,47 +1230,82 @@ int ofx_track_pair(ofx_engine *e, const float *i1, const float *i2, int w, int h
                    const float *init_ux, const float *init_uy, const ofx_config *cfg, ofx_result *out) {
     return guarded([&] {
         validate(cfg);
-        if (!i1 || !i2 || !out) throw Error(OFX_EINVAL, "input or output pointer is NULL");
-        if ((init_ux == nullptr) != (init_uy == nullptr)) throw Error(OFX_EINVAL, "init needs both components");
-        check_window_impl(w, h, *cfg);
-        if (std::min(w, h) < cfg->poc_win)
-            throw Error(OFX_EINVAL, "image smaller than poc_win (" + std::to_string(cfg->poc_win) + ")");
-        const size_t n = (size_t)w * h;
-        if (mask) {
-            bool any = false;
-            for (size_t i = 0; i < n && !any; i++) any = mask[i] != 0;
-            if (!any) throw Error(OFX_EINVAL, "ROI mask is empty");
-        }
+        check_pair_args(i1, i2, w, h, mask, init_ux, init_uy, *cfg, out);
         LaneGuard g(e);
-        Workspace &ws = *g.lane->ws;
-        hipStream_t s = g.lane->stream;
-        float *d1 = B<float>(ws.bi1, n), *d2 = B<float>(ws.bi2, n);
-        HIPCHECK(hipMemcpyAsync(d1, i1, n * sizeof(float), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(d2, i2, n * sizeof(float), hipMemcpyHostToDevice, s));
-        uint8_t *dm = nullptr;
-        if (mask) {
-            dm = B<uint8_t>(ws.bmask, n);
-            HIPCHECK(hipMemcpyAsync(dm, mask, n, hipMemcpyHostToDevice, s));
+        track_pair_on_lane(*g.lane, i1, i2, w, h, mask, init_ux, init_uy, *cfg, out);
+    });
+}
+
+// Independent pairs over the engine's free lanes (the batched workload of many uneven pairs):
+// one host thread per lane pulls the next pair, largest first, and runs it on the lane's stream
+// exactly as ofx_track_pair would, so each result is bit-identical to the single call.
+int ofx_track_pairs(ofx_engine *e, int n, const ofx_pair *pairs, const ofx_config *cfg, ofx_result *results) {
+    return guarded([&] {
+        validate(cfg);
+        if (n < 0 || (n > 0 && (!pairs || !results))) throw Error(OFX_EINVAL, "pairs or results is NULL");
+        for (int i = 0; i < n; i++) {
+            const ofx_pair &p = pairs[i];
+            try {
+                check_pair_args(p.i1, p.i2, p.w, p.h, p.mask, p.init_ux, p.init_uy, *cfg, &results[i]);
+            } catch (const Error &er) {
+                throw Error(er.code, "pair " + std::to_string(i) + ": " + er.what());
+            }
         }
-        double *ix = nullptr, *iy = nullptr;
-        if (init_ux) {
-            float *fx = B<float>(ws.binx, n), *fy = B<float>(ws.biny, n);
-            HIPCHECK(hipMemcpyAsync(fx, init_ux, n * sizeof(float), hipMemcpyHostToDevice, s));
-            HIPCHECK(hipMemcpyAsync(fy, init_uy, n * sizeof(float), hipMemcpyHostToDevice, s));
-            ix = B<double>(ws.bstart_x, n);
-            iy = B<double>(ws.bstart_y, n);
-            to_f64(fx, ix, n, s);
-            to_f64(fy, iy, n, s);
+        if (!e) throw Error(OFX_EINVAL, "engine is NULL");
+        if (n == 0) return;
+        std::vector<Lane *> held;
+        for (auto &l : e->lanes) {
+            bool expected = false;
+            if (l->busy.compare_exchange_strong(expected, true)) held.push_back(l.get());
         }
-        Staged st = stage_outputs(ws, n, out);
-        pair_device(ws, d1, d2, w, h, dm, ix, iy, *cfg, &st.d, s);
-        record_times(ws, &st.d);
-        download(st, out, n, s);
-        out->t_coarse = st.d.t_coarse;
-        out->t_fine = st.d.t_fine;
-        out->t_strain = st.d.t_strain;
-        out->reference = 0;
-        out->mean_conf = 0;
+        if (held.empty()) throw Error(OFX_EBUSY, "every lane of the GPU engine is busy");
+        struct Release {
+            std::vector<Lane *> &h;
+            ~Release() {
+                for (Lane *l : h) l->busy.store(false);
+            }
+        } release{held};
+        std::vector<int> order(n);
+        for (int i = 0; i < n; i++) order[i] = i;
+        std::stable_sort(order.begin(), order.end(), [&](int a, int b) {
+            return (long long)pairs[a].w * pairs[a].h > (long long)pairs[b].w * pairs[b].h;
+        });
+        std::atomic<int> next{0};
+        std::vector<int> codes(n, OFX_OK);
+        std::vector<std::string> msgs(n);
+        auto work = [&](Lane *lane) {
+            if (hipSetDevice(e->device) != hipSuccess) {
+                for (int k; (k = next.fetch_add(1)) < n;) {
+                    codes[order[k]] = OFX_EDEVICE;
+                    msgs[order[k]] = "hipSetDevice failed";
+                }
+                return;
+            }
+            if (!lane->ws) lane->ws.reset(new Workspace);
+            for (int k; (k = next.fetch_add(1)) < n;) {
+                const int i = order[k];
+                const ofx_pair &p = pairs[i];
+                try {
+                    track_pair_on_lane(*lane, p.i1, p.i2, p.w, p.h, p.mask, p.init_ux, p.init_uy, *cfg, &results[i]);
+                } catch (const Error &er) {
+                    codes[i] = er.code;
+                    msgs[i] = er.what();
+                } catch (const std::exception &ex) {
+                    codes[i] = OFX_EDEVICE;
+                    msgs[i] = ex.what();
+                }
+            }
+        };
+        std::vector<std::thread> threads;
+        try {
+            for (size_t t = 1; t < held.size(); t++) threads.emplace_back(work, held[t]);
+        } catch (const std::exception &) {
+            // no thread for the remaining lanes: the running workers take their pairs
+        }
+        work(held[0]);
+        for (auto &t : threads) t.join();
+        for (int i = 0; i < n; i++)
+            if (codes[i] != OFX_OK) throw Error(codes[i], "pair " + std::to_string(i) + ": " + msgs[i]);
     });
 }
 

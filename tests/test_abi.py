@@ -126,3 +126,14 @@ def test_sequence_handle_errors_need_no_device():
     assert b"sequence is NULL" in lib.ofx_last_error()
     assert lib.ofx_seq_step_device(None, None, None) == ofx.OFX_EINVAL
     lib.ofx_seq_end(None)
+
+
+def test_batched_pair_checks_name_the_pair():
+    """Argument checks of ofx_track_pairs run before any device work and name the failing pair."""
+    import pytest
+    ofx_pairs = (ofx.OfxPair * 1)()
+    ofx_pairs[0] = ofx.OfxPair(None, None, 64, 64, None, None, None)
+    res = (ofx.OfxResult * 1)()
+    c = ofx.to_c(Config())
+    assert ofx.lib().ofx_track_pairs(None, 1, ofx_pairs, ctypes.byref(c), res) == ofx.OFX_EINVAL
+    assert b"pair 0" in ofx.lib().ofx_last_error()

@@ -248,3 +248,32 @@ def test_streaming_sequence_matches_the_frame_stack(eng, mode):
             np.testing.assert_array_equal(a.fb, b.fb)
     # the sequence released its lane: a pair call on the same engine runs
     eng.track_pair(frames[0], frames[1], Config())
+
+
+def test_batched_pairs_match_single_pairs(eng):
+    """ofx_track_pairs spreads uneven pairs over three lanes (threads and streams); each result is
+    bit-identical to ofx_track_pair on the same pair."""
+    import optflow_soil_amd as ofx
+    from optflow_soil_amd import synth
+    sizes = [(96, 128), (128, 128), (160, 96), (112, 112), (128, 96)]
+    pairs = []
+    for k, (h, w) in enumerate(sizes):
+        pat = synth.make_pattern(h, w, seed=20 + k)
+        f = synth.stretch_translate(0.004 * (k + 1), (0.7 * k - 1.0, 0.4), (w / 2, h / 2))
+        a = synth.as_float(synth.render(pat))
+        b = synth.as_float(synth.render(pat, f, noise=0.005, seed=k))
+        mask = np.zeros((h, w), bool)
+        mask[8:-8, 10:-6] = True
+        pairs.append((a, b, mask if k % 2 else None))
+    e3 = ofx.Engine(0, lanes=3)
+    try:
+        print("batched: 5 pairs on 3 lanes", flush=True)
+        got = e3.track_pairs(pairs, Config())
+    finally:
+        e3.close()
+    for k, (p, r) in enumerate(zip(pairs, got)):
+        one = eng.track_pair(p[0], p[1], Config(), mask=p[2])
+        for name in ("ux", "uy", "conf", "raw_ux", "raw_uy"):
+            np.testing.assert_array_equal(getattr(one, name), getattr(r, name), err_msg=f"pair {k} {name}")
+        for name in one.strain:
+            np.testing.assert_array_equal(one.strain[name], r.strain[name], err_msg=f"pair {k} {name}")
