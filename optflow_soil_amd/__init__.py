@@ -36,7 +36,8 @@ class OfxConfig(ctypes.Structure):
                 ("edge_guard_min_area", ctypes.c_double), ("phase_tol", ctypes.c_double),
                 ("strain_win", ctypes.c_int), ("ref_update_conf", ctypes.c_double), ("seq_hybrid", ctypes.c_int),
                 ("fb_max", ctypes.c_int), ("fb_radius", ctypes.c_double), ("fb_min_support", ctypes.c_int),
-                ("fb_tol", ctypes.c_double), ("prewarp_freeze_at", ctypes.c_int)]
+                ("fb_tol", ctypes.c_double), ("prewarp_freeze_at", ctypes.c_int),
+                ("tile", ctypes.c_int)]
 
 
 _FP = ctypes.POINTER(ctypes.c_float)

@@ -57,6 +57,10 @@ class Config:
     tavg_align: bool = False
     tavg_model: str = "affine"
     prewarp_freeze_at: int = 0
+    # Not a key of the reference's config: the fine stage's tiling. 0 refines the whole frame at
+    # once; N > 0 refines frames with a side over N px in N x N cores with the reference's margin,
+    # as its tiled driver does with N = 1024 (rust/crates/optflow-core/src/tile.rs:56-98, track.rs:583).
+    tile: int = 0
 
     def to_json(self) -> str:
         return json.dumps(asdict(self), indent=2)
@@ -142,6 +146,8 @@ class Config:
             raise ValueError("tavg_align must be true or false")
         if self.prewarp_freeze_at < 0:
             raise ValueError("prewarp_freeze_at must be >= 0")
+        if not isinstance(self.tile, int) or isinstance(self.tile, bool) or (self.tile != 0 and self.tile < 64):
+            raise ValueError("tile must be 0 (whole frame) or an integer >= 64")
         if (self.n_orient * len(self.scales) > 64 or self.sigma_w > 256 or self.sigma_s > 256
                 or self.strain_win > 255 or self.poc_win > 1024):
             raise ValueError("at most 64 filters (n_orient * len(scales)); sigma_w, sigma_s <= 256, "

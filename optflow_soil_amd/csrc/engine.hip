@@ -124,6 +124,7 @@ struct Workspace {
     DevBuf bparent, bsizes, blarge, bsrc, bnear, bmx, bmy, bytmp, bcolsum, bpftmp;
     DevBuf bstart_x, bstart_y;
     DevBuf btav_acc, btav_a, btav_fx, btav_fy, btav_c, btav_tmp, bpix;  // temporal averaging, frame input
+    DevBuf bfsx, bfsy, bjx, bjy, bje, bjr, btile1, btile2, bpart;         // tiled fine stage
     DevBuf bout;                                       // staging of host outputs (f32 x 10 + u8)
     DevBuf btaps_s, btaps_w;
     double taps_s_sigma = -1, taps_w_sigma = -1;
@@ -217,6 +218,7 @@ void validate(const ofx_config *cfg) {
     if (!(c.conf_min >= 0 && c.conf_min <= 1) || !(c.poc_peak_min >= 0 && c.poc_peak_min <= 1))
         throw Error(OFX_EINVAL, "conf_min and poc_peak_min must lie in [0, 1]");
     if (c.prewarp_freeze_at < 0) throw Error(OFX_EINVAL, "prewarp_freeze_at must be >= 0");
+    if (c.tile != 0 && c.tile < 64) throw Error(OFX_EINVAL, "tile must be 0 (whole frame) or an integer >= 64");
     if (c.n_orient * c.n_scales > 64 || c.sigma_w > 256 || c.sigma_s > 256 || c.strain_win > 255 || c.poc_win > 1024)
         throw Error(OFX_EINVAL,
                     "at most 64 filters (n_orient * len(scales)); sigma_w, sigma_s <= 256, strain_win <= 255 and "
@@ -390,6 +392,173 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
     confidence(energy, resid, med, ux, uy, c.phase_tol, c.conf_min, Bw, w, h, nullptr, conf, conf64, s);
 }
 
+// ---------------------------------------------------------------- tiled fine stage
+// rust/crates/optflow-core/src/tile.rs:17-146 (geometry) and :225-500 (the tiled driver and
+// join): cores of c.tile px partition the frame; each is refined on its extension (core plus
+// tile_margin, clipped) against the window of image 2 its start field reaches, and only the core
+// is kept. The confidence is formed afterwards from the joined energy (median over the frame) and
+// the joined field (border rule), as for a whole frame.
+struct TRect {
+    int x0, y0, x1, y1;
+    long long area() const { return (long long)(x1 - x0) * (y1 - y0); }
+};
+
+// per block: the finite range of (float) ux and uy over a rectangle of the frame
+__global__ void __launch_bounds__(256) k_rect_minmax(const double *__restrict__ ux, const double *__restrict__ uy,
+                                                     int w, int x0, int y0, int rw, int rh, double *__restrict__ part) {
+    __shared__ double red[4][256];
+    double lox = INFINITY, hix = -INFINITY, loy = INFINITY, hiy = -INFINITY;
+    const long long total = (long long)rw * rh;
+    for (long long k = (long long)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (long long)gridDim.x * blockDim.x) {
+        const int iy = (int)(k / rw), ix = (int)(k - (long long)iy * rw);
+        const size_t i = (size_t)(y0 + iy) * w + (x0 + ix);
+        const double a = (double)(float)ux[i], b = (double)(float)uy[i];
+        if (isfinite(a)) {
+            lox = fmin(lox, a);
+            hix = fmax(hix, a);
+        }
+        if (isfinite(b)) {
+            loy = fmin(loy, b);
+            hiy = fmax(hiy, b);
+        }
+    }
+    red[0][threadIdx.x] = lox;
+    red[1][threadIdx.x] = hix;
+    red[2][threadIdx.x] = loy;
+    red[3][threadIdx.x] = hiy;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) {
+            red[0][threadIdx.x] = fmin(red[0][threadIdx.x], red[0][threadIdx.x + off]);
+            red[1][threadIdx.x] = fmax(red[1][threadIdx.x], red[1][threadIdx.x + off]);
+            red[2][threadIdx.x] = fmin(red[2][threadIdx.x], red[2][threadIdx.x + off]);
+            red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; q++) part[4 * blockIdx.x + q] = red[q][0];
+}
+
+// the core of a refined tile (ew wide, at ex0, ey0 in the frame) into the joined planes
+__global__ void k_write_core(const double *__restrict__ tux, const double *__restrict__ tuy,
+                             const double *__restrict__ ten, const double *__restrict__ tres, int ew, int ex0, int ey0,
+                             int cx0, int cy0, int cw, int w, double *__restrict__ jx, double *__restrict__ jy,
+                             double *__restrict__ je, double *__restrict__ jr) {
+    const int dx = blockIdx.x * blockDim.x + threadIdx.x, y = cy0 + blockIdx.y;
+    if (dx >= cw) return;
+    const int x = cx0 + dx;
+    const size_t l = (size_t)(y - ey0) * ew + (x - ex0), i = (size_t)y * w + x;
+    jx[i] = tux[l];
+    jy[i] = tuy[l];
+    je[i] = ten[l];
+    jr[i] = tres[l];
+}
+
+// tile.rs:80-95
+int tile_margin(const ofx_config &c) {
+    return std::max({64, (int)std::ceil(4.0 * c.sigma_w), (int)std::ceil(4.0 * c.sigma_s),
+                     (int)std::ceil(std::max(edge_guard(c), 0.0)), c.strain_win});
+}
+
+// The fine stage of a pair whose dense start is in ws.bux / ws.buy (w x h): the whole frame, or
+// tiled when the frame has a side over c.tile (track.rs:580-598). Leaves the field in ws.bux /
+// ws.buy, energy and residual in ws.benergy / ws.bresid, the confidence in ws.bconf / ws.bconf64.
+void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, const ofx_config &c, hipStream_t s) {
+    if (c.tile <= 0 || std::max(w, h) <= c.tile) {
+        refine_device(ws, i1, i2, w, h, w, h, 0, 0, c, s);
+        return;
+    }
+    const size_t n = (size_t)w * h;
+    double *fsx = B<double>(ws.bfsx, n), *fsy = B<double>(ws.bfsy, n);
+    HIPCHECK(hipMemcpyAsync(fsx, ws.bux.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipMemcpyAsync(fsy, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    double *jx = B<double>(ws.bjx, n), *jy = B<double>(ws.bjy, n), *je = B<double>(ws.bje, n),
+           *jr = B<double>(ws.bjr, n);
+    const int t = c.tile, m = tile_margin(c);
+    // tile.rs:38-76: row by row, then the largest extension first (stable)
+    std::vector<std::pair<TRect, TRect>> ts;  // (core, ext)
+    auto axis = [](int nn, int tt, int mm, int k, int &c0, int &c1, int &e0, int &e1) {
+        c0 = k * tt;
+        c1 = std::min(nn, c0 + tt);
+        e0 = std::max(0, c0 - mm);
+        e1 = std::min(nn, c1 + mm);
+    };
+    for (int ky = 0; ky < (h + t - 1) / t; ky++)
+        for (int kx = 0; kx < (w + t - 1) / t; kx++) {
+            TRect core, ext;
+            axis(w, t, m, kx, core.x0, core.x1, ext.x0, ext.x1);
+            axis(h, t, m, ky, core.y0, core.y1, ext.y0, ext.y1);
+            ts.push_back({core, ext});
+        }
+    std::vector<size_t> order(ts.size());
+    for (size_t k = 0; k < ts.size(); k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return ts[a].second.area() > ts[b].second.area(); });
+    const int nblk = 256;
+    double *part = B<double>(ws.bpart, 4 * nblk);
+    std::vector<double> hpart(4 * nblk);
+    for (size_t k : order) {
+        const TRect core = ts[k].first, ext = ts[k].second;
+        const int ew = ext.x1 - ext.x0, eh = ext.y1 - ext.y0;
+        const size_t ne = (size_t)ew * eh;
+        // tile.rs:112-146: the extension moved by the finite range of its (f32) start field
+        hipLaunchKernelGGL(k_rect_minmax, dim3(nblk), dim3(256), 0, s, (const double *)fsx, (const double *)fsy, w,
+                           ext.x0, ext.y0, ew, eh, part);
+        LAUNCHCHECK();
+        HIPCHECK(hipMemcpyAsync(hpart.data(), part, hpart.size() * sizeof(double), hipMemcpyDeviceToHost, s));
+        HIPCHECK(hipStreamSynchronize(s));
+        double lox = INFINITY, hix = -INFINITY, loy = INFINITY, hiy = -INFINITY;
+        for (int b = 0; b < nblk; b++) {
+            lox = std::min(lox, hpart[4 * b]);
+            hix = std::max(hix, hpart[4 * b + 1]);
+            loy = std::min(loy, hpart[4 * b + 2]);
+            hiy = std::max(hiy, hpart[4 * b + 3]);
+        }
+        if (lox > hix) lox = hix = 0.0;
+        if (loy > hiy) loy = hiy = 0.0;
+        auto span = [&](int first, int end, double lo, double hi, int nn, int &a, int &b) {
+            double fa = std::floor(first + lo) - m, fb = std::ceil((end - 1) + hi) + 1.0 + m;
+            fa = std::min(std::max(fa, 0.0), (double)(nn - 1));
+            a = (int)fa;
+            fb = std::min(std::max(fb, (double)(a + 1)), (double)nn);
+            b = (int)fb;
+        };
+        TRect win;
+        span(ext.x0, ext.x1, lox, hix, w, win.x0, win.x1);
+        span(ext.y0, ext.y1, loy, hiy, h, win.y0, win.y1);
+        const int ww = win.x1 - win.x0, wh = win.y1 - win.y0;
+        float *t1 = B<float>(ws.btile1, ne), *t2 = B<float>(ws.btile2, (size_t)ww * wh);
+        HIPCHECK(hipMemcpy2DAsync(t1, ew * sizeof(float), i1 + (size_t)ext.y0 * w + ext.x0, w * sizeof(float),
+                                  ew * sizeof(float), eh, hipMemcpyDeviceToDevice, s));
+        HIPCHECK(hipMemcpy2DAsync(t2, ww * sizeof(float), i2 + (size_t)win.y0 * w + win.x0, w * sizeof(float),
+                                  ww * sizeof(float), wh, hipMemcpyDeviceToDevice, s));
+        double *sx = B<double>(ws.bux, ne), *sy = B<double>(ws.buy, ne);
+        HIPCHECK(hipMemcpy2DAsync(sx, ew * sizeof(double), fsx + (size_t)ext.y0 * w + ext.x0, w * sizeof(double),
+                                  ew * sizeof(double), eh, hipMemcpyDeviceToDevice, s));
+        HIPCHECK(hipMemcpy2DAsync(sy, ew * sizeof(double), fsy + (size_t)ext.y0 * w + ext.x0, w * sizeof(double),
+                                  ew * sizeof(double), eh, hipMemcpyDeviceToDevice, s));
+        refine_device(ws, t1, t2, ew, eh, ww, wh, ext.x0 - win.x0, ext.y0 - win.y0, c, s);
+        const int cw = core.x1 - core.x0;
+        hipLaunchKernelGGL(k_write_core, dim3((cw + 255) / 256, core.y1 - core.y0), dim3(256), 0, s,
+                           (const double *)ws.bux.p, (const double *)ws.buy.p, (const double *)ws.benergy.p,
+                           (const double *)ws.bresid.p, ew, ext.x0, ext.y0, core.x0, core.y0, cw, w, jx, jy, je, jr);
+        LAUNCHCHECK();
+    }
+    double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
+    double *energy = B<double>(ws.benergy, n), *resid = B<double>(ws.bresid, n);
+    HIPCHECK(hipMemcpyAsync(ux, jx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipMemcpyAsync(uy, jy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipMemcpyAsync(energy, je, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    HIPCHECK(hipMemcpyAsync(resid, jr, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    double *med = B<double>(ws.bmed, 4);
+    ws.prof.mark(ST_CONFIDENCE, s);
+    median(energy, n, ws.bmedscr.get<char>(median_scratch_bytes()), med, s);
+    const double Bw = edge_guard(c) > 0 ? 4.0 * c.sigma_w : 0.0;
+    confidence(energy, resid, med, ux, uy, c.phase_tol, c.conf_min, Bw, w, h, nullptr, B<float>(ws.bconf, n),
+               B<double>(ws.bconf64, n), s);
+}
+
 // Validity, guard and strain (track.py:56-108) of the field (ux, uy): low = conf < conf_min, or the
 // given low mask (the hybrid sequence's, hybrid.rs:495-503).
 void finish_device(Workspace &ws, const double *ux, const double *uy, const float *conf, const uint8_t *low_in,
@@ -469,7 +638,7 @@ void pair_device(Workspace &ws, const float *i1, const float *i2, int w, int h, 
     double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
     coarse_dense(i1, i2, w, h, c, init_x, init_y, ux, uy, ws.coarse, s);
     HIPCHECK(hipEventRecord(ws.ev[1], s));
-    refine_device(ws, i1, i2, w, h, w, h, 0, 0, c, s);
+    fine_stage(ws, i1, i2, w, h, c, s);
     HIPCHECK(hipEventRecord(ws.ev[2], s));
     ux = (double *)ws.bux.p;
     uy = (double *)ws.buy.p;
@@ -617,7 +786,7 @@ struct Seq {
         double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
         coarse_dense(a, b, w, h, c, ix, iy, ux, uy, ws.coarse, s);
         HIPCHECK(hipEventRecord(ws.ev[1], s));
-        refine_device(ws, a, b, w, h, w, h, 0, 0, c, s);
+        fine_stage(ws, a, b, w, h, c, s);
         HIPCHECK(hipEventRecord(ws.ev[2], s));
         HIPCHECK(hipMemcpyAsync(ox, ws.bux.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         HIPCHECK(hipMemcpyAsync(oy, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
@@ -911,6 +1080,7 @@ void ofx_config_default(ofx_config *c) {
     c->shape_order = 1;
     c->sigma_s = 4.0;
     c->outer_iters = 8;
+    c->tile = 0;
     c->inner_iters = 2;
     c->conf_min = 0.2;
     c->edge_guard = -1.0;

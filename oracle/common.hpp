@@ -53,6 +53,14 @@ void coarse_dense(const double *I1, const double *I2, int H, int W, const orc_cf
                   const double *init_y, double *gx_out, double *gy_out, std::vector<double> *grid_ux,
                   std::vector<double> *grid_uy, std::vector<double> *grid_peak);
 
+// fine stage (flow.cpp): refine of an image-2 window (phaseflow.rs:1441-1456) and the confidence
+// of a whole field
+void refine_shifted(const double *i1, int h, int w, const double *i2, int h2, int w2, long shx, long shy,
+                    const double *ux0, const double *uy0, const orc_cfg &c, double *ux_out, double *uy_out,
+                    double *conf_out, double *resid_out, double *energy_out);
+void confidence_of(const double *energy, const double *resid, const double *ux, const double *uy, int h, int w,
+                   const orc_cfg &c, double *conf_out);
+
 struct Cfg {
     const orc_cfg &c;
     explicit Cfg(const orc_cfg &cc) : c(cc) {}

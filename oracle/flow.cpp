@@ -12,6 +12,9 @@
 
 namespace orc {
 
+void confidence_of(const double *energy, const double *resid, const double *ux, const double *uy, int h, int w,
+                   const orc_cfg &c, double *conf_out);
+
 static thread_local std::string g_err;
 void set_error(const std::string &msg) { g_err = msg; }
 
@@ -401,8 +404,24 @@ void refine_shifted(const double *i1, int h, int w, const double *i2, int h2, in
             uy.swap(nuy);
         }
     }
-    confidence_raw(energy.data(), resid.data(), n, c.phase_tol, conf.data());
-    const double B = C.edge_guard_px() > 0 ? 4.0 * c.sigma_w : 0.0;
+    confidence_of(energy.data(), resid.data(), ux.data(), uy.data(), h, w, c, conf_out);
+    for (size_t i = 0; i < n; i++) {
+        ux_out[i] = ux[i];
+        uy_out[i] = uy[i];
+        if (resid_out) resid_out[i] = resid[i];
+        if (energy_out) energy_out[i] = energy[i];
+    }
+}
+
+// phaseflow.py:320-323 and the border rule of phaseflow.rs:1617-1757: the clamped confidence of
+// a whole field (its texture scale the median energy of all of it), 0 where the pixel or its
+// match lies within B of the border
+void confidence_of(const double *energy, const double *resid, const double *ux, const double *uy, int h, int w,
+                   const orc_cfg &c, double *conf_out) {
+    const size_t n = (size_t)h * w;
+    std::vector<double> conf(n);
+    confidence_raw(energy, resid, n, c.phase_tol, conf.data());
+    const double B = Cfg(c).edge_guard_px() > 0 ? 4.0 * c.sigma_w : 0.0;
     for (int y = 0; y < h; y++)
         for (int x = 0; x < w; x++) {
             size_t i = (size_t)y * w + x;
@@ -410,10 +429,6 @@ void refine_shifted(const double *i1, int h, int w, const double *i2, int h2, in
             double cl = std::min(std::max(conf[i], 0.0), 1.0);
             if (std::isnan(conf[i])) cl = conf[i];
             conf_out[i] = rep ? cl : 0.0;
-            ux_out[i] = ux[i];
-            uy_out[i] = uy[i];
-            if (resid_out) resid_out[i] = resid[i];
-            if (energy_out) energy_out[i] = energy[i];
         }
 }
 

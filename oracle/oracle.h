@@ -50,6 +50,7 @@ typedef struct orc_cfg {
     int fb_min_support;
     double fb_tol;
     int prewarp_freeze_at;
+    int tile; /* 0 whole frame; N: tiles of N px for frames with a side over N (tile.rs) */
 } orc_cfg;
 
 /* ---- FFT (any length; mixed radix with direct DFT for prime factors) ---- */

@@ -78,6 +78,128 @@ void finish(const double *ux, const double *uy, const double *conf, int h, int w
                   out.exy.data(), out.gmax.data(), out.vol.data());
 }
 
+// ---------------------------------------------------------------- tiled fine stage
+// rust/crates/optflow-core/src/tile.rs:17-146 (geometry) and :225-500 (refine_tiled_from_on,
+// join_tiles): cores of `tile` px that partition the frame, each refined on its extension (the core
+// plus tile_margin, clipped) against the window of image 2 that the extension's start field
+// reaches; only the cores are kept, and the confidence is formed afterwards from the joined
+// energy (median over the whole frame) and the joined field (border rule).
+struct Rect {
+    int x0, y0, x1, y1;
+    long area() const { return (long)(x1 - x0) * (y1 - y0); }
+};
+struct Tile {
+    Rect core, ext;
+};
+
+// tile.rs:38-48
+void tile_axis(int n, int t, int m, int k, int &c0, int &c1, int &e0, int &e1) {
+    c0 = k * t;
+    c1 = std::min(n, c0 + t);
+    e0 = std::max(0, c0 - m);
+    e1 = std::min(n, c1 + m);
+}
+
+// tile.rs:53-76: row by row
+std::vector<Tile> tiles(int w, int h, int t, int m) {
+    std::vector<Tile> out;
+    const int nx = (w + t - 1) / t, ny = (h + t - 1) / t;
+    for (int ky = 0; ky < ny; ky++)
+        for (int kx = 0; kx < nx; kx++) {
+            Tile tl;
+            tile_axis(w, t, m, kx, tl.core.x0, tl.core.x1, tl.ext.x0, tl.ext.x1);
+            tile_axis(h, t, m, ky, tl.core.y0, tl.core.y1, tl.ext.y0, tl.ext.y1);
+            out.push_back(tl);
+        }
+    return out;
+}
+
+// tile.rs:80-95
+int tile_margin(const orc_cfg &c) {
+    const Cfg C(c);
+    return std::max({64, (int)std::ceil(4.0 * c.sigma_w), (int)std::ceil(4.0 * c.sigma_s),
+                     (int)std::ceil(std::max(C.edge_guard_px(), 0.0)), c.strain_win});
+}
+
+// tile.rs:112-146: the extension moved by the finite range of its start field, widened by the
+// margin, clipped (never empty)
+Rect displaced_window(const Rect &ext, const double *ux, const double *uy, int ew, int eh, int w, int h, int m) {
+    auto range = [&](const double *u, double &lo, double &hi) {
+        lo = INFINITY;
+        hi = -INFINITY;
+        for (long i = 0; i < (long)ew * eh; i++) {
+            // the start field is f32 in the reference
+            const double v = (double)(float)u[i];
+            if (std::isfinite(v)) {
+                lo = std::min(lo, v);
+                hi = std::max(hi, v);
+            }
+        }
+        if (lo > hi) lo = hi = 0.0;
+    };
+    auto axis = [&](int first, int end, double lo, double hi, int n, int &a, int &b) {
+        double fa = std::floor(first + lo) - m;
+        double fb = std::ceil((end - 1) + hi) + 1.0 + m;
+        fa = std::min(std::max(fa, 0.0), (double)(n - 1));
+        a = (int)fa;
+        fb = std::min(std::max(fb, (double)(a + 1)), (double)n);
+        b = (int)fb;
+    };
+    double lx, hx, ly, hy;
+    range(ux, lx, hx);
+    range(uy, ly, hy);
+    Rect r;
+    axis(ext.x0, ext.x1, lx, hx, w, r.x0, r.x1);
+    axis(ext.y0, ext.y1, ly, hy, h, r.y0, r.y1);
+    return r;
+}
+
+std::vector<double> crop(const double *a, int w, const Rect &r) {
+    const int rw = r.x1 - r.x0;
+    std::vector<double> out((size_t)rw * (r.y1 - r.y0));
+    for (int y = r.y0; y < r.y1; y++)
+        std::copy(a + (size_t)y * w + r.x0, a + (size_t)y * w + r.x1, out.begin() + (size_t)(y - r.y0) * rw);
+    return out;
+}
+
+// The fine stage of a pair from the dense start (gx, gy): whole frame, or tiled when the frame has
+// a side over c.tile (track.rs:580-598)
+void fine_stage(const double *i1, const double *i2, int h, int w, const double *gx, const double *gy,
+                const orc_cfg &c, double *ux, double *uy, double *conf) {
+    if (c.tile <= 0 || std::max(w, h) <= c.tile) {
+        refine(i1, i2, h, w, gx, gy, c, ux, uy, conf, nullptr, nullptr);
+        return;
+    }
+    const int m = tile_margin(c);
+    const std::vector<Tile> ts = tiles(w, h, c.tile, m);
+    std::vector<size_t> order(ts.size());
+    for (size_t k = 0; k < ts.size(); k++) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return ts[a].ext.area() > ts[b].ext.area(); });
+    const size_t n = (size_t)w * h;
+    std::vector<double> energy(n), resid(n);
+    for (size_t k : order) {
+        const Tile &t = ts[k];
+        const int ew = t.ext.x1 - t.ext.x0, eh = t.ext.y1 - t.ext.y0;
+        const std::vector<double> sx = crop(gx, w, t.ext), sy = crop(gy, w, t.ext);
+        const Rect win = displaced_window(t.ext, sx.data(), sy.data(), ew, eh, w, h, m);
+        const std::vector<double> a = crop(i1, w, t.ext), b = crop(i2, w, win);
+        const size_t ne = (size_t)ew * eh;
+        std::vector<double> tux(ne), tuy(ne), tconf(ne), tres(ne), ten(ne);
+        refine_shifted(a.data(), eh, ew, b.data(), win.y1 - win.y0, win.x1 - win.x0, t.ext.x0 - win.x0,
+                       t.ext.y0 - win.y0, sx.data(), sy.data(), c, tux.data(), tuy.data(), tconf.data(), tres.data(),
+                       ten.data());
+        for (int y = t.core.y0; y < t.core.y1; y++)
+            for (int x = t.core.x0; x < t.core.x1; x++) {
+                const size_t l = (size_t)(y - t.ext.y0) * ew + (x - t.ext.x0), i = (size_t)y * w + x;
+                ux[i] = tux[l];
+                uy[i] = tuy[l];
+                resid[i] = tres[l];
+                energy[i] = ten[l];
+            }
+    }
+    confidence_of(energy.data(), resid.data(), ux, uy, h, w, c, conf);
+}
+
 // track.py:115-146
 int pair(const double *i1, const double *i2, int h, int w, const uint8_t *mask, const double *ix,
          const double *iy, const orc_cfg &c, Pair &out) {
@@ -95,8 +217,7 @@ int pair(const double *i1, const double *i2, int h, int w, const uint8_t *mask, 
     out.raw_ux.resize(n);
     out.raw_uy.resize(n);
     out.conf.resize(n);
-    refine(i1, i2, h, w, gx.data(), gy.data(), c, out.raw_ux.data(), out.raw_uy.data(), out.conf.data(),
-           nullptr, nullptr);
+    fine_stage(i1, i2, h, w, gx.data(), gy.data(), c, out.raw_ux.data(), out.raw_uy.data(), out.conf.data());
     double t2 = omp_get_wtime();
     finish(out.raw_ux.data(), out.raw_uy.data(), out.conf.data(), h, w, c, mask, out);
     double t3 = omp_get_wtime();

@@ -277,3 +277,23 @@ def test_batched_pairs_match_single_pairs(eng):
             np.testing.assert_array_equal(getattr(one, name), getattr(r, name), err_msg=f"pair {k} {name}")
         for name in one.strain:
             np.testing.assert_array_equal(one.strain[name], r.strain[name], err_msg=f"pair {k} {name}")
+
+
+def test_tiled_pair(eng):
+    """Config.tile: the reference's tiled fine stage (tile.rs:225-500) on the GPU, 2 x 3 tiles of
+    128 px: against the oracle's tiled pair at the pair bar, and against the GPU's whole-frame pair
+    to the reference's tiling bar, 1e-3 px."""
+    from optflow_soil_amd import synth
+    h, w = 224, 288
+    pat = synth.make_pattern(h, w, seed=4)
+    f = synth.stretch_translate(0.01, (1.7, -0.9), (w / 2, h / 2))
+    a = synth.as_float(synth.render(pat))
+    b = synth.as_float(synth.render(pat, f, noise=0.005, seed=2))
+    print("tiled pair 224x288, tile 128", flush=True)
+    r = eng.track_pair(a, b, Config(tile=128))
+    o = O.track_pair(f64(a), f64(b), Config(tile=128))
+    compare_pair(r, o, "tiled pair")
+    whole = eng.track_pair(a, b, Config())
+    fin = np.isfinite(whole.ux) & np.isfinite(r.ux)
+    print(f"tiled vs whole frame: max ux {np.abs(whole.ux - r.ux)[fin].max():.2e}", flush=True)
+    assert np.abs(whole.ux - r.ux)[fin].max() < 1e-3 and np.abs(whole.uy - r.uy)[fin].max() < 1e-3

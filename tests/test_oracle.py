@@ -395,3 +395,28 @@ def test_sequence_hybrid_fixtures(name, fb_max):
         assert rms(r["ux"], ux_py[k]) < 1e-3
         if name == "A":
             assert rms(r["conf"], load("hybA_conf")[k]) < 1e-3
+
+
+def test_tiled_fine_stage_matches_the_whole_frame():
+    """The tiled fine stage (tile.rs:225-500; cores of `tile` px, each refined on its extension
+    against the window of image 2 its start reaches) agrees with the whole-frame refine to the
+    reference's own bar for tiling, 1e-3 px (coret/tile_equivalence.rs:118-136): 2 x 3 tiles of
+    128 px with the 64 px margin."""
+    from optflow_soil_amd import synth
+    h, w = 224, 288
+    pat = synth.make_pattern(h, w, seed=4)
+    f = synth.stretch_translate(0.01, (1.7, -0.9), (w / 2, h / 2))
+    a = synth.as_float(synth.render(pat)).astype(np.float64)
+    b = synth.as_float(synth.render(pat, f, noise=0.005, seed=2)).astype(np.float64)
+    whole = O.track_pair(a, b, Config())
+    tiled = O.track_pair(a, b, Config(tile=128))
+    assert (np.isfinite(whole["ux"]) == np.isfinite(tiled["ux"])).mean() >= 0.999
+    fin = np.isfinite(whole["ux"]) & np.isfinite(tiled["ux"])
+    assert np.abs(whole["ux"] - tiled["ux"])[fin].max() < 1e-3
+    assert np.abs(whole["uy"] - tiled["uy"])[fin].max() < 1e-3
+    # a frame no wider than the tile is refined whole: identical bits
+    small = O.track_pair(a[:128, :128], b[:128, :128], Config(tile=128))
+    ref = O.track_pair(a[:128, :128], b[:128, :128], Config())
+    assert np.array_equal(small["raw_ux"], ref["raw_ux"])
+    with pytest.raises(ValueError, match="tile must be 0"):
+        Config(tile=32).validate()
