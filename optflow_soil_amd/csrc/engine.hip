@@ -7,6 +7,8 @@
 // rust/crates/optflow-core/src/track.rs:1112-1225 (chained sequence),
 // rust/crates/optflow-gpu/src/engine.rs:633-1071 (lanes, check_window, errors).
 #include <atomic>
+#include <cstdio>
+#include <cstdlib>
 #include <thread>
 #include <chrono>
 #include <cmath>
@@ -21,6 +23,52 @@ namespace ofx {
 // Largest side of an image-2 window: a row of its spline prefilter is held in LDS (two padded
 // float lines within 64 KiB).
 constexpr int MAX_I2_SIDE = 7680;
+
+// ---------------------------------------------------------------- watchdog
+// One thread per process watches the calls in progress. The markers are plain atomics written by
+// the calling threads (no HIP call from the watchdog, so a thread blocked inside the runtime
+// cannot block it); with several lanes at work they name the most recent writer.
+namespace {
+std::atomic<const char *> g_step{"-"};
+std::atomic<const char *> g_launch_file{"-"};
+std::atomic<int> g_launch_line{0};
+std::atomic<long long> g_call_start_ms{0};  // 0: no call in progress
+std::atomic<long long> g_call_id{0};
+
+long long now_ms() {
+    return std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+void watchdog_start() {
+    static std::once_flag once;
+    std::call_once(once, [] {
+        const char *v = std::getenv("OFX_WATCHDOG_S");
+        const double limit_s = v ? std::atof(v) : 60.0;
+        if (limit_s <= 0) return;
+        std::thread([limit_s] {
+            long long reported = -1;
+            for (;;) {
+                std::this_thread::sleep_for(std::chrono::seconds(1));
+                const long long t0 = g_call_start_ms.load(), id = g_call_id.load();
+                if (t0 == 0 || id == reported) continue;
+                const double el = (now_ms() - t0) * 1e-3;
+                if (el < limit_s) continue;
+                std::fprintf(stderr, "ofx watchdog: a call has run %.0f s; stage %s; last kernel launch %s:%d\n", el,
+                             g_step.load(), g_launch_file.load(), g_launch_line.load());
+                std::fflush(stderr);
+                reported = id;
+            }
+        }).detach();
+    });
+}
+}  // namespace
+
+void step_mark(const char *what) { g_step.store(what, std::memory_order_relaxed); }
+void launch_mark(const char *file, int line) {
+    g_launch_file.store(file, std::memory_order_relaxed);
+    g_launch_line.store(line, std::memory_order_relaxed);
+}
 
 namespace {
 
@@ -292,6 +340,7 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
     float2 *spec = B<float2>(ws.bspec, np), *tmp = B<float2>(ws.btmp, np), *tmp2 = B<float2>(ws.btmp2, np);
     float2 *resp = B<float2>(ws.bresp, std::max<size_t>(F, 3) * np);
     float4 *refp = B<float4>(ws.brefp, F * n);
+    step_mark("fine: reference pack");
     // reference pack of image 1 (phaseflow.py:100-111): three inverses per filter
     ws.prof.mark(ST_REFERENCE, s);
     fft_forward_padded(i1, w, h, spec, tmp, tmp2, s);
@@ -325,6 +374,7 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
     const double R = auto_edge_guard(c);
     const int freeze = c.prewarp_freeze_at;
     bool have_kept = false;
+    step_mark("fine: outer iterations");
     for (int it = 0; it < c.outer_iters; it++) {
         if (it == 0) {
             ws.prof.mark(ST_SMOOTH, s);
@@ -475,6 +525,10 @@ void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, c
     HIPCHECK(hipMemcpyAsync(fsy, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     double *jx = B<double>(ws.bjx, n), *jy = B<double>(ws.bjy, n), *je = B<double>(ws.bje, n),
            *jr = B<double>(ws.bjr, n);
+    // every plane the tiles use and the joined result is copied back into holds the whole frame
+    // before the first tile: refine_device swaps bux / bnux, and growing a buffer later, while the
+    // last tile's kernels still read it on the stream, would free memory in use
+    for (DevBuf *b : {&ws.bux, &ws.buy, &ws.bnux, &ws.bnuy, &ws.benergy, &ws.bresid}) (void)B<double>(*b, n);
     const int t = c.tile, m = tile_margin(c);
     // tile.rs:38-76: row by row, then the largest extension first (stable)
     std::vector<std::pair<TRect, TRect>> ts;  // (core, ext)
@@ -499,6 +553,7 @@ void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, c
     double *part = B<double>(ws.bpart, 4 * nblk);
     std::vector<double> hpart(4 * nblk);
     for (size_t k : order) {
+        step_mark("fine: a tile");
         const TRect core = ts[k].first, ext = ts[k].second;
         const int ew = ext.x1 - ext.x0, eh = ext.y1 - ext.y0;
         const size_t ne = (size_t)ew * eh;
@@ -636,9 +691,12 @@ void pair_device(Workspace &ws, const float *i1, const float *i2, int w, int h, 
     HIPCHECK(hipEventRecord(ws.ev[0], s));
     ws.prof.mark(ST_COARSE, s);
     double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
+    step_mark("pair: coarse stage");
     coarse_dense(i1, i2, w, h, c, init_x, init_y, ux, uy, ws.coarse, s);
     HIPCHECK(hipEventRecord(ws.ev[1], s));
+    step_mark("pair: fine stage");
     fine_stage(ws, i1, i2, w, h, c, s);
+    step_mark("pair: guard and strain");
     HIPCHECK(hipEventRecord(ws.ev[2], s));
     ux = (double *)ws.bux.p;
     uy = (double *)ws.buy.p;
@@ -651,6 +709,7 @@ void pair_device(Workspace &ws, const float *i1, const float *i2, int w, int h, 
 }
 
 void record_times(Workspace &ws, ofx_result *o) {
+    step_mark("pair: waiting for the end event");
     HIPCHECK(hipEventSynchronize(ws.ev[3]));
     ws.prof.flush();
     o->t_coarse = elapsed(ws.ev[0], ws.ev[1]) * 1e-3;
@@ -683,6 +742,7 @@ void download(const Staged &st, ofx_result *host, size_t n, hipStream_t s) {
     for (int k = 0; k < 10; k++)
         if (hst[k]) HIPCHECK(hipMemcpyAsync(hst[k], d[k], n * sizeof(float), hipMemcpyDeviceToHost, s));
     if (host->valid) HIPCHECK(hipMemcpyAsync(host->valid, st.d.valid, n, hipMemcpyDeviceToHost, s));
+    step_mark("download: waiting for the result copies");
     HIPCHECK(hipStreamSynchronize(s));
 }
 
@@ -738,6 +798,21 @@ void track_pair_on_lane(Lane &lane, const float *i1, const float *i2, int w, int
 
 template <class F>
 int guarded(F &&f) {
+    // the outermost call on this thread times itself for the watchdog
+    static thread_local int depth = 0;
+    struct Timed {
+        bool outer;
+        Timed() : outer(depth++ == 0) {
+            if (outer) {
+                g_call_id.fetch_add(1);
+                g_call_start_ms.store(now_ms());
+            }
+        }
+        ~Timed() {
+            depth--;
+            if (outer) g_call_start_ms.store(0);
+        }
+    } timed;
     try {
         f();
         return OFX_OK;
@@ -784,13 +859,16 @@ struct Seq {
         HIPCHECK(hipEventRecord(ws.ev[0], s));
         ws.prof.mark(ST_COARSE, s);
         double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
+        step_mark("sequence: coarse stage of a leg");
         coarse_dense(a, b, w, h, c, ix, iy, ux, uy, ws.coarse, s);
+        step_mark("sequence: fine stage of a leg");
         HIPCHECK(hipEventRecord(ws.ev[1], s));
         fine_stage(ws, a, b, w, h, c, s);
         HIPCHECK(hipEventRecord(ws.ev[2], s));
         HIPCHECK(hipMemcpyAsync(ox, ws.bux.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         HIPCHECK(hipMemcpyAsync(oy, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         HIPCHECK(hipMemcpyAsync(oc, ws.bconf64.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        step_mark("sequence: waiting for a leg");
         HIPCHECK(hipEventSynchronize(ws.ev[2]));
         return std::make_pair(elapsed(ws.ev[0], ws.ev[1]) * 1e-3, elapsed(ws.ev[1], ws.ev[2]) * 1e-3);
     }
@@ -808,6 +886,7 @@ struct Seq {
                   (const double *)ws.bu0x.p, (const double *)ws.bu0y.p, dm, Bw, w, h, macc, s);
         double acc[2];
         HIPCHECK(hipMemcpyAsync(acc, macc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
+        step_mark("sequence: waiting for the mean confidence");
         HIPCHECK(hipStreamSynchronize(s));
         return acc[1] > 0 ? acc[0] / acc[1] : 0.0;
     }
@@ -899,6 +978,7 @@ struct Seq {
             tl = run_leg(false);
             cm = leg_mean();
         }
+        step_mark("chain: compose and finish");
         compose(u0x, u0y, lx, ly, w, h, cx, cy, s);
         conf_at(lc, u0x, u0y, w, h, ca, s);
         hipLaunchKernelGGL(k_min_pair, dim3((n + 255) / 256), dim3(256), 0, s, (const double *)c0r, (const double *)ca,
@@ -954,6 +1034,7 @@ struct Seq {
         auto counters = [&]() {
             std::vector<int> hc(HYB_NCOUNTERS);
             HIPCHECK(hipMemcpyAsync(hc.data(), cnt, HYB_NCOUNTERS * sizeof(int), hipMemcpyDeviceToHost, s));
+            step_mark("hybrid: waiting for the counters");
             HIPCHECK(hipStreamSynchronize(s));
             return hc;
         };
@@ -1106,6 +1187,7 @@ int ofx_create(int device, int lanes, ofx_engine **out) {
         if (hipGetDeviceCount(&count) != hipSuccess || count == 0) throw Error(OFX_ENODEVICE, "no HIP device found");
         if (device < 0 || device >= count) throw Error(OFX_ENODEVICE, "HIP device " + std::to_string(device) + " does not exist");
         if (lanes < 1 || lanes > 8) throw Error(OFX_EINVAL, "lanes must lie in [1, 8]");
+        watchdog_start();
         HIPCHECK(hipSetDevice(device));
         std::unique_ptr<ofx_engine> e(new ofx_engine);
         e->device = device;

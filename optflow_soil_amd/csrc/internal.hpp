@@ -29,7 +29,17 @@ struct Error : std::runtime_error {
             throw ::ofx::Error(OFX_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-#define LAUNCHCHECK() HIPCHECK(hipGetLastError())
+// Progress markers for the watchdog (engine.hip): the stage a call is in and the source line of
+// the last kernel launch. A call that runs past OFX_WATCHDOG_S seconds (default 60) gets one line
+// on stderr naming both, so a run that stops making progress says where.
+void step_mark(const char *what);
+void launch_mark(const char *file, int line);
+
+#define LAUNCHCHECK()                                   \
+    do {                                                \
+        ::ofx::launch_mark(__FILE__, __LINE__);         \
+        HIPCHECK(hipGetLastError());                    \
+    } while (0)
 
 // ---------------------------------------------------------------- FFT plans and passes
 struct FftPlan {
