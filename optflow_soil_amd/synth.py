@@ -100,8 +100,10 @@ def bowl_and_band(h, w, depth=6.0, radius=300.0, slip=3.0, band_width=40.0, angl
     return f
 
 
-def convex_polygon_mask(h, w, seed=0, n=8, coverage=0.6):
-    """A seeded random convex n-gon around the centre scaled to cover about `coverage` of the frame."""
+def convex_polygon_mask(h, w, seed=0, n=8, coverage=0.6, search_step=1):
+    """A seeded random convex n-gon around the centre scaled to cover about `coverage` of the frame.
+    search_step > 1 measures the coverage on every search_step-th pixel while searching the scale
+    (faster, the coverage then holds to about a percent)."""
     rng = np.random.default_rng(seed)
     ang = np.sort(rng.uniform(0, 2 * np.pi, n))
     rad = rng.uniform(0.8, 1.0, n)
@@ -125,16 +127,17 @@ def convex_polygon_mask(h, w, seed=0, n=8, coverage=0.6):
     u = (xx - (w - 1) / 2.0) / ((w - 1) / 2.0)
     v = (yy - (h - 1) / 2.0) / ((h - 1) / 2.0)
 
-    def inside(scale):
-        m = np.ones((h, w), bool)
+    def inside(scale, u=u, v=v):
+        m = np.ones(u.shape, bool)
         for i in range(len(hull)):
             a, b = hull[i] * scale, hull[(i + 1) % len(hull)] * scale
             m &= (b[0] - a[0]) * (v - a[1]) - (b[1] - a[1]) * (u - a[0]) >= 0
         return m
+    us, vs = u[::search_step, ::search_step], v[::search_step, ::search_step]
     lo, hi = 0.1, 2.0
     for _ in range(40):
         mid = 0.5 * (lo + hi)
-        if inside(mid).mean() < coverage:
+        if inside(mid, us, vs).mean() < coverage:
             lo = mid
         else:
             hi = mid
@@ -163,3 +166,41 @@ def workload_c2(size=2048, seed=0):
     roi = convex_polygon_mask(h, w, seed=seed)
     yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
     return a, b, roi, f(xx, yy)
+
+
+C4_SIZES = ((512, 512), (768, 1024), (1024, 1024), (1536, 1536), (1024, 2048))  # (h, w)
+
+
+def affine_field(exx, eyy, exy, rot_deg, t, center):
+    """u(X) = (E + R - I)(X - c) + t: a small strain (exx, eyy, exy), a rotation and a translation."""
+    r = np.deg2rad(rot_deg)
+    a11, a12 = exx + (np.cos(r) - 1.0), exy - np.sin(r)
+    a21, a22 = exy + np.sin(r), eyy + (np.cos(r) - 1.0)
+
+    def f(X, Y):
+        dx, dy = X - center[0], Y - center[1]
+        return a11 * dx + a12 * dy + t[0], a21 * dx + a22 * dy + t[1]
+    return f
+
+
+def workload_c4(n_pairs=64, distinct_per_size=2, seed=0):
+    """The batched uneven workload: n_pairs pair indices with sizes drawn uniformly from C4_SIZES;
+    each drawn pair is one of `distinct_per_size` rendered pairs of its size (a random affine field:
+    strains within +-2 %, rotation +-1 deg, translation +-8 px; noise sigma 0.005) with a convex
+    8-gon ROI of its own covering 40-90 %. Returns the list of (I1 u8, I2 u8, roi)."""
+    rng = np.random.default_rng(seed)
+    rendered = {}
+    for si, (h, w) in enumerate(C4_SIZES):
+        for d in range(distinct_per_size):
+            s = 1000 * si + d
+            pat = make_pattern(h, w, seed=s)
+            f = affine_field(*rng.uniform(-0.02, 0.02, 3), rng.uniform(-1.0, 1.0), rng.uniform(-8.0, 8.0, 2),
+                             ((w - 1) / 2.0, (h - 1) / 2.0))
+            rendered[(si, d)] = (render(pat, None, noise=0.005, seed=s + 1), render(pat, f, noise=0.005, seed=s + 2))
+    out = []
+    for k in range(n_pairs):
+        si = int(rng.integers(len(C4_SIZES)))
+        a, b = rendered[(si, k % distinct_per_size)]
+        h, w = C4_SIZES[si]
+        out.append((a, b, convex_polygon_mask(h, w, seed=k, coverage=float(rng.uniform(0.4, 0.9)), search_step=8)))
+    return out
