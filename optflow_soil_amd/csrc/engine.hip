@@ -25,20 +25,32 @@ namespace ofx {
 constexpr int MAX_I2_SIDE = 7680;
 
 // ---------------------------------------------------------------- watchdog
-// One thread per process watches the calls in progress. The markers are plain atomics written by
-// the calling threads (no HIP call from the watchdog, so a thread blocked inside the runtime
-// cannot block it); with several lanes at work they name the most recent writer.
+// A fixed pool of per-thread progress records; a thread takes a free one on its first call and
+// returns it when it exits. One detached thread reports on stderr, once per call and thread, every
+// thread whose call has run past the limit. It only reads the atomics, never calls HIP, so a
+// thread blocked inside the runtime cannot block it.
 namespace {
-std::atomic<const char *> g_step{"-"};
-std::atomic<const char *> g_launch_file{"-"};
-std::atomic<int> g_launch_line{0};
-std::atomic<long long> g_call_start_ms{0};  // 0: no call in progress
-std::atomic<long long> g_call_id{0};
+constexpr int NPROG = 256;
+Progress g_prog[NPROG];
+Progress g_overflow;  // shared by threads beyond the pool (reports then name the last writer)
 
-long long now_ms() {
-    return std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now().time_since_epoch())
-        .count();
-}
+struct ProgressHolder {
+    Progress *p = nullptr;
+    ProgressHolder() {
+        for (int i = 0; i < NPROG && !p; i++) {
+            bool expected = false;
+            if (g_prog[i].used.compare_exchange_strong(expected, true)) p = &g_prog[i];
+        }
+        if (!p) p = &g_overflow;
+        p->step = "-";
+        p->hip_call = nullptr;
+        p->call_since_ms = 0;
+        p->pending = -1;
+    }
+    ~ProgressHolder() {
+        if (p != &g_overflow) p->used.store(false);
+    }
+};
 
 void watchdog_start() {
     static std::once_flag once;
@@ -47,28 +59,76 @@ void watchdog_start() {
         const double limit_s = v ? std::atof(v) : 60.0;
         if (limit_s <= 0) return;
         std::thread([limit_s] {
-            long long reported = -1;
+            static long long reported[NPROG + 1] = {0};
             for (;;) {
                 std::this_thread::sleep_for(std::chrono::seconds(1));
-                const long long t0 = g_call_start_ms.load(), id = g_call_id.load();
-                if (t0 == 0 || id == reported) continue;
-                const double el = (now_ms() - t0) * 1e-3;
-                if (el < limit_s) continue;
-                std::fprintf(stderr, "ofx watchdog: a call has run %.0f s; stage %s; last kernel launch %s:%d\n", el,
-                             g_step.load(), g_launch_file.load(), g_launch_line.load());
-                std::fflush(stderr);
-                reported = id;
+                const long long now = now_ms();
+                for (int i = 0; i <= NPROG; i++) {
+                    Progress &p = i < NPROG ? g_prog[i] : g_overflow;
+                    const long long t0 = p.call_since_ms.load(), id = p.call_id.load();
+                    if (t0 == 0 || reported[i] == id || (now - t0) * 1e-3 < limit_s) continue;
+                    const char *call = p.hip_call.load();
+                    const long long hs = p.hip_since_ms.load();
+                    const int pend = p.pending.load();
+                    std::fprintf(stderr,
+                                 "ofx watchdog: thread slot %d: call running %.0f s; stage %s; %s%s%s%.0f s; last "
+                                 "kernel launch %s:%d; last wait found device work %s\n",
+                                 i, (now - t0) * 1e-3, p.step.load(), call ? "inside " : "in no HIP call (", call ? call : "",
+                                 call ? " for " : "host code for ", call ? (now - hs) * 1e-3 : 0.0, p.launch_file.load(),
+                                 p.launch_line.load(), pend < 0 ? "(no wait yet)" : pend ? "pending" : "done");
+                    std::fflush(stderr);
+                    reported[i] = id;
+                }
             }
         }).detach();
     });
 }
 }  // namespace
 
-void step_mark(const char *what) { g_step.store(what, std::memory_order_relaxed); }
-void launch_mark(const char *file, int line) {
-    g_launch_file.store(file, std::memory_order_relaxed);
-    g_launch_line.store(line, std::memory_order_relaxed);
+Progress &progress() {
+    static thread_local ProgressHolder h;
+    return *h.p;
 }
+long long now_ms() {
+    return std::chrono::duration_cast<std::chrono::milliseconds>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+void step_mark(const char *what) { progress().step.store(what, std::memory_order_relaxed); }
+void launch_mark(const char *file, int line) {
+    Progress &p = progress();
+    p.launch_file.store(file, std::memory_order_relaxed);
+    p.launch_line.store(line, std::memory_order_relaxed);
+}
+HipCallMark::HipCallMark(const char *call) {
+    Progress &p = progress();
+    p.hip_since_ms.store(now_ms(), std::memory_order_relaxed);
+    p.hip_call.store(call, std::memory_order_relaxed);
+}
+HipCallMark::~HipCallMark() { progress().hip_call.store(nullptr, std::memory_order_relaxed); }
+
+namespace {
+template <class Q>
+void poll(Q query, const char *what) {
+    Progress &p = progress();
+    HipCallMark mark(what);
+    for (long k = 0;; k++) {
+        const hipError_t e = query();
+        if (e == hipSuccess) {
+            p.pending.store(0, std::memory_order_relaxed);
+            return;
+        }
+        if (e != hipErrorNotReady) throw Error(OFX_EDEVICE, std::string(what) + ": " + hipGetErrorString(e));
+        p.pending.store(1, std::memory_order_relaxed);
+        if (k < 4000)
+            std::this_thread::yield();
+        else
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+    }
+}
+}  // namespace
+
+void wait_stream(hipStream_t s) { poll([&] { return hipStreamQuery(s); }, "hipStreamQuery (waiting for a stream)"); }
+void wait_event(hipEvent_t e) { poll([&] { return hipEventQuery(e); }, "hipEventQuery (waiting for an event)"); }
 
 namespace {
 
@@ -139,7 +199,7 @@ struct Profiler {
     }
     void flush() {
         if (!on || used == 0) return;
-        HIPCHECK(hipEventSynchronize(pool[used - 1]));
+        wait_event(pool[used - 1]);
         for (size_t i = 0; i + 1 < used; i++) {
             float t = 0;
             HIPCHECK(hipEventElapsedTime(&t, pool[i], pool[i + 1]));
@@ -562,7 +622,7 @@ void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, c
                            ext.x0, ext.y0, ew, eh, part);
         LAUNCHCHECK();
         HIPCHECK(hipMemcpyAsync(hpart.data(), part, hpart.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
+        wait_stream(s);
         double lox = INFINITY, hix = -INFINITY, loy = INFINITY, hiy = -INFINITY;
         for (int b = 0; b < nblk; b++) {
             lox = std::min(lox, hpart[4 * b]);
@@ -710,7 +770,7 @@ void pair_device(Workspace &ws, const float *i1, const float *i2, int w, int h, 
 
 void record_times(Workspace &ws, ofx_result *o) {
     step_mark("pair: waiting for the end event");
-    HIPCHECK(hipEventSynchronize(ws.ev[3]));
+    wait_event(ws.ev[3]);
     ws.prof.flush();
     o->t_coarse = elapsed(ws.ev[0], ws.ev[1]) * 1e-3;
     o->t_fine = elapsed(ws.ev[1], ws.ev[2]) * 1e-3;
@@ -743,7 +803,7 @@ void download(const Staged &st, ofx_result *host, size_t n, hipStream_t s) {
         if (hst[k]) HIPCHECK(hipMemcpyAsync(hst[k], d[k], n * sizeof(float), hipMemcpyDeviceToHost, s));
     if (host->valid) HIPCHECK(hipMemcpyAsync(host->valid, st.d.valid, n, hipMemcpyDeviceToHost, s));
     step_mark("download: waiting for the result copies");
-    HIPCHECK(hipStreamSynchronize(s));
+    wait_stream(s);
 }
 
 // the argument checks of track_pair (track.rs:655-663, track.py:115-125)
@@ -804,13 +864,15 @@ int guarded(F &&f) {
         bool outer;
         Timed() : outer(depth++ == 0) {
             if (outer) {
-                g_call_id.fetch_add(1);
-                g_call_start_ms.store(now_ms());
+                Progress &p = progress();
+                p.call_id.fetch_add(1);
+                p.pending.store(-1);
+                p.call_since_ms.store(now_ms());
             }
         }
         ~Timed() {
             depth--;
-            if (outer) g_call_start_ms.store(0);
+            if (outer) progress().call_since_ms.store(0);
         }
     } timed;
     try {
@@ -869,7 +931,7 @@ struct Seq {
         HIPCHECK(hipMemcpyAsync(oy, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         HIPCHECK(hipMemcpyAsync(oc, ws.bconf64.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         step_mark("sequence: waiting for a leg");
-        HIPCHECK(hipEventSynchronize(ws.ev[2]));
+        wait_event(ws.ev[2]);
         return std::make_pair(elapsed(ws.ev[0], ws.ev[1]) * 1e-3, elapsed(ws.ev[1], ws.ev[2]) * 1e-3);
     }
     // the direct leg reference -> current into (lx, ly, lc), from the previous leg's raw field
@@ -887,7 +949,7 @@ struct Seq {
         double acc[2];
         HIPCHECK(hipMemcpyAsync(acc, macc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
         step_mark("sequence: waiting for the mean confidence");
-        HIPCHECK(hipStreamSynchronize(s));
+        wait_stream(s);
         return acc[1] > 0 ? acc[0] / acc[1] : 0.0;
     }
 
@@ -922,9 +984,9 @@ struct Seq {
             hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, upy, 0.0, n);
             hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, caa, 0.0, n);
             LAUNCHCHECK();
-            HIPCHECK(hipStreamSynchronize(s));  // the host vectors above go out of scope
+            wait_stream(s);  // the host vectors above go out of scope
         }
-        HIPCHECK(hipStreamSynchronize(s));
+        wait_stream(s);
         k = 0;
         ref_idx = 0;
         have_prev = have_prev_adj = false;
@@ -952,7 +1014,7 @@ struct Seq {
     }
     void deliver(const Staged &st, ofx_result *out, bool device_out) {
         if (device_out)
-            HIPCHECK(hipStreamSynchronize(s));
+            wait_stream(s);
         else
             download(st, out, n, s);
     }
@@ -993,7 +1055,7 @@ struct Seq {
         HIPCHECK(hipEventRecord(ws.ev[3], s));
         deliver(st, out, device_out);
         ws.prof.flush();
-        HIPCHECK(hipEventSynchronize(ws.ev[3]));
+        wait_event(ws.ev[3]);
         out->t_coarse = tl.first;
         out->t_fine = tl.second;
         out->t_strain = elapsed(ws.ev[2], ws.ev[3]) * 1e-3;
@@ -1005,7 +1067,7 @@ struct Seq {
         HIPCHECK(hipMemcpyAsync(pc, lc, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         HIPCHECK(hipMemcpyAsync(fprev, fcur, n * sizeof(float), hipMemcpyDeviceToDevice, s));
         have_prev = true;
-        HIPCHECK(hipStreamSynchronize(s));
+        wait_stream(s);
     }
 
     // seq_mode "hybrid": the direct leg of the chain plus, per pixel, the gated adjacent-frame
@@ -1035,7 +1097,7 @@ struct Seq {
             std::vector<int> hc(HYB_NCOUNTERS);
             HIPCHECK(hipMemcpyAsync(hc.data(), cnt, HYB_NCOUNTERS * sizeof(int), hipMemcpyDeviceToHost, s));
             step_mark("hybrid: waiting for the counters");
-            HIPCHECK(hipStreamSynchronize(s));
+            wait_stream(s);
             return hc;
         };
         const bool gate_on = c.fb_min_support > 0;
@@ -1131,7 +1193,7 @@ struct Seq {
         HIPCHECK(hipMemcpyAsync(pc, lc, n * sizeof(double), hipMemcpyDeviceToDevice, s));
         HIPCHECK(hipMemcpyAsync(fprev, fcur, n * sizeof(float), hipMemcpyDeviceToDevice, s));
         have_prev = true;
-        HIPCHECK(hipStreamSynchronize(s));
+        wait_stream(s);
     }
 };
 }  // namespace
@@ -1271,7 +1333,7 @@ int ofx_release_caches(ofx_engine *e) {
         for (auto &l : e->lanes) {
             bool expected = false;
             if (!l->busy.compare_exchange_strong(expected, true)) continue;  // in use: keep
-            HIPCHECK(hipStreamSynchronize(l->stream));
+            wait_stream(l->stream);
             l->ws.reset();
             l->busy.store(false);
         }
@@ -1306,7 +1368,7 @@ int ofx_refine_raw(ofx_engine *e, const float *i1, int w, int h, const float *i2
         if (out_uy) HIPCHECK(hipMemcpyAsync(out_uy, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
         if (out_energy) HIPCHECK(hipMemcpyAsync(out_energy, ws.benergy.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
         if (out_resid) HIPCHECK(hipMemcpyAsync(out_resid, ws.bresid.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
+        wait_stream(s);
     });
 }
 
@@ -1340,7 +1402,7 @@ int ofx_coarse_flow(ofx_engine *e, const float *i1, const float *i2, int w, int 
         coarse_dense(d1, d2, w, h, *cfg, ix, iy, ux, uy, ws.coarse, s);
         HIPCHECK(hipMemcpyAsync(out_ux, ux, n * sizeof(double), hipMemcpyDeviceToHost, s));
         HIPCHECK(hipMemcpyAsync(out_uy, uy, n * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
+        wait_stream(s);
     });
 }
 
@@ -1425,7 +1487,7 @@ int ofx_tavg_window(ofx_engine *e, const float *frames, int nf, int centre, int 
             mean_finish(acc, (double)used, dout, n, s);
             HIPCHECK(hipMemcpyAsync(out, dout, n * sizeof(float), hipMemcpyDeviceToHost, s));
         }
-        HIPCHECK(hipStreamSynchronize(s));
+        wait_stream(s);
         if (n_used) *n_used = used;
     });
 }
@@ -1449,7 +1511,7 @@ int ofx_frame_to_gray(ofx_engine *e, const void *pixels, int bits, int channels,
         HIPCHECK(hipMemcpyAsync(dp, pixels, bytes, hipMemcpyHostToDevice, s));
         frame_to_gray(dp, bits, channels, w, h, factor, dout, s);
         HIPCHECK(hipMemcpyAsync(out, dout, (size_t)ow * oh * sizeof(float), hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipStreamSynchronize(s));
+        wait_stream(s);
     });
 }
 
@@ -1526,6 +1588,13 @@ int ofx_track_pairs(ofx_engine *e, int n, const ofx_pair *pairs, const ofx_confi
         std::vector<int> codes(n, OFX_OK);
         std::vector<std::string> msgs(n);
         auto work = [&](Lane *lane) {
+            Progress &pg = progress();
+            pg.call_id.fetch_add(1);
+            pg.call_since_ms.store(now_ms());
+            struct Clear {
+                Progress &p;
+                ~Clear() { p.call_since_ms.store(0); }
+            } clear{pg};
             if (hipSetDevice(e->device) != hipSuccess) {
                 for (int k; (k = next.fetch_add(1)) < n;) {
                     codes[order[k]] = OFX_EDEVICE;

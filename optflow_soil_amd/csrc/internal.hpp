@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -22,18 +23,42 @@ struct Error : std::runtime_error {
     Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
 };
 
+// Progress of each host thread in the library, for the watchdog (engine.hip): the stage its call
+// is in, the HIP runtime call it is inside (and since when), the source line of its last kernel
+// launch, and whether its last wait found work still pending on the device. A call that runs past
+// OFX_WATCHDOG_S seconds (default 60) gets one report on stderr per thread, so a run that stops
+// making progress says where every thread stands.
+struct Progress {
+    std::atomic<bool> used{false};
+    std::atomic<const char *> step{"-"};
+    std::atomic<const char *> hip_call{nullptr};
+    std::atomic<long long> hip_since_ms{0};
+    std::atomic<const char *> launch_file{"-"};
+    std::atomic<int> launch_line{0};
+    std::atomic<long long> call_since_ms{0};
+    std::atomic<long long> call_id{0};
+    std::atomic<int> pending{-1};  // last wait: 1 device work still pending, 0 done, -1 none yet
+};
+Progress &progress();
+long long now_ms();
+void step_mark(const char *what);
+void launch_mark(const char *file, int line);
+struct HipCallMark {
+    explicit HipCallMark(const char *call);
+    ~HipCallMark();
+};
+// hipStreamSynchronize / hipEventSynchronize by polling the stream or event, which records whether
+// device work is still pending while the thread waits
+void wait_stream(hipStream_t s);
+void wait_event(hipEvent_t e);
+
 #define HIPCHECK(x)                                                                          \
     do {                                                                                     \
+        ::ofx::HipCallMark mark_(#x);                                                        \
         hipError_t e_ = (x);                                                                 \
         if (e_ != hipSuccess)                                                                \
             throw ::ofx::Error(OFX_EDEVICE, std::string(#x) + ": " + hipGetErrorString(e_)); \
     } while (0)
-
-// Progress markers for the watchdog (engine.hip): the stage a call is in and the source line of
-// the last kernel launch. A call that runs past OFX_WATCHDOG_S seconds (default 60) gets one line
-// on stderr naming both, so a run that stops making progress says where.
-void step_mark(const char *what);
-void launch_mark(const char *file, int line);
 
 #define LAUNCHCHECK()                                   \
     do {                                                \

@@ -686,7 +686,7 @@ void coarse_grid(const CoarseWs *ws, int w, int h, const ofx_config &c, std::vec
     HIPCHECK(hipMemcpyAsync(ux.data(), ws->gux, np * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipMemcpyAsync(uy.data(), ws->guy, np * sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHECK(hipMemcpyAsync(peak.data(), ws->gpk, np * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipStreamSynchronize(s));
+    wait_stream(s);
 }
 
 void coarse_release(CoarseWs *&ws) {
