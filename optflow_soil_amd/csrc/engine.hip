@@ -131,6 +131,51 @@ void wait_stream(hipStream_t s) { poll([&] { return hipStreamQuery(s); }, "hipSt
 void wait_event(hipEvent_t e) { poll([&] { return hipEventQuery(e); }, "hipEventQuery (waiting for an event)"); }
 
 namespace {
+thread_local DeferFrees *g_defer = nullptr;
+}
+DeferFrees::DeferFrees() : prev(g_defer) { g_defer = this; }
+DeferFrees::~DeferFrees() {
+    g_defer = prev;
+    for (void *p : held) (void)hipFree(p);
+}
+void release_device(void *p) {
+    if (!p) return;
+    if (g_defer)
+        g_defer->held.push_back(p);
+    else
+        (void)hipFree(p);
+}
+
+PinnedBuf::~PinnedBuf() {
+    if (p) (void)hipHostFree(p);
+}
+char *PinnedBuf::get() {
+    if (!p) HIPCHECK(hipHostMalloc((void **)&p, PIN_CHUNK, hipHostMallocDefault));
+    return p;
+}
+void copy_h2d(PinnedBuf &pin, void *dev, const void *host, size_t bytes, hipStream_t s) {
+    if (!bytes) return;
+    char *b = pin.get();
+    for (size_t off = 0; off < bytes; off += PIN_CHUNK) {
+        const size_t len = std::min(PIN_CHUNK, bytes - off);
+        wait_stream(s);  // the staging buffer is free again
+        std::memcpy(b, (const char *)host + off, len);
+        HIPCHECK(hipMemcpyAsync((char *)dev + off, b, len, hipMemcpyHostToDevice, s));
+    }
+    wait_stream(s);
+}
+void copy_d2h(PinnedBuf &pin, void *host, const void *dev, size_t bytes, hipStream_t s) {
+    if (!bytes) return;
+    char *b = pin.get();
+    for (size_t off = 0; off < bytes; off += PIN_CHUNK) {
+        const size_t len = std::min(PIN_CHUNK, bytes - off);
+        HIPCHECK(hipMemcpyAsync(b, (const char *)dev + off, len, hipMemcpyDeviceToHost, s));
+        wait_stream(s);
+        std::memcpy((char *)host + off, b, len);
+    }
+}
+
+namespace {
 
 thread_local std::string g_last_error;
 
@@ -157,7 +202,7 @@ struct DevBuf {
     T *get(size_t n) {
         const size_t b = n * sizeof(T);
         if (bytes < b) {
-            if (p) (void)hipFree(p);
+            if (p) release_device(p);
             p = nullptr;
             bytes = 0;
             HIPCHECK(hipMalloc(&p, b));
@@ -233,6 +278,7 @@ struct Workspace {
     DevBuf bstart_x, bstart_y;
     DevBuf btav_acc, btav_a, btav_fx, btav_fy, btav_c, btav_tmp, bpix;  // temporal averaging, frame input
     DevBuf bfsx, bfsy, bjx, bjy, bje, bjr, btile1, btile2, bpart;         // tiled fine stage
+    PinnedBuf pin;                                                       // staging of host copies
     DevBuf bout;                                       // staging of host outputs (f32 x 10 + u8)
     DevBuf btaps_s, btaps_w;
     double taps_s_sigma = -1, taps_w_sigma = -1;
@@ -364,7 +410,7 @@ void set_taps(Workspace &ws, const ofx_config &c, hipStream_t s) {
     if (ws.taps_s_sigma != c.sigma_s) {
         std::vector<double> k = gauss_taps(c.sigma_s);
         double *d = ws.btaps_s.get<double>(k.size());
-        HIPCHECK(hipMemcpy(d, k.data(), k.size() * sizeof(double), hipMemcpyHostToDevice));
+        copy_h2d(ws.pin, d, k.data(), k.size() * sizeof(double), s);
         ws.taps_s_host = k;
         ws.taps_s_sigma = c.sigma_s;
         ws.taps_s_r = (int)k.size() / 2;
@@ -377,12 +423,11 @@ void set_taps(Workspace &ws, const ofx_config &c, hipStream_t s) {
         for (int p = 0; p < 5; p++)
             for (int t = -r; t <= r; t++) kt[p * klen + t + r] = k[t + r] * std::pow((double)t, p);
         double *d = ws.btaps_w.get<double>(kt.size());
-        HIPCHECK(hipMemcpy(d, kt.data(), kt.size() * sizeof(double), hipMemcpyHostToDevice));
+        copy_h2d(ws.pin, d, kt.data(), kt.size() * sizeof(double), s);
         ws.taps_w_host = kt;
         ws.taps_w_sigma = c.sigma_w;
         ws.taps_w_r = r;
     }
-    (void)s;
 }
 
 // The fine stage (phaseflow.py:427-511, phaseflow.rs:1441-1613) on image 1 (w x h) against image 2
@@ -621,8 +666,7 @@ void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, c
         hipLaunchKernelGGL(k_rect_minmax, dim3(nblk), dim3(256), 0, s, (const double *)fsx, (const double *)fsy, w,
                            ext.x0, ext.y0, ew, eh, part);
         LAUNCHCHECK();
-        HIPCHECK(hipMemcpyAsync(hpart.data(), part, hpart.size() * sizeof(double), hipMemcpyDeviceToHost, s));
-        wait_stream(s);
+        copy_d2h(ws.pin, hpart.data(), part, hpart.size() * sizeof(double), s);
         double lox = INFINITY, hix = -INFINITY, loy = INFINITY, hiy = -INFINITY;
         for (int b = 0; b < nblk; b++) {
             lox = std::min(lox, hpart[4 * b]);
@@ -702,6 +746,14 @@ void finish_device(Workspace &ws, const double *ux, const double *uy, const floa
 }
 
 __global__ void k_fill(double *a, double v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ void k_fill_u8(uint8_t *a, uint8_t v, size_t n) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] = v;
+}
+__global__ void k_fill_i32(int *a, int v, size_t n) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) a[i] = v;
 }
@@ -794,16 +846,15 @@ Staged stage_outputs(Workspace &ws, size_t n, const ofx_result *host) {
     return st;
 }
 
-void download(const Staged &st, ofx_result *host, size_t n, hipStream_t s) {
+void download(Workspace &ws, const Staged &st, ofx_result *host, size_t n, hipStream_t s) {
     float *const d[10] = {st.d.ux, st.d.uy, st.d.conf, st.d.raw_ux, st.d.raw_uy,
                           st.d.exx, st.d.eyy, st.d.exy, st.d.gmax, st.d.vol};
     float *const hst[10] = {host->ux, host->uy, host->conf, host->raw_ux, host->raw_uy,
                             host->exx, host->eyy, host->exy, host->gmax, host->vol};
     for (int k = 0; k < 10; k++)
-        if (hst[k]) HIPCHECK(hipMemcpyAsync(hst[k], d[k], n * sizeof(float), hipMemcpyDeviceToHost, s));
-    if (host->valid) HIPCHECK(hipMemcpyAsync(host->valid, st.d.valid, n, hipMemcpyDeviceToHost, s));
-    step_mark("download: waiting for the result copies");
-    wait_stream(s);
+        if (hst[k]) copy_d2h(ws.pin, hst[k], d[k], n * sizeof(float), s);
+    step_mark("download: the result copies");
+    if (host->valid) copy_d2h(ws.pin, host->valid, st.d.valid, n, s);
 }
 
 // the argument checks of track_pair (track.rs:655-663, track.py:115-125)
@@ -828,18 +879,18 @@ void track_pair_on_lane(Lane &lane, const float *i1, const float *i2, int w, int
     hipStream_t s = lane.stream;
     const size_t n = (size_t)w * h;
     float *d1 = B<float>(ws.bi1, n), *d2 = B<float>(ws.bi2, n);
-    HIPCHECK(hipMemcpyAsync(d1, i1, n * sizeof(float), hipMemcpyHostToDevice, s));
-    HIPCHECK(hipMemcpyAsync(d2, i2, n * sizeof(float), hipMemcpyHostToDevice, s));
+    copy_h2d(ws.pin, d1, i1, n * sizeof(float), s);
+    copy_h2d(ws.pin, d2, i2, n * sizeof(float), s);
     uint8_t *dm = nullptr;
     if (mask) {
         dm = B<uint8_t>(ws.bmask, n);
-        HIPCHECK(hipMemcpyAsync(dm, mask, n, hipMemcpyHostToDevice, s));
+        copy_h2d(ws.pin, dm, mask, n, s);
     }
     double *ix = nullptr, *iy = nullptr;
     if (init_ux) {
         float *fx = B<float>(ws.binx, n), *fy = B<float>(ws.biny, n);
-        HIPCHECK(hipMemcpyAsync(fx, init_ux, n * sizeof(float), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(fy, init_uy, n * sizeof(float), hipMemcpyHostToDevice, s));
+        copy_h2d(ws.pin, fx, init_ux, n * sizeof(float), s);
+        copy_h2d(ws.pin, fy, init_uy, n * sizeof(float), s);
         ix = B<double>(ws.bstart_x, n);
         iy = B<double>(ws.bstart_y, n);
         to_f64(fx, ix, n, s);
@@ -848,7 +899,7 @@ void track_pair_on_lane(Lane &lane, const float *i1, const float *i2, int w, int
     Staged st = stage_outputs(ws, n, out);
     pair_device(ws, d1, d2, w, h, dm, ix, iy, c, &st.d, s);
     record_times(ws, &st.d);
-    download(st, out, n, s);
+    download(ws, st, out, n, s);
     out->t_coarse = st.d.t_coarse;
     out->t_fine = st.d.t_fine;
     out->t_strain = st.d.t_strain;
@@ -947,22 +998,23 @@ struct Seq {
         mean_conf((const double *)ws.bleg_c.p, (const double *)ws.bstart_x.p, (const double *)ws.bstart_y.p,
                   (const double *)ws.bu0x.p, (const double *)ws.bu0y.p, dm, Bw, w, h, macc, s);
         double acc[2];
-        HIPCHECK(hipMemcpyAsync(acc, macc, 2 * sizeof(double), hipMemcpyDeviceToHost, s));
-        step_mark("sequence: waiting for the mean confidence");
-        wait_stream(s);
+        step_mark("sequence: the mean confidence");
+        copy_d2h(ws.pin, acc, macc, 2 * sizeof(double), s);
         return acc[1] > 0 ? acc[0] / acc[1] : 0.0;
     }
 
     void begin(const float *frame0, bool device_frame, const uint8_t *mask) {
         if (mask) {
             dm = B<uint8_t>(ws.bmask, n);
-            HIPCHECK(hipMemcpyAsync(dm, mask, n, hipMemcpyHostToDevice, s));
+            copy_h2d(ws.pin, dm, mask, n, s);
         }
         float *fref = B<float>(ws.bframe_ref, n), *fprev = B<float>(ws.bframe_prev, n);
         (void)B<float>(ws.bi2, n);
-        const hipMemcpyKind kind = device_frame ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
-        HIPCHECK(hipMemcpyAsync(fref, frame0, n * sizeof(float), kind, s));
-        HIPCHECK(hipMemcpyAsync(fprev, frame0, n * sizeof(float), kind, s));
+        if (device_frame)
+            HIPCHECK(hipMemcpyAsync(fref, frame0, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        else
+            copy_h2d(ws.pin, fref, frame0, n * sizeof(float), s);
+        HIPCHECK(hipMemcpyAsync(fprev, fref, n * sizeof(float), hipMemcpyDeviceToDevice, s));
         double *u0x = B<double>(ws.bu0x, n), *u0y = B<double>(ws.bu0y, n), *c0r = B<double>(ws.bc0r, n);
         for (DevBuf *b : {&ws.bcx, &ws.bcy, &ws.bconf0, &ws.bca, &ws.bprev_x, &ws.bprev_y, &ws.bprev_c, &ws.bleg_c,
                           &ws.bstart_x, &ws.bstart_y})
@@ -975,16 +1027,13 @@ struct Seq {
             uint8_t *anch = B<uint8_t>(ws.banch, n), *good_prev = B<uint8_t>(ws.bgood_prev, n);
             double *upx = B<double>(ws.bupx, n), *upy = B<double>(ws.bupy, n), *caa = B<double>(ws.bca_adj, n);
             int *nfb = B<int>(ws.bnfb, n);
-            std::vector<uint8_t> ones8(n, 1);
-            std::vector<int> zi(n, 0);
-            HIPCHECK(hipMemcpyAsync(anch, ones8.data(), n, hipMemcpyHostToDevice, s));
-            HIPCHECK(hipMemcpyAsync(good_prev, ones8.data(), n, hipMemcpyHostToDevice, s));
-            HIPCHECK(hipMemcpyAsync(nfb, zi.data(), n * sizeof(int), hipMemcpyHostToDevice, s));
+            hipLaunchKernelGGL(k_fill_u8, dim3((n + 255) / 256), dim3(256), 0, s, anch, (uint8_t)1, n);
+            hipLaunchKernelGGL(k_fill_u8, dim3((n + 255) / 256), dim3(256), 0, s, good_prev, (uint8_t)1, n);
+            hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, s, nfb, 0, n);
             hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, upx, 0.0, n);
             hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, upy, 0.0, n);
             hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, caa, 0.0, n);
             LAUNCHCHECK();
-            wait_stream(s);  // the host vectors above go out of scope
         }
         wait_stream(s);
         k = 0;
@@ -994,8 +1043,10 @@ struct Seq {
 
     void step(const float *frame, bool device_frame, ofx_result *out, bool device_out) {
         k++;
-        HIPCHECK(hipMemcpyAsync(ws.bi2.p, frame, n * sizeof(float),
-                                device_frame ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+        if (device_frame)
+            HIPCHECK(hipMemcpyAsync(ws.bi2.p, frame, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        else
+            copy_h2d(ws.pin, ws.bi2.p, frame, n * sizeof(float), s);
         if (c.seq_hybrid)
             step_hybrid(out, device_out);
         else
@@ -1016,7 +1067,7 @@ struct Seq {
         if (device_out)
             wait_stream(s);
         else
-            download(st, out, n, s);
+            download(ws, st, out, n, s);
     }
 
     void step_chain(ofx_result *out, bool device_out) {
@@ -1095,9 +1146,8 @@ struct Seq {
         int32_t *dfb = B<int32_t>(ws.bfb, n);
         auto counters = [&]() {
             std::vector<int> hc(HYB_NCOUNTERS);
-            HIPCHECK(hipMemcpyAsync(hc.data(), cnt, HYB_NCOUNTERS * sizeof(int), hipMemcpyDeviceToHost, s));
-            step_mark("hybrid: waiting for the counters");
-            wait_stream(s);
+            step_mark("hybrid: the counters");
+            copy_d2h(ws.pin, hc.data(), cnt, HYB_NCOUNTERS * sizeof(int), s);
             return hc;
         };
         const bool gate_on = c.fb_min_support > 0;
@@ -1167,9 +1217,12 @@ struct Seq {
         if (st.d.conf) HIPCHECK(hipMemcpyAsync(st.d.conf, ws.bconf.p, n * sizeof(float), hipMemcpyDeviceToDevice, s));
         ws.prof.mark(ST_END, s);
         HIPCHECK(hipEventRecord(ws.ev[3], s));
-        if (out->fb)
-            HIPCHECK(hipMemcpyAsync(out->fb, dfb, n * sizeof(int32_t),
-                                    device_out ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+        if (out->fb) {
+            if (device_out)
+                HIPCHECK(hipMemcpyAsync(out->fb, dfb, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+            else
+                copy_d2h(ws.pin, out->fb, dfb, n * sizeof(int32_t), s);
+        }
         deliver(st, out, device_out);
         ws.prof.flush();
         const std::vector<int> hc = counters();
@@ -1357,17 +1410,17 @@ int ofx_refine_raw(ofx_engine *e, const float *i1, int w, int h, const float *i2
         const size_t n = (size_t)w * h, n2 = (size_t)w2 * h2;
         float *d1 = B<float>(ws.bi1, n), *d2 = B<float>(ws.bi2, n2);
         float *dx = B<float>(ws.binx, n), *dy = B<float>(ws.biny, n);
-        HIPCHECK(hipMemcpyAsync(d1, i1, n * sizeof(float), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(d2, i2, n2 * sizeof(float), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(dx, ux, n * sizeof(float), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(dy, uy, n * sizeof(float), hipMemcpyHostToDevice, s));
+        copy_h2d(ws.pin, d1, i1, n * sizeof(float), s);
+        copy_h2d(ws.pin, d2, i2, n2 * sizeof(float), s);
+        copy_h2d(ws.pin, dx, ux, n * sizeof(float), s);
+        copy_h2d(ws.pin, dy, uy, n * sizeof(float), s);
         to_f64(dx, B<double>(ws.bux, n), n, s);
         to_f64(dy, B<double>(ws.buy, n), n, s);
         refine_device(ws, d1, d2, w, h, w2, h2, (long)shift_x, (long)shift_y, *cfg, s);
-        if (out_ux) HIPCHECK(hipMemcpyAsync(out_ux, ws.bux.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-        if (out_uy) HIPCHECK(hipMemcpyAsync(out_uy, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-        if (out_energy) HIPCHECK(hipMemcpyAsync(out_energy, ws.benergy.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
-        if (out_resid) HIPCHECK(hipMemcpyAsync(out_resid, ws.bresid.p, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        if (out_ux) copy_d2h(ws.pin, out_ux, ws.bux.p, n * sizeof(double), s);
+        if (out_uy) copy_d2h(ws.pin, out_uy, ws.buy.p, n * sizeof(double), s);
+        if (out_energy) copy_d2h(ws.pin, out_energy, ws.benergy.p, n * sizeof(double), s);
+        if (out_resid) copy_d2h(ws.pin, out_resid, ws.bresid.p, n * sizeof(double), s);
         wait_stream(s);
     });
 }
@@ -1386,13 +1439,13 @@ int ofx_coarse_flow(ofx_engine *e, const float *i1, const float *i2, int w, int 
         hipStream_t s = g.lane->stream;
         const size_t n = (size_t)w * h;
         float *d1 = B<float>(ws.bi1, n), *d2 = B<float>(ws.bi2, n);
-        HIPCHECK(hipMemcpyAsync(d1, i1, n * sizeof(float), hipMemcpyHostToDevice, s));
-        HIPCHECK(hipMemcpyAsync(d2, i2, n * sizeof(float), hipMemcpyHostToDevice, s));
+        copy_h2d(ws.pin, d1, i1, n * sizeof(float), s);
+        copy_h2d(ws.pin, d2, i2, n * sizeof(float), s);
         double *ix = nullptr, *iy = nullptr;
         if (init_ux) {
             float *fx = B<float>(ws.binx, n), *fy = B<float>(ws.biny, n);
-            HIPCHECK(hipMemcpyAsync(fx, init_ux, n * sizeof(float), hipMemcpyHostToDevice, s));
-            HIPCHECK(hipMemcpyAsync(fy, init_uy, n * sizeof(float), hipMemcpyHostToDevice, s));
+            copy_h2d(ws.pin, fx, init_ux, n * sizeof(float), s);
+            copy_h2d(ws.pin, fy, init_uy, n * sizeof(float), s);
             ix = B<double>(ws.bstart_x, n);
             iy = B<double>(ws.bstart_y, n);
             to_f64(fx, ix, n, s);
@@ -1400,8 +1453,8 @@ int ofx_coarse_flow(ofx_engine *e, const float *i1, const float *i2, int w, int 
         }
         double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
         coarse_dense(d1, d2, w, h, *cfg, ix, iy, ux, uy, ws.coarse, s);
-        HIPCHECK(hipMemcpyAsync(out_ux, ux, n * sizeof(double), hipMemcpyDeviceToHost, s));
-        HIPCHECK(hipMemcpyAsync(out_uy, uy, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        copy_d2h(ws.pin, out_ux, ux, n * sizeof(double), s);
+        copy_d2h(ws.pin, out_uy, uy, n * sizeof(double), s);
         wait_stream(s);
     });
 }
@@ -1422,7 +1475,7 @@ bool align_device(Workspace &ws, const float *centre, const float *other, int w,
     } else {
         std::vector<int> gxs, gys;
         std::vector<double> gux, guy, gpk, xs, ys, vx, vy;
-        coarse_grid(ws.coarse, w, h, c, gxs, gys, gux, guy, gpk, s);
+        coarse_grid(ws.coarse, w, h, c, gxs, gys, gux, guy, gpk, ws.pin, s);
         for (size_t iy = 0; iy < gys.size(); iy++)
             for (size_t ix = 0; ix < gxs.size(); ix++) {
                 const size_t j = iy * gxs.size() + ix;
@@ -1463,12 +1516,12 @@ int ofx_tavg_window(ofx_engine *e, const float *frames, int nf, int centre, int 
         const size_t n = (size_t)w * h;
         float *dc = B<float>(ws.bi1, n), *dother = B<float>(ws.bi2, n), *da = B<float>(ws.btav_a, n);
         double *acc = B<double>(ws.btav_acc, n);
-        HIPCHECK(hipMemcpyAsync(dc, frames + (size_t)centre * n, n * sizeof(float), hipMemcpyHostToDevice, s));
+        copy_h2d(ws.pin, dc, frames + (size_t)centre * n, n * sizeof(float), s);
         mean_add(acc, dc, n, true, s);
         int used = 1;
         for (int i = 0; i < nf; i++) {
             if (i == centre) continue;
-            HIPCHECK(hipMemcpyAsync(dother, frames + (size_t)i * n, n * sizeof(float), hipMemcpyHostToDevice, s));
+            copy_h2d(ws.pin, dother, frames + (size_t)i * n, n * sizeof(float), s);
             if (!align) {
                 mean_add(acc, dother, n, false, s);
                 used++;
@@ -1481,11 +1534,11 @@ int ofx_tavg_window(ofx_engine *e, const float *frames, int nf, int centre, int 
             }
         }
         if (used == 1)
-            HIPCHECK(hipMemcpyAsync(out, dc, n * sizeof(float), hipMemcpyDeviceToHost, s));
+            copy_d2h(ws.pin, out, dc, n * sizeof(float), s);
         else {
             float *dout = B<float>(ws.btav_fx, n);
             mean_finish(acc, (double)used, dout, n, s);
-            HIPCHECK(hipMemcpyAsync(out, dout, n * sizeof(float), hipMemcpyDeviceToHost, s));
+            copy_d2h(ws.pin, out, dout, n * sizeof(float), s);
         }
         wait_stream(s);
         if (n_used) *n_used = used;
@@ -1508,9 +1561,9 @@ int ofx_frame_to_gray(ofx_engine *e, const void *pixels, int bits, int channels,
         if (ow < 1 || oh < 1) return;
         char *dp = B<char>(ws.bpix, bytes);
         float *dout = B<float>(ws.btav_fx, (size_t)ow * oh);
-        HIPCHECK(hipMemcpyAsync(dp, pixels, bytes, hipMemcpyHostToDevice, s));
+        copy_h2d(ws.pin, dp, pixels, bytes, s);
         frame_to_gray(dp, bits, channels, w, h, factor, dout, s);
-        HIPCHECK(hipMemcpyAsync(out, dout, (size_t)ow * oh * sizeof(float), hipMemcpyDeviceToHost, s));
+        copy_d2h(ws.pin, out, dout, (size_t)ow * oh * sizeof(float), s);
         wait_stream(s);
     });
 }
@@ -1587,6 +1640,9 @@ int ofx_track_pairs(ofx_engine *e, int n, const ofx_pair *pairs, const ofx_confi
         std::atomic<int> next{0};
         std::vector<int> codes(n, OFX_OK);
         std::vector<std::string> msgs(n);
+        // memory a growing buffer gives up is freed after every lane has stopped
+        std::mutex held_mu;
+        std::vector<void *> given_up;
         auto work = [&](Lane *lane) {
             Progress &pg = progress();
             pg.call_id.fetch_add(1);
@@ -1603,6 +1659,17 @@ int ofx_track_pairs(ofx_engine *e, int n, const ofx_pair *pairs, const ofx_confi
                 return;
             }
             if (!lane->ws) lane->ws.reset(new Workspace);
+            DeferFrees defer;
+            struct Hand {
+                DeferFrees &d;
+                std::mutex &mu;
+                std::vector<void *> &out;
+                ~Hand() {
+                    std::lock_guard<std::mutex> lk(mu);
+                    out.insert(out.end(), d.held.begin(), d.held.end());
+                    d.held.clear();
+                }
+            } hand{defer, held_mu, given_up};
             for (int k; (k = next.fetch_add(1)) < n;) {
                 const int i = order[k];
                 const ofx_pair &p = pairs[i];
@@ -1625,6 +1692,7 @@ int ofx_track_pairs(ofx_engine *e, int n, const ofx_pair *pairs, const ofx_confi
         }
         work(held[0]);
         for (auto &t : threads) t.join();
+        for (void *p : given_up) (void)hipFree(p);
         for (int i = 0; i < n; i++)
             if (codes[i] != OFX_OK) throw Error(codes[i], "pair " + std::to_string(i) + ": " + msgs[i]);
     });

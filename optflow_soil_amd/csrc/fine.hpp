@@ -66,7 +66,8 @@ void coarse_dense(const float *i1, const float *i2, int w, int h, const ofx_conf
                   const double *init_uy, double *ux, double *uy, CoarseWs *&ws, hipStream_t s);
 void coarse_release(CoarseWs *&ws);
 void coarse_grid(const CoarseWs *ws, int w, int h, const ofx_config &c, std::vector<int> &xs, std::vector<int> &ys,
-                 std::vector<double> &ux, std::vector<double> &uy, std::vector<double> &peak, hipStream_t s);
+                 std::vector<double> &ux, std::vector<double> &uy, std::vector<double> &peak, PinnedBuf &pin,
+                 hipStream_t s);
 
 // tavg.hip: temporal averaging and frame conversion (tavg.rs, io.rs)
 // acc (f64, n) = frame (first) or acc += frame (in the order of the calls)

@@ -52,6 +52,27 @@ struct HipCallMark {
 void wait_stream(hipStream_t s);
 void wait_event(hipEvent_t e);
 
+// Device memory a growing buffer gives up: freed at once, or, while a deferral scope is active on
+// this thread (the batched call, whose other lanes keep working), kept until the scope ends.
+void release_device(void *p);
+struct DeferFrees {
+    DeferFrees();
+    ~DeferFrees();  // frees what was deferred on this thread
+    std::vector<void *> held;
+    DeferFrees *prev;
+};
+
+// Host copies go through pinned staging of at most PIN_CHUNK bytes per step (pageable memory is
+// never handed to the runtime's asynchronous copies): each call returns with the data in place.
+constexpr size_t PIN_CHUNK = 8u << 20;
+struct PinnedBuf {
+    char *p = nullptr;
+    ~PinnedBuf();
+    char *get();
+};
+void copy_h2d(PinnedBuf &pin, void *dev, const void *host, size_t bytes, hipStream_t s);
+void copy_d2h(PinnedBuf &pin, void *host, const void *dev, size_t bytes, hipStream_t s);
+
 #define HIPCHECK(x)                                                                          \
     do {                                                                                     \
         ::ofx::HipCallMark mark_(#x);                                                        \

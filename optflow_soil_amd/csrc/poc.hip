@@ -17,6 +17,7 @@ constexpr int POC_MAX = 4096;  // window elements a workgroup holds (64 x 64, 16
 struct CoarseWs {
     int w = 0, h = 0, L = 0;
     std::vector<float *> pa, pb;   // pyramid levels 1..L (level 0 = the inputs)
+    std::vector<size_t> pcap;      // capacity of pa[l], pb[l] in floats
     std::vector<int> lw, lh;
     double *tmp = nullptr;         // blur scratch
     double *gux = nullptr, *guy = nullptr, *gpk = nullptr, *ix = nullptr, *iy = nullptr;
@@ -660,7 +661,7 @@ __global__ void k_dense(GridGeom g, const double *gux, const double *guy, const 
 template <class T>
 void ensure(T *&p, size_t n, size_t &cap) {
     if (p && cap >= n) return;
-    if (p) (void)hipFree(p);
+    if (p) release_device(p);
     HIPCHECK(hipMalloc(&p, n * sizeof(T)));
     cap = n;
 }
@@ -670,7 +671,8 @@ void ensure(T *&p, size_t n, size_t &cap) {
 // The level-0 grid of the last coarse_dense call with this workspace: positions and (ux, uy,
 // peak) per point, ny x nx, on the host.
 void coarse_grid(const CoarseWs *ws, int w, int h, const ofx_config &c, std::vector<int> &xs, std::vector<int> &ys,
-                 std::vector<double> &ux, std::vector<double> &uy, std::vector<double> &peak, hipStream_t s) {
+                 std::vector<double> &ux, std::vector<double> &uy, std::vector<double> &peak, PinnedBuf &pin,
+                 hipStream_t s) {
     if (!ws || !ws->gux) throw Error(OFX_EINVAL, "no coarse grid computed");
     int x0, nx, y0, ny;
     grid_axis(w, c.poc_win, c.grid_step, x0, nx);
@@ -683,10 +685,9 @@ void coarse_grid(const CoarseWs *ws, int w, int h, const ofx_config &c, std::vec
     ux.resize(np);
     uy.resize(np);
     peak.resize(np);
-    HIPCHECK(hipMemcpyAsync(ux.data(), ws->gux, np * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(uy.data(), ws->guy, np * sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHECK(hipMemcpyAsync(peak.data(), ws->gpk, np * sizeof(double), hipMemcpyDeviceToHost, s));
-    wait_stream(s);
+    copy_d2h(pin, ux.data(), ws->gux, np * sizeof(double), s);
+    copy_d2h(pin, uy.data(), ws->guy, np * sizeof(double), s);
+    copy_d2h(pin, peak.data(), ws->gpk, np * sizeof(double), s);
 }
 
 void coarse_release(CoarseWs *&ws) {
@@ -709,28 +710,34 @@ void coarse_release(CoarseWs *&ws) {
 void coarse_dense(const float *i1, const float *i2, int w, int h, const ofx_config &c, const double *init_ux,
                   const double *init_uy, double *ux, double *uy, CoarseWs *&ws, hipStream_t s) {
     const int L = n_levels(c.max_disp, c.poc_win, h, w);
-    if (ws && (ws->w != w || ws->h != h || ws->L != L)) coarse_release(ws);
+    // the workspace keeps every buffer while it is large enough (no frees when the size shrinks)
     if (!ws) {
         ws = new CoarseWs;
-        ws->w = w;
-        ws->h = h;
-        ws->L = L;
+        HIPCHECK(hipMalloc(&ws->hann, 2 * (size_t)POC_MAX * sizeof(float)));
+    }
+    ws->w = w;
+    ws->h = h;
+    ws->L = L;
+    ws->lw.assign(L, 0);
+    ws->lh.assign(L, 0);
+    {
         int lw = w, lh = h;
         for (int l = 0; l < L; l++) {
             const int nw = (lw + 1) / 2, nh = (lh + 1) / 2;
-            float *a, *b;
-            HIPCHECK(hipMalloc(&a, (size_t)nw * nh * sizeof(float)));
-            HIPCHECK(hipMalloc(&b, (size_t)nw * nh * sizeof(float)));
-            ws->pa.push_back(a);
-            ws->pb.push_back(b);
-            ws->lw.push_back(nw);
-            ws->lh.push_back(nh);
+            if ((int)ws->pa.size() <= l) {
+                ws->pa.push_back(nullptr);
+                ws->pb.push_back(nullptr);
+                ws->pcap.push_back(0);
+            }
+            size_t cap_b = ws->pcap[l];
+            ensure(ws->pa[l], (size_t)nw * nh, ws->pcap[l]);
+            ensure(ws->pb[l], (size_t)nw * nh, cap_b);
+            ws->lw[l] = nw;
+            ws->lh[l] = nh;
             lw = nw;
             lh = nh;
         }
-        size_t cap = 0;
-        ensure(ws->tmp, (size_t)w * ((h + 1) / 2), cap);
-        HIPCHECK(hipMalloc(&ws->hann, 2 * (size_t)POC_MAX * sizeof(float)));
+        ensure(ws->tmp, (size_t)w * ((h + 1) / 2), ws->tmpcap);
     }
     // pyramid
     for (int l = 0; l < L; l++) {
@@ -757,8 +764,8 @@ void coarse_dense(const float *i1, const float *i2, int w, int h, const ofx_conf
         npmax = std::max(npmax, (size_t)nx * ny);
     }
     if (!ws->gux || ws->gcap < npmax) {
-        (void)hipFree(ws->gux); (void)hipFree(ws->guy); (void)hipFree(ws->gpk);
-        (void)hipFree(ws->ix); (void)hipFree(ws->iy); (void)hipFree(ws->gflag); (void)hipFree(ws->gwork);
+        release_device(ws->gux); release_device(ws->guy); release_device(ws->gpk);
+        release_device(ws->ix); release_device(ws->iy); release_device(ws->gflag); release_device(ws->gwork);
         HIPCHECK(hipMalloc(&ws->gux, npmax * sizeof(double) * 2));
         HIPCHECK(hipMalloc(&ws->guy, npmax * sizeof(double) * 2));
         HIPCHECK(hipMalloc(&ws->gpk, npmax * sizeof(double)));
