@@ -150,7 +150,10 @@ PinnedBuf::~PinnedBuf() {
     if (p) (void)hipHostFree(p);
 }
 char *PinnedBuf::get() {
-    if (!p) HIPCHECK(hipHostMalloc((void **)&p, PIN_CHUNK, hipHostMallocDefault));
+    if (!p) {
+        HIPCHECK(hipHostMalloc((void **)&p, PIN_CHUNK, hipHostMallocMapped | hipHostMallocCoherent));
+        HIPCHECK(hipHostGetDevicePointer((void **)&dp, p, 0));
+    }
     return p;
 }
 void copy_h2d(PinnedBuf &pin, void *dev, const void *host, size_t bytes, hipStream_t s) {
@@ -160,7 +163,7 @@ void copy_h2d(PinnedBuf &pin, void *dev, const void *host, size_t bytes, hipStre
         const size_t len = std::min(PIN_CHUNK, bytes - off);
         wait_stream(s);  // the staging buffer is free again
         std::memcpy(b, (const char *)host + off, len);
-        HIPCHECK(hipMemcpyAsync((char *)dev + off, b, len, hipMemcpyHostToDevice, s));
+        dcopy((char *)dev + off, pin.dp, len, s);
     }
     wait_stream(s);
 }
@@ -169,7 +172,7 @@ void copy_d2h(PinnedBuf &pin, void *host, const void *dev, size_t bytes, hipStre
     char *b = pin.get();
     for (size_t off = 0; off < bytes; off += PIN_CHUNK) {
         const size_t len = std::min(PIN_CHUNK, bytes - off);
-        HIPCHECK(hipMemcpyAsync(b, (const char *)dev + off, len, hipMemcpyDeviceToHost, s));
+        dcopy(pin.dp, (const char *)dev + off, len, s);
         wait_stream(s);
         std::memcpy((char *)host + off, b, len);
     }
@@ -459,7 +462,7 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
     ws.prof.mark(ST_WARP, s);
     const size_t n2 = (size_t)w2 * h2;
     float *c2 = B<float>(ws.bc2, n2);
-    HIPCHECK(hipMemcpyAsync(c2, i2, n2 * sizeof(float), hipMemcpyDeviceToDevice, s));
+    dcopy(c2, i2, n2 * sizeof(float), s);
     prefilter_real(c2, B<float>(ws.bpftmp, std::max(n, n2)), w2, h2, s);
     double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
     double *nux = B<double>(ws.bnux, n), *nuy = B<double>(ws.bnuy, n);
@@ -494,7 +497,7 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
             } else {
                 confidence(energy, resid, med, ux, uy, c.phase_tol, c.conf_min, 0.0, w, h, low, nullptr, nullptr, s);
                 if (freeze >= 1 && it == freeze) {
-                    HIPCHECK(hipMemcpyAsync(kept, low, n, hipMemcpyDeviceToDevice, s));
+                    dcopy(kept, low, n, s);
                     have_kept = true;
                 }
             }
@@ -514,8 +517,8 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
         fft_forward_padded(warped, w, h, spec, tmp, tmp2, s);
         for (int k = 0; k < F; k++) fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G, true, tmp, tmp2, resp + k * np, s);
         ws.prof.mark(ST_SMOOTH, s);
-        HIPCHECK(hipMemcpyAsync(csx, sx, n * sizeof(float), hipMemcpyDeviceToDevice, s));
-        HIPCHECK(hipMemcpyAsync(csy, sy, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        dcopy(csx, sx, n * sizeof(float), s);
+        dcopy(csy, sy, n * sizeof(float), s);
         prefilter_real(csx, B<float>(ws.bpftmp, n), w, h, s);
         prefilter_real(csy, B<float>(ws.bpftmp, n), w, h, s);
         for (int in = 0; in < c.inner_iters; in++) {
@@ -626,8 +629,8 @@ void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, c
     }
     const size_t n = (size_t)w * h;
     double *fsx = B<double>(ws.bfsx, n), *fsy = B<double>(ws.bfsy, n);
-    HIPCHECK(hipMemcpyAsync(fsx, ws.bux.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    HIPCHECK(hipMemcpyAsync(fsy, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    dcopy(fsx, ws.bux.p, n * sizeof(double), s);
+    dcopy(fsy, ws.buy.p, n * sizeof(double), s);
     double *jx = B<double>(ws.bjx, n), *jy = B<double>(ws.bjy, n), *je = B<double>(ws.bje, n),
            *jr = B<double>(ws.bjr, n);
     // every plane the tiles use and the joined result is copied back into holds the whole frame
@@ -688,15 +691,15 @@ void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, c
         span(ext.y0, ext.y1, loy, hiy, h, win.y0, win.y1);
         const int ww = win.x1 - win.x0, wh = win.y1 - win.y0;
         float *t1 = B<float>(ws.btile1, ne), *t2 = B<float>(ws.btile2, (size_t)ww * wh);
-        HIPCHECK(hipMemcpy2DAsync(t1, ew * sizeof(float), i1 + (size_t)ext.y0 * w + ext.x0, w * sizeof(float),
-                                  ew * sizeof(float), eh, hipMemcpyDeviceToDevice, s));
-        HIPCHECK(hipMemcpy2DAsync(t2, ww * sizeof(float), i2 + (size_t)win.y0 * w + win.x0, w * sizeof(float),
-                                  ww * sizeof(float), wh, hipMemcpyDeviceToDevice, s));
+        dcopy2d(t1, ew * sizeof(float), i1 + (size_t)ext.y0 * w + ext.x0, w * sizeof(float),
+                                  ew * sizeof(float), eh, s);
+        dcopy2d(t2, ww * sizeof(float), i2 + (size_t)win.y0 * w + win.x0, w * sizeof(float),
+                                  ww * sizeof(float), wh, s);
         double *sx = B<double>(ws.bux, ne), *sy = B<double>(ws.buy, ne);
-        HIPCHECK(hipMemcpy2DAsync(sx, ew * sizeof(double), fsx + (size_t)ext.y0 * w + ext.x0, w * sizeof(double),
-                                  ew * sizeof(double), eh, hipMemcpyDeviceToDevice, s));
-        HIPCHECK(hipMemcpy2DAsync(sy, ew * sizeof(double), fsy + (size_t)ext.y0 * w + ext.x0, w * sizeof(double),
-                                  ew * sizeof(double), eh, hipMemcpyDeviceToDevice, s));
+        dcopy2d(sx, ew * sizeof(double), fsx + (size_t)ext.y0 * w + ext.x0, w * sizeof(double),
+                                  ew * sizeof(double), eh, s);
+        dcopy2d(sy, ew * sizeof(double), fsy + (size_t)ext.y0 * w + ext.x0, w * sizeof(double),
+                                  ew * sizeof(double), eh, s);
         refine_device(ws, t1, t2, ew, eh, ww, wh, ext.x0 - win.x0, ext.y0 - win.y0, c, s);
         const int cw = core.x1 - core.x0;
         hipLaunchKernelGGL(k_write_core, dim3((cw + 255) / 256, core.y1 - core.y0), dim3(256), 0, s,
@@ -706,10 +709,10 @@ void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, c
     }
     double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
     double *energy = B<double>(ws.benergy, n), *resid = B<double>(ws.bresid, n);
-    HIPCHECK(hipMemcpyAsync(ux, jx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    HIPCHECK(hipMemcpyAsync(uy, jy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    HIPCHECK(hipMemcpyAsync(energy, je, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    HIPCHECK(hipMemcpyAsync(resid, jr, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    dcopy(ux, jx, n * sizeof(double), s);
+    dcopy(uy, jy, n * sizeof(double), s);
+    dcopy(energy, je, n * sizeof(double), s);
+    dcopy(resid, jr, n * sizeof(double), s);
     double *med = B<double>(ws.bmed, 4);
     ws.prof.mark(ST_CONFIDENCE, s);
     median(energy, n, ws.bmedscr.get<char>(median_scratch_bytes()), med, s);
@@ -727,7 +730,7 @@ void finish_device(Workspace &ws, const double *ux, const double *uy, const floa
     const double guard = edge_guard(c);
     const int guard_on = guard > 0;
     ws.prof.mark(ST_GUARD, s);
-    if (low_in) HIPCHECK(hipMemcpyAsync(low, low_in, n, hipMemcpyDeviceToDevice, s));
+    if (low_in) dcopy(low, low_in, n, s);
     else low_mask(conf, c.conf_min, n, low, s);
     if (guard_on) {
         uint8_t *large = B<uint8_t>(ws.blarge, n), *src = B<uint8_t>(ws.bsrc, n);
@@ -815,7 +818,7 @@ void pair_device(Workspace &ws, const float *i1, const float *i2, int w, int h, 
     finish_device(ws, ux, uy, (const float *)ws.bconf.p, nullptr, w, h, mask, c, o, s);
     to_f32(ux, o->raw_ux, n, s);
     to_f32(uy, o->raw_uy, n, s);
-    if (o->conf) HIPCHECK(hipMemcpyAsync(o->conf, ws.bconf.p, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    if (o->conf) dcopy(o->conf, ws.bconf.p, n * sizeof(float), s);
     ws.prof.mark(ST_END, s);
     HIPCHECK(hipEventRecord(ws.ev[3], s));
 }
@@ -978,9 +981,9 @@ struct Seq {
         HIPCHECK(hipEventRecord(ws.ev[1], s));
         fine_stage(ws, a, b, w, h, c, s);
         HIPCHECK(hipEventRecord(ws.ev[2], s));
-        HIPCHECK(hipMemcpyAsync(ox, ws.bux.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        HIPCHECK(hipMemcpyAsync(oy, ws.buy.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        HIPCHECK(hipMemcpyAsync(oc, ws.bconf64.p, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        dcopy(ox, ws.bux.p, n * sizeof(double), s);
+        dcopy(oy, ws.buy.p, n * sizeof(double), s);
+        dcopy(oc, ws.bconf64.p, n * sizeof(double), s);
         step_mark("sequence: waiting for a leg");
         wait_event(ws.ev[2]);
         return std::make_pair(elapsed(ws.ev[0], ws.ev[1]) * 1e-3, elapsed(ws.ev[1], ws.ev[2]) * 1e-3);
@@ -1011,10 +1014,10 @@ struct Seq {
         float *fref = B<float>(ws.bframe_ref, n), *fprev = B<float>(ws.bframe_prev, n);
         (void)B<float>(ws.bi2, n);
         if (device_frame)
-            HIPCHECK(hipMemcpyAsync(fref, frame0, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+            dcopy(fref, frame0, n * sizeof(float), s);
         else
             copy_h2d(ws.pin, fref, frame0, n * sizeof(float), s);
-        HIPCHECK(hipMemcpyAsync(fprev, fref, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        dcopy(fprev, fref, n * sizeof(float), s);
         double *u0x = B<double>(ws.bu0x, n), *u0y = B<double>(ws.bu0y, n), *c0r = B<double>(ws.bc0r, n);
         for (DevBuf *b : {&ws.bcx, &ws.bcy, &ws.bconf0, &ws.bca, &ws.bprev_x, &ws.bprev_y, &ws.bprev_c, &ws.bleg_c,
                           &ws.bstart_x, &ws.bstart_y})
@@ -1044,7 +1047,7 @@ struct Seq {
     void step(const float *frame, bool device_frame, ofx_result *out, bool device_out) {
         k++;
         if (device_frame)
-            HIPCHECK(hipMemcpyAsync(ws.bi2.p, frame, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+            dcopy(ws.bi2.p, frame, n * sizeof(float), s);
         else
             copy_h2d(ws.pin, ws.bi2.p, frame, n * sizeof(float), s);
         if (c.seq_hybrid)
@@ -1084,9 +1087,9 @@ struct Seq {
             conf_at(pc, u0x, u0y, w, h, ca, s);
             hipLaunchKernelGGL(k_min_inplace, dim3((n + 255) / 256), dim3(256), 0, s, c0r, (const double *)ca, n);
             compose(u0x, u0y, px, py, w, h, cx, cy, s);
-            HIPCHECK(hipMemcpyAsync(u0x, cx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HIPCHECK(hipMemcpyAsync(u0y, cy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HIPCHECK(hipMemcpyAsync(fref, fprev, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+            dcopy(u0x, cx, n * sizeof(double), s);
+            dcopy(u0y, cy, n * sizeof(double), s);
+            dcopy(fref, fprev, n * sizeof(float), s);
             ref_idx = k - 1;
             tl = run_leg(false);
             cm = leg_mean();
@@ -1101,7 +1104,7 @@ struct Seq {
         finish_device(ws, cx, cy, (const float *)ws.bconf.p, nullptr, w, h, dm, c, &st.d, s);
         to_f32(cx, st.d.raw_ux, n, s);
         to_f32(cy, st.d.raw_uy, n, s);
-        if (st.d.conf) HIPCHECK(hipMemcpyAsync(st.d.conf, ws.bconf.p, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        if (st.d.conf) dcopy(st.d.conf, ws.bconf.p, n * sizeof(float), s);
         ws.prof.mark(ST_END, s);
         HIPCHECK(hipEventRecord(ws.ev[3], s));
         deliver(st, out, device_out);
@@ -1113,10 +1116,10 @@ struct Seq {
         out->reference = ref_idx;
         out->mean_conf = cm;
         // this leg becomes prev
-        HIPCHECK(hipMemcpyAsync(px, lx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        HIPCHECK(hipMemcpyAsync(py, ly, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        HIPCHECK(hipMemcpyAsync(pc, lc, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        HIPCHECK(hipMemcpyAsync(fprev, fcur, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        dcopy(px, lx, n * sizeof(double), s);
+        dcopy(py, ly, n * sizeof(double), s);
+        dcopy(pc, lc, n * sizeof(double), s);
+        dcopy(fprev, fcur, n * sizeof(float), s);
         have_prev = true;
         wait_stream(s);
     }
@@ -1157,9 +1160,9 @@ struct Seq {
         if (cm < c.ref_update_conf && ref_idx != k - 1) {
             conf_at(pc, u0x, u0y, w, h, ca, s);
             hyb_refmove(good_prev, ca, c0r, anch, n, s);
-            HIPCHECK(hipMemcpyAsync(u0x, upx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HIPCHECK(hipMemcpyAsync(u0y, upy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HIPCHECK(hipMemcpyAsync(fref, fprev, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+            dcopy(u0x, upx, n * sizeof(double), s);
+            dcopy(u0y, upy, n * sizeof(double), s);
+            dcopy(fref, fprev, n * sizeof(float), s);
             ref_idx = k - 1;
             tl = run_leg(false);
             cm = leg_mean();
@@ -1185,12 +1188,12 @@ struct Seq {
             conf_at(rc, upx, upy, w, h, caa, s);
             hyb_carried(cax, cay, upx, upy, ax, ay, n, s);
             // the adjacent leg's raw field starts the next adjacent leg
-            HIPCHECK(hipMemcpyAsync(pajx, rx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HIPCHECK(hipMemcpyAsync(pajy, ry, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            dcopy(pajx, rx, n * sizeof(double), s);
+            dcopy(pajy, ry, n * sizeof(double), s);
             have_adj = true;
         } else {
-            HIPCHECK(hipMemcpyAsync(ax, upx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HIPCHECK(hipMemcpyAsync(ay, upy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            dcopy(ax, upx, n * sizeof(double), s);
+            dcopy(ay, upy, n * sizeof(double), s);
             hipLaunchKernelGGL(k_fill, dim3((n + 255) / 256), dim3(256), 0, s, caa, 0.0, n);
             LAUNCHCHECK();
         }
@@ -1214,12 +1217,12 @@ struct Seq {
         finish_device(ws, hux, huy, (const float *)ws.bconf.p, hlow, w, h, dm, c, &st.d, s);
         to_f32(hux, st.d.raw_ux, n, s);
         to_f32(huy, st.d.raw_uy, n, s);
-        if (st.d.conf) HIPCHECK(hipMemcpyAsync(st.d.conf, ws.bconf.p, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        if (st.d.conf) dcopy(st.d.conf, ws.bconf.p, n * sizeof(float), s);
         ws.prof.mark(ST_END, s);
         HIPCHECK(hipEventRecord(ws.ev[3], s));
         if (out->fb) {
             if (device_out)
-                HIPCHECK(hipMemcpyAsync(out->fb, dfb, n * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+                dcopy(out->fb, dfb, n * sizeof(int32_t), s);
             else
                 copy_d2h(ws.pin, out->fb, dfb, n * sizeof(int32_t), s);
         }
@@ -1237,14 +1240,14 @@ struct Seq {
         out->fb_gate_rejected = hc[HYB_N_REJECTED] / nroi;
         out->reacquired = hc[HYB_N_RE] / nroi;
         // state for frame k + 1
-        HIPCHECK(hipMemcpyAsync(upx, hux, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        HIPCHECK(hipMemcpyAsync(upy, huy, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        dcopy(upx, hux, n * sizeof(double), s);
+        dcopy(upy, huy, n * sizeof(double), s);
         hyb_good_of(flags, good_prev, n, s);
         have_prev_adj = have_adj;
-        HIPCHECK(hipMemcpyAsync(px, lx, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        HIPCHECK(hipMemcpyAsync(py, ly, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        HIPCHECK(hipMemcpyAsync(pc, lc, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-        HIPCHECK(hipMemcpyAsync(fprev, fcur, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+        dcopy(px, lx, n * sizeof(double), s);
+        dcopy(py, ly, n * sizeof(double), s);
+        dcopy(pc, lc, n * sizeof(double), s);
+        dcopy(fprev, fcur, n * sizeof(float), s);
         have_prev = true;
         wait_stream(s);
     }
@@ -1492,7 +1495,7 @@ bool align_device(Workspace &ws, const float *centre, const float *other, int w,
         affine_field(cx, cy, w, h, fx, fy, s);
     }
     float *coef = B<float>(ws.btav_c, n), *tmp = B<float>(ws.btav_tmp, n);
-    HIPCHECK(hipMemcpyAsync(coef, other, n * sizeof(float), hipMemcpyDeviceToDevice, s));
+    dcopy(coef, other, n * sizeof(float), s);
     prefilter_real(coef, tmp, w, h, s);
     warp(coef, w, h, 0, 0, fx, fy, out, w, h, s);
     return true;

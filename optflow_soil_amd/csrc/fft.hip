@@ -172,7 +172,7 @@ std::map<int, FftPlan> g_plans;
 
 }  // namespace
 
-const FftPlan &fft_plan(int n) {
+const FftPlan &fft_plan(int n, hipStream_t s) {
     std::lock_guard<std::mutex> lock(g_plan_mu);
     auto it = g_plans.find(n);
     if (it != g_plans.end()) return it->second;
@@ -195,13 +195,11 @@ const FftPlan &fft_plan(int n) {
         if (m % f == 0) push(f);
         else f += 2;
     }
-    std::vector<float2> tw(n);
-    for (int e = 0; e < n; e++) {
-        double a = -2.0 * M_PI * (double)e / (double)n;
-        tw[e] = make_float2((float)std::cos(a), (float)std::sin(a));
-    }
+    // exp(-2 pi i e / n) from the double angle, rounded to float; ready before any other lane
+    // can find the plan
     HIPCHECK(hipMalloc(&p.tw, n * sizeof(float2)));
-    HIPCHECK(hipMemcpy(p.tw, tw.data(), n * sizeof(float2), hipMemcpyHostToDevice));
+    make_twiddles(p.tw, n, s);
+    wait_stream(s);
     return g_plans[n] = p;
 }
 
@@ -234,7 +232,7 @@ size_t line_lds_bytes(int n) { return (2 * (size_t)lp_len<4>(n) + (size_t)n) * s
 template <int MODE>
 void launch_lines(const float2 *in, float2 *out, int n, int lines, const Bank *bank, int k, const LineArgs &A,
                   hipStream_t s) {
-    const FftPlan &P = fft_plan(n);
+    const FftPlan &P = fft_plan(n, s);
     const size_t lds = line_lds_bytes(n);
     if (lds > 65536) {
         static std::once_flag once;

@@ -395,7 +395,7 @@ void hyb_gate(const uint8_t *flags, const uint8_t *mask, const double *cx, const
     const int r = (int)std::max(1.0, std::nearbyint(radius));
     const double ridge = 1e-3 * (double)r * (double)r;
     hipLaunchKernelGGL(k_gate_support, dim3(nblk(n)), dim3(256), 0, s, flags, mask, cx, cy, support, gx, gy, n);
-    HIPCHECK(hipMemcpyAsync(sel, support, n, hipMemcpyDeviceToDevice, s));
+    dcopy(sel, support, n, s);
     for (int pass = 0; pass < 2; pass++) {
         hipLaunchKernelGGL(k_gate_rows, grid2(w, h), dim3(256), 0, s, sel, support, gx, gy, w, h, r, rows);
         hipLaunchKernelGGL(k_gate_cols_fit, dim3((w + 255) / 256, (h + GATE_SEG - 1) / GATE_SEG), dim3(256), 0, s,

@@ -62,11 +62,17 @@ struct DeferFrees {
     DeferFrees *prev;
 };
 
-// Host copies go through pinned staging of at most PIN_CHUNK bytes per step (pageable memory is
-// never handed to the runtime's asynchronous copies): each call returns with the data in place.
+// Copies by kernel (copy.hip): device or host-mapped pinned memory on either side, enqueued on s.
+void dcopy(void *dst, const void *src, size_t bytes, hipStream_t s);
+void dcopy2d(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width, size_t height, hipStream_t s);
+void make_twiddles(float2 *tw, int n, hipStream_t s);
+
+// Host copies go through host-mapped pinned staging of at most PIN_CHUNK bytes per step, moved
+// by dcopy: each call returns with the data in place.
 constexpr size_t PIN_CHUNK = 8u << 20;
 struct PinnedBuf {
-    char *p = nullptr;
+    char *p = nullptr;   // host address
+    char *dp = nullptr;  // the same memory as the device sees it
     ~PinnedBuf();
     char *get();
 };
@@ -95,7 +101,7 @@ struct FftPlan {
     float2 *tw = nullptr;  // device: exp(-2 pi i e / n), e in [0, n)
 };
 
-const FftPlan &fft_plan(int n);  // cached per length, device tables never freed
+const FftPlan &fft_plan(int n, hipStream_t s);  // cached per length, device tables never freed
 void fft_release();
 
 // Filter bank parameters for the on-the-fly evaluation of G_k (phaseflow.py:73-89).

@@ -795,8 +795,8 @@ void coarse_dense(const float *i1, const float *i2, int w, int h, const ofx_conf
         double *pux = ws->gux + ws->gcap, *puy = ws->guy + ws->gcap;
         int have_init = 0;
         if (have_prev) {
-            HIPCHECK(hipMemcpyAsync(pux, ws->gux, (size_t)prev.nx * prev.ny * sizeof(double), hipMemcpyDeviceToDevice, s));
-            HIPCHECK(hipMemcpyAsync(puy, ws->guy, (size_t)prev.nx * prev.ny * sizeof(double), hipMemcpyDeviceToDevice, s));
+            dcopy(pux, ws->gux, (size_t)prev.nx * prev.ny * sizeof(double), s);
+            dcopy(puy, ws->guy, (size_t)prev.nx * prev.ny * sizeof(double), s);
             hipLaunchKernelGGL(k_level_init, dim3((np + 255) / 256), dim3(256), 0, s, prev, pux, puy, cur, ws->ix, ws->iy);
             have_init = 1;
         } else if (init_ux) {
@@ -805,7 +805,7 @@ void coarse_dense(const float *i1, const float *i2, int w, int h, const ofx_conf
             have_init = 1;
         }
         LAUNCHCHECK();
-        const FftPlan &PX = fft_plan(wx), &PY = fft_plan(wy);
+        const FftPlan &PX = fft_plan(wx, s), &PY = fft_plan(wy, s);
         PocArgs P;
         std::memset(&P, 0, sizeof(P));
         P.W = lw;

@@ -337,3 +337,53 @@ def test_refine_border_fixture(eng):
          "uy_max": float(np.abs(uy - load("refine_border_uy")).max())}
     print("border " + " ".join(f"{k}={v:.3g}" for k, v in e.items()), flush=True)
     assert e["ux_max"] < 1e-3 and e["uy_max"] < 1e-3
+
+
+PAIR_VARIANTS = {
+    "freeze3": dict(prewarp_freeze_at=3),
+    "order2": dict(shape_order=2),
+    "bank6x2": dict(n_orient=6, scales=(0.8, 1.3)),
+    "windows3": dict(sigma_w=3.0, sigma_s=2.5),
+    "guard_off": dict(edge_guard=0.0),
+    "guard20": dict(edge_guard=20.0, edge_guard_min_area=100.0),
+    "strain21": dict(strain_win=21),
+    "iters4x3": dict(outer_iters=4, inner_iters=3),
+    "strict": dict(conf_min=0.5, phase_tol=0.3),
+    "two_levels": dict(max_disp=48.0),
+}
+
+
+@pytest.mark.parametrize("variant", sorted(PAIR_VARIANTS))
+def test_pair_settings(eng, variant):
+    """Whole pairs with non-default settings, GPU against the oracle at the pair bar: the freeze of
+    the low mask (prewarp.rs LowFreeze), the quadratic shape function, another filter bank, other
+    window widths (the generic moment kernels), the guard off or wider, a wider strain window, other
+    iteration counts, stricter confidence, and max_disp 48 (two POC pyramid levels and non-square
+    level windows on the generic POC kernel)."""
+    from optflow_soil_amd import synth
+    h, w = (256, 256) if variant == "two_levels" else (192, 224)
+    pat = synth.make_pattern(h, w, seed=7)
+    f = synth.stretch_translate(0.008, (2.3, -1.1), (w / 2, h / 2))
+    a = synth.as_float(synth.render(pat))
+    b = synth.as_float(synth.render(pat, f, noise=0.005, seed=3))
+    roi = synth.convex_polygon_mask(h, w, seed=7, coverage=0.8)
+    cfg = Config(**PAIR_VARIANTS[variant])
+    print(f"settings {variant}: {h}x{w}", flush=True)
+    r = eng.track_pair(a, b, cfg, mask=roi)
+    o = O.track_pair(f64(a), f64(b), cfg, mask=roi)
+    compare_pair(r, o, f"settings {variant}")
+
+
+def test_engine_errors_are_the_reference_messages(eng):
+    """Argument errors come back as the reference's messages, before any device work."""
+    a = np.zeros((64, 80), np.float32)
+    with pytest.raises(ValueError, match="ROI mask is empty"):
+        eng.track_pair(a, a, Config(), mask=np.zeros((64, 80), bool))
+    with pytest.raises(ValueError, match="image smaller than poc_win"):
+        eng.track_pair(a[:40], a[:40], Config())
+    with pytest.raises(ValueError, match="ROI mask shape"):
+        eng.track_pair(a, a, Config(), mask=np.ones((10, 10), bool))
+    with pytest.raises(ValueError, match="poc_win must be a power of two"):
+        eng.track_pair(a, a, Config(poc_win=48))
+    with pytest.raises(ValueError, match="no frames"):
+        list(eng.iter_sequence(iter([]), Config()))
