@@ -40,9 +40,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--workload", choices=("c2", "c4"), default="c2",
-                    help="c2: one 2048^2 pair (default); c4: 64 uneven pairs through the batched entry point")
+    ap.add_argument("--workload", choices=("c2", "c4", "c5"), default="c2",
+                    help="c2: one 2048^2 pair (default); c4: 64 uneven pairs through the batched entry point; "
+                         "c5: one 4000 x 3000 pair with displacements up to 40 px (plain POC-initialised method)")
     ap.add_argument("--lanes", type=int, default=4, help="engine lanes for --workload c4")
+    ap.add_argument("--tile", type=int, default=0,
+                    help="fine-stage tile size for c2/c5 (0: whole frame; 1024 is the reference CLI's tiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only copy MIB MiB device to device (twice) and exit: the reference kernel that scales "
                          "the HBM byte counters of a profiler run")
@@ -72,24 +75,40 @@ def main():
     from optflow_soil_amd import synth
     from optflow_soil_amd.config import Config
 
-    size = args.size
+    print(f"# rendering the {args.workload} pair", file=sys.stderr, flush=True)
     t = time.perf_counter()
-    a, b, roi, (tx, ty) = synth.workload_c2(size)
-    print(f"# rendered {size}x{size} pair in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
-    cfg = Config()
+    if args.workload == "c5":
+        h, w = 3000, 4000
+        a, b, (tx, ty) = synth.workload_c5()
+        roi = None
+        cfg = Config(max_disp=48.0, tile=args.tile)
+        metric = "Mpix/s flow+strain at 4000x3000, plain POC-initialised; EPE px vs truth"
+        desc = ("single pair 4000x3000 uint8 speckle, six Gaussian-bump fields up to 40 px, noise 0.01, "
+                "plain POC-initialised method, max_disp 48 (two POC levels), full frame, "
+                + (f"tiles of {args.tile}" if args.tile else "untiled"))
+    else:
+        h = w = args.size
+        a, b, roi, (tx, ty) = synth.workload_c2(args.size)
+        cfg = Config(tile=args.tile)
+        metric = METRIC
+        desc = (f"single pair {w}x{h} uint8 speckle, settlement bowl + 35 deg shear band, "
+                f"convex 8-gon ROI ~60 %, plain POC-initialised method, default settings, "
+                + (f"tiles of {args.tile}" if args.tile else "untiled"))
+    print(f"# rendered {w}x{h} pair in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     dev = torch.device("cuda:0")
     d_a = torch.from_numpy(synth.as_float(a)).to(dev)
     d_b = torch.from_numpy(synth.as_float(b)).to(dev)
-    d_roi = torch.from_numpy(roi.astype(np.uint8)).to(dev)
-    outs = {k: torch.empty((size, size), dtype=torch.float32, device=dev)
+    d_roi = None if roi is None else torch.from_numpy(roi.astype(np.uint8)).to(dev)
+    outs = {k: torch.empty((h, w), dtype=torch.float32, device=dev)
             for k in ("ux", "uy", "conf", "raw_ux", "raw_uy", "exx", "eyy", "exy", "gmax", "vol")}
-    outs["valid"] = torch.empty((size, size), dtype=torch.uint8, device=dev)
+    outs["valid"] = torch.empty((h, w), dtype=torch.uint8, device=dev)
     ptrs = {k: v.data_ptr() for k, v in outs.items()}
     eng = ofx.Engine(0)
-    eng.check_window(size, size, cfg)
+    eng.check_window(w, h, cfg)
 
     def step():
-        return eng.track_pair_device(d_a.data_ptr(), d_b.data_ptr(), size, size, cfg, ptrs, d_mask=d_roi.data_ptr())
+        return eng.track_pair_device(d_a.data_ptr(), d_b.data_ptr(), w, h, cfg, ptrs,
+                                     d_mask=0 if d_roi is None else d_roi.data_ptr())
 
     for i in range(args.warmup):
         step()
@@ -105,22 +124,25 @@ def main():
     prof = eng.profile_read()
     eng.profile(False)
     dt = (t1 - t0) / args.steps
-    value = size * size / 1e6 / dt
+    value = w * h / 1e6 / dt
 
     # accuracy of the last step against the synthetic truth (valid pixels inside the ROI)
     ux, uy = outs["ux"].cpu().numpy(), outs["uy"].cpu().numpy()
     fin = np.isfinite(ux) & np.isfinite(uy)
     epe = float(np.sqrt(np.mean((ux[fin] - tx[fin]) ** 2 + (uy[fin] - ty[fin]) ** 2))) if fin.any() else float("nan")
-    valid_frac = float(fin[roi].mean())
+    valid_frac = float(fin[roi].mean()) if roi is not None else float(fin.mean())
 
-    # roofline of the accumulate kernel (one launch per inner iteration)
+    # roofline of the accumulate kernel: one launch per inner iteration over the frame, or per tile
+    # and inner iteration when tiled (then the frame's bytes per iteration over the tiles' total time)
     acc_ms, acc_calls = prof["accumulate"]
     acc_avg_s = acc_ms / max(acc_calls, 1) * 1e-3
-    acc_bytes = accumulate_bytes(size, size, cfg.n_filters())
-    achieved = acc_bytes / acc_avg_s / 1e9
+    acc_bytes = accumulate_bytes(w, h, cfg.n_filters())
+    n_tiles = -(-w // cfg.tile) * -(-h // cfg.tile) if cfg.tile else 1
+    frame_passes = acc_calls / n_tiles
+    achieved = acc_bytes * frame_passes / (acc_ms * 1e-3) / 1e9 if acc_ms > 0 else float("nan")
     traffic = None
     tfile = os.path.join(ROOT, "profiles", "accumulate_traffic_2048.json")
-    if os.path.exists(tfile):
+    if args.workload == "c2" and args.size == 2048 and os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
     stages = {k: round(v[0] / args.steps, 3) for k, v in prof.items() if v[1]}
@@ -130,12 +152,10 @@ def main():
         cpu = cpu_baseline(a, b, roi, cfg)
 
     line = {
-        "metric": METRIC, "value": round(value, 3), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
+        "metric": metric, "value": round(value, 3), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(dt * 1e3, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"single pair {size}x{size} uint8 speckle, settlement bowl + 35 deg shear band, "
-                               f"convex 8-gon ROI ~60 %, plain POC-initialised method, default settings, untiled",
-                   "pixels_per_step": size * size},
+        "config": {"workload": desc, "pixels_per_step": w * h},
         "roofline": {"kernel": "k_accumulate", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": acc_bytes, "avg_launch_ms": round(acc_avg_s * 1e3, 4)},
@@ -200,12 +220,12 @@ def cpu_baseline(a, b, roi, cfg):
     cores = min(os.cpu_count() or 1, 16)
     oracle.set_threads(cores)
     c = 1024
-    y0 = (a.shape[0] - c) // 2
-    sl = np.s_[y0:y0 + c, y0:y0 + c]
+    y0, x0 = (a.shape[0] - c) // 2, (a.shape[1] - c) // 2
+    sl = np.s_[y0:y0 + c, x0:x0 + c]
     A = synth.as_float(a[sl]).astype(np.float64)
     Bm = synth.as_float(b[sl]).astype(np.float64)
     t = time.perf_counter()
-    oracle.track_pair(A, Bm, cfg, mask=roi[sl])
+    oracle.track_pair(A, Bm, cfg, mask=None if roi is None else roi[sl])
     dt = time.perf_counter() - t
     return {"value": round(c * c / 1e6 / dt, 4), "unit": "Mpix/s", "cores": cores, "kind": "port",
             "sample": f"central {c}x{c} crop of the benchmark pair (same ROI), one pair, {dt:.2f} s"}

@@ -204,3 +204,43 @@ def workload_c4(n_pairs=64, distinct_per_size=2, seed=0):
         h, w = C4_SIZES[si]
         out.append((a, b, convex_polygon_mask(h, w, seed=k, coverage=float(rng.uniform(0.4, 0.9)), search_step=8)))
     return out
+
+
+def gaussian_bumps(h, w, n=6, amp=40.0, scale=(150.0, 600.0), seed=0):
+    """A sum of n Gaussian-bump vector fields u = a exp(-|X - c|^2 / (2 s^2)) with centres uniform over
+    the frame, scales s uniform in `scale` and vectors a of random direction, rescaled together so
+    that the largest |u| over the frame is `amp` px."""
+    rng = np.random.default_rng(seed)
+    cx, cy = rng.uniform(0, w, n), rng.uniform(0, h, n)
+    s = rng.uniform(scale[0], scale[1], n)
+    ang = rng.uniform(0, 2 * np.pi, n)
+    mag = rng.uniform(0.3, 1.0, n)
+    ax, ay = mag * np.cos(ang), mag * np.sin(ang)
+
+    def raw(X, Y):
+        ux, uy = np.zeros_like(X), np.zeros_like(Y)
+        for k in range(n):
+            g = np.exp(-((X - cx[k]) ** 2 + (Y - cy[k]) ** 2) / (2.0 * s[k] ** 2))
+            ux += ax[k] * g
+            uy += ay[k] * g
+        return ux, uy
+    yy, xx = np.mgrid[0:h:4, 0:w:4].astype(np.float64)
+    ux, uy = raw(xx, yy)
+    k = amp / float(np.sqrt(ux ** 2 + uy ** 2).max())
+
+    def f(X, Y):
+        ux, uy = raw(X, Y)
+        return k * ux, k * uy
+    return f
+
+
+def workload_c5(seed=0):
+    """A 4000 x 3000 pair with large smooth motion: six Gaussian bumps, |u| up to 40 px, noise sigma
+    0.01, full frame. Returns (I1 u8, I2 u8, truth (ux, uy))."""
+    h, w = 3000, 4000
+    pat = make_pattern(h, w, seed=seed)
+    f = gaussian_bumps(h, w, seed=seed + 7)
+    a = render(pat, None, noise=0.01, seed=1)
+    b = render(pat, f, noise=0.01, seed=2)
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    return a, b, f(xx, yy)
