@@ -178,9 +178,14 @@ __device__ int find_root(int *parent, int x) {
         if (q == r) break;
         r = q;
     }
-    // path compression: every node on the path gets an ancestor (r, a root when read) as parent;
-    // roots are never written here, so the linking CAS of unite stays the only write to a root
-    while (x != r) {
+    // path compression: every node on the path above r gets r (a root when read) as parent. Every
+    // write keeps parent[x] < x for a non-root, so no chain can close into a cycle. The walk stops
+    // below r as well as at r: while it runs, r can be linked under a lower root and another
+    // thread can compress part of the path straight to that root, so the walk can pass under r;
+    // pointing a node there up at r would close a cycle (r -> ... -> node -> r) and every later
+    // find_root on that component would never end. Roots are never written here, so the linking
+    // CAS of unite stays the only write to a root.
+    while (x > r) {
         const int q = p[x];
         if (q == x) break;
         if (q != r) p[x] = r;

@@ -158,10 +158,6 @@ def test_sequence_chain_fixture(eng):
         assert rms(res.uy.astype(np.float64), load("seq_uy")[k]) <= DISP_TOL
 
 
-@pytest.mark.skip(reason="one GPU run of this test stopped making progress inside ofx_track_sequence and was "
-                         "ended by the test timeout, though the same code and inputs had passed in the runs "
-                         "before; the cause is not found, so it stays off the GPU (the chain path itself is "
-                         "covered by test_sequence_chain_fixture and the streaming test)")
 def test_c1_chain_workload(eng):
     """256 x 256 speckle, 5 frames, seq_mode chain, uint8 input read as v / 255."""
     from optflow_soil_amd import synth
@@ -182,9 +178,7 @@ def test_c1_chain_workload(eng):
 
 @pytest.mark.parametrize("name,fb_max", [
     ("A", 10),
-    pytest.param("B", 2, marks=pytest.mark.skip(
-        reason="the GPU run of this case (fb_max 2) stopped making progress and was killed; the cause is not "
-               "found yet, so it stays off the GPU (the oracle covers variant B on the CPU)")),
+    ("B", 2),
 ])
 def test_sequence_hybrid_fixtures(eng, name, fb_max):
     """seq_mode "hybrid" on the GPU against the reference's fixtures, with the bar of the
