@@ -90,25 +90,65 @@ struct LineArgs {
     double scale;
 };
 
-// Padded LDS line layout (lines.hpp) with SH = 4; the twiddle table exp(-2 pi i e / n) is staged
-// in LDS behind the two line buffers.
-// Three workgroups share a CU (LDS): the register budget is capped at three wavefronts per SIMD.
-template <int MODE, int SH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_lines(const float2 *__restrict__ in, float2 *__restrict__ out, PlanArgs P,
-                                               const float2 *__restrict__ tw_g, Bank bank, int k, LineArgs A) {
-    extern __shared__ float2 lds[];
-    const int n = P.n;
-    const int ln = lp_len<SH>(n);
-    float2 *a = lds, *b = lds + ln, *tw = lds + 2 * ln;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = tw_g[i];
+// The transform of one line whose element e is load(e), by direct Stockham passes over LDS buffers
+// a / b with the twiddle table tw; returns the buffer holding the result.
+template <int SH>
+struct DirectCore {
+    const PlanArgs &P;
+    const float2 *tw;
+    template <class L>
+    __device__ float2 *operator()(L load, float2 *a, float2 *b, int tid, int nt) const {
+        return fft_line_from<SH>(load, a, b, P.n, P.fac, P.magic, P.nfac, tw, tid, nt);
+    }
+};
+
+// Chirp-z (Bluestein) form for lengths with a large prime factor, whose direct pass would cost
+// n p operations per line: X_k = conj(b_k) sum_j (x_j conj(b_j)) b_{k-j} with b_j = exp(i pi j^2 / n)
+// is a cyclic convolution of length m >= 2n - 1, done with two length-m transforms:
+// A = FFT_m(x conj(b)), then conv = conj(FFT_m(conj(A C))) with C the length-m DFT of the wrapped
+// chirp divided by m, so X_k = conj(b_k FFT_m(conj(A C))_k). The tables live in global memory
+// (L2-resident); the LDS holds only the two line buffers of length m.
+struct BlueArgs {
+    PlanArgs M;            // the plan of length m
+    const float2 *tw;      // exp(-2 pi i e / m)
+    const float2 *chirp;   // b_j, j < n
+    const float2 *spec;    // C / m
+};
+
+template <int SH>
+struct BlueCore {
+    const BlueArgs &B;
+    int n;
+    template <class L>
+    __device__ float2 *operator()(L load, float2 *a, float2 *b, int tid, int nt) const {
+        const float2 *ch = B.chirp, *C = B.spec;
+        const int n_ = n;
+        float2 *r = fft_line_from<SH>(
+            [&](int e) { return e < n_ ? cmul(load(e), cconj(ch[e])) : make_float2(0.f, 0.f); }, a, b, B.M.n,
+            B.M.fac, B.M.magic, B.M.nfac, B.tw, tid, nt);
+        float2 *o = (r == a) ? b : a;
+        // the first pass reads r through the loader and writes o, so r is free for the later passes
+        float2 *r2 = fft_line_from<SH>([&](int e) { return cconj(cmul(r[lp<SH>(e)], C[e])); }, r, o, B.M.n,
+                                       B.M.fac, B.M.magic, B.M.nfac, B.tw, tid, nt);
+        for (int k = tid; k < n_; k += nt) r2[lp<SH>(k)] = cconj(cmul(ch[k], r2[lp<SH>(k)]));
+        __syncthreads();
+        return r2;
+    }
+};
+
+// One line per workgroup: load (with the mode's fused pre-processing), transform, post-process,
+// store. The LDS buffers a / b hold at least n elements each (padded layout, lines.hpp).
+template <int MODE, int SH, class Core>
+__device__ __forceinline__ void line_body(const float2 *__restrict__ in, float2 *__restrict__ out, int n,
+                                          const Bank &bank, int k, const LineArgs &A, float2 *a, float2 *b,
+                                          const Core &core) {
     const int line = blockIdx.x;
-    __syncthreads();
+    const int tid = threadIdx.x, nt = blockDim.x;
     float2 *r;
     if (MODE == MODE_FWD_REFLECT) {
         const float *row = A.img + (size_t)reflect_idx(line - PAD, A.h) * A.w;
         const int w = A.w;
-        r = fft_line_from<SH>([&](int x) { return make_float2(row[reflect_idx(x - PAD, w)], 0.f); }, a, b, n, P.fac,
-                              P.magic, P.nfac, tw, threadIdx.x, blockDim.x);
+        r = core([&](int x) { return make_float2(row[reflect_idx(x - PAD, w)], 0.f); }, a, b, tid, nt);
     } else if (MODE == MODE_INV_FIRST) {
         // G_k = (1 - exp(-|f|^2 s_hp^2 / 2)) exp(-|f - w0_k|^2 / (2 s_k^2)) (phaseflow.py:73-89)
         const float2 *src = in + (size_t)line * n;
@@ -118,7 +158,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
         const float dw = (float)(6.283185307179586 / (double)n);
         const float gx = ax * ax * i2s, hx = fx * fx * shp;
         const int mul = A.mul;
-        r = fft_line_from<SH>(
+        r = core(
             [&](int y) {
                 const float fy = freq_step(y, n, dw);
                 const float hp = one_minus_exp_neg(fmaf(fy * fy, shp, hx));
@@ -129,25 +169,73 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k
                 else if (mul == MUL_G_IFY) v = make_float2(-v.y * fy, v.x * fy);
                 return cconj(v);
             },
-            a, b, n, P.fac, P.magic, P.nfac, tw, threadIdx.x, blockDim.x);
+            a, b, tid, nt);
     } else {
         const float2 *src = in + (size_t)line * n;
-        r = fft_line_from<SH>([&](int x) { return src[x]; }, a, b, n, P.fac, P.magic, P.nfac, tw, threadIdx.x,
-                              blockDim.x);
+        r = core([&](int x) { return src[x]; }, a, b, tid, nt);
     }
     float2 *o = out + (size_t)line * n;
     if ((MODE == MODE_INV_FIRST || MODE == MODE_INV_SECOND) && A.prefilter) {
         float2 *sc = (r == a) ? b : a;
-        prefilter_lines<float2, SH, Sgl2>(r, sc, r, n, 1, MODE == MODE_INV_SECOND ? A.scale : 1.0, threadIdx.x,
-                                          blockDim.x);
-        for (int x = threadIdx.x; x < n; x += blockDim.x)
-            o[x] = MODE == MODE_INV_SECOND ? cconj(r[lp<SH>(x)]) : r[lp<SH>(x)];
+        prefilter_lines<float2, SH, Sgl2>(r, sc, r, n, 1, MODE == MODE_INV_SECOND ? A.scale : 1.0, tid, nt);
+        for (int x = tid; x < n; x += nt) o[x] = MODE == MODE_INV_SECOND ? cconj(r[lp<SH>(x)]) : r[lp<SH>(x)];
     } else if (MODE == MODE_INV_SECOND) {
         const float s = (float)A.scale;
-        for (int x = threadIdx.x; x < n; x += blockDim.x) o[x] = cconj(cscale(r[lp<SH>(x)], s));
+        for (int x = tid; x < n; x += nt) o[x] = cconj(cscale(r[lp<SH>(x)], s));
     } else {
-        for (int x = threadIdx.x; x < n; x += blockDim.x) o[x] = r[lp<SH>(x)];
+        for (int x = tid; x < n; x += nt) o[x] = r[lp<SH>(x)];
     }
+}
+
+// Direct passes: the twiddle table is staged in LDS behind the two line buffers. Three workgroups
+// share a CU (LDS): the register budget is capped at three wavefronts per SIMD.
+template <int MODE, int SH>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_lines(const float2 *__restrict__ in, float2 *__restrict__ out, PlanArgs P,
+                                               const float2 *__restrict__ tw_g, Bank bank, int k, LineArgs A) {
+    extern __shared__ float2 lds[];
+    const int n = P.n;
+    const int ln = lp_len<SH>(n);
+    float2 *a = lds, *b = lds + ln, *tw = lds + 2 * ln;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = tw_g[i];
+    __syncthreads();
+    line_body<MODE, SH>(in, out, n, bank, k, A, a, b, DirectCore<SH>{P, tw});
+}
+
+// The chirp-z form: the two line buffers of length m (m <= 8192: at most 136 KiB) leave room for
+// one workgroup per CU, so a workgroup of 1024 threads keeps 16 wavefronts on the CU.
+template <int MODE, int SH>
+__global__ void __launch_bounds__(1024) k_lines_blue(const float2 *__restrict__ in, float2 *__restrict__ out, int n,
+                                                     BlueArgs B, Bank bank, int k, LineArgs A) {
+    extern __shared__ float2 lds[];
+    const int lm = lp_len<SH>(B.M.n);
+    line_body<MODE, SH>(in, out, n, bank, k, A, lds, lds + lm, BlueCore<SH>{B, n});
+}
+
+// b_j = exp(i pi (j^2 mod 2n) / n) from the double angle (the product j^2 is exact in 64 bits)
+__global__ void k_chirp(float2 *__restrict__ b, int n) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const long long t = ((long long)j * j) % (2LL * n);
+    const double ph = M_PI * (double)t / (double)n;
+    b[j] = make_float2((float)cos(ph), (float)sin(ph));
+}
+
+// C_k = (1 / m) sum_j c_j exp(-2 pi i j k / m) for the wrapped chirp c (c_j = b_j for j < n,
+// c_{m-j} = b_j for 0 < j < n, 0 between), summed in double by direct evaluation (once per length)
+__global__ void k_chirp_spectrum(float2 *__restrict__ C, int n, int m) {
+    const int kk = blockIdx.x * blockDim.x + threadIdx.x;
+    if (kk >= m) return;
+    double sr = 0, si = 0;
+    for (int j = 0; j < m; j++) {
+        const int jj = j < n ? j : (j > m - n ? m - j : -1);
+        if (jj < 0) continue;
+        const long long t = ((long long)jj * jj) % (2LL * n);
+        const long long e = ((long long)j * kk) % m;
+        const double ph = M_PI * (double)t / (double)n - 2.0 * M_PI * (double)e / (double)m;
+        sr += cos(ph);
+        si += sin(ph);
+    }
+    C[kk] = make_float2((float)(sr / m), (float)(si / m));
 }
 
 // out[c][r] = in[r][c] for an R x C array, 32 x 32 tiles through LDS.
@@ -172,19 +260,22 @@ std::map<int, FftPlan> g_plans;
 
 }  // namespace
 
-const FftPlan &fft_plan(int n, hipStream_t s) {
-    std::lock_guard<std::mutex> lock(g_plan_mu);
+namespace {
+// Factors and twiddles of a length-n plan; the caller holds g_plan_mu.
+const FftPlan &plan_locked(int n, int max_n, hipStream_t s) {
     auto it = g_plans.find(n);
     if (it != g_plans.end()) return it->second;
-    if (n < 1 || n > 4096)
-        throw Error(OFX_ETOOLARGE, "FFT length " + std::to_string(n) + " exceeds the 4096 samples an LDS line holds");
+    if (n < 1 || n > max_n)
+        throw Error(OFX_ETOOLARGE, "FFT length " + std::to_string(n) + " exceeds the " + std::to_string(max_n) +
+                                       " samples an LDS line holds");
     FftPlan p;
     p.n = n;
-    int m = n;
+    int m = n, big = 1;
     auto push = [&](int r) {
         if (p.nfac >= MAX_FACTORS) throw Error(OFX_EUNSUPPORTED, "FFT length has too many factors");
         p.fac[p.nfac++] = r;
         m /= r;
+        big = std::max(big, r);
     };
     while (m % 8 == 0) push(8);
     while (m % 4 == 0) push(4);
@@ -199,9 +290,37 @@ const FftPlan &fft_plan(int n, hipStream_t s) {
     // can find the plan
     HIPCHECK(hipMalloc(&p.tw, n * sizeof(float2)));
     make_twiddles(p.tw, n, s);
+    if (big >= BLUE_MIN_PRIME) {
+        // the smallest 2^a 3^b >= 2n - 1 (at most 8192 for n <= 4096)
+        int best = 0;
+        for (long long p2 = 1; p2 <= 16384; p2 *= 2)
+            for (long long v = p2; v <= 16384; v *= 3)
+                if (v >= 2LL * n - 1 && (best == 0 || v < best)) best = (int)v;
+        plan_locked(best, 2 * 4096, s);
+        p.blue_m = best;
+        HIPCHECK(hipMalloc(&p.chirp, n * sizeof(float2)));
+        HIPCHECK(hipMalloc(&p.chirp_spec, best * sizeof(float2)));
+        hipLaunchKernelGGL(k_chirp, dim3((n + 255) / 256), dim3(256), 0, s, p.chirp, n);
+        hipLaunchKernelGGL(k_chirp_spectrum, dim3((best + 63) / 64), dim3(64), 0, s, p.chirp_spec, n, best);
+        LAUNCHCHECK();
+    }
     wait_stream(s);
     return g_plans[n] = p;
 }
+}  // namespace
+
+const FftPlan &fft_plan(int n, hipStream_t s) {
+    std::lock_guard<std::mutex> lock(g_plan_mu);
+    return plan_locked(n, 4096, s);
+}
+
+namespace {
+// The plan of a chirp-z convolution length (up to 8192; created with the plan that uses it)
+const FftPlan &fft_plan_m(int m, hipStream_t s) {
+    std::lock_guard<std::mutex> lock(g_plan_mu);
+    return plan_locked(m, 2 * 4096, s);
+}
+}  // namespace
 
 Bank make_bank(const ofx_config &c) {
     Bank b;
@@ -233,6 +352,21 @@ template <int MODE>
 void launch_lines(const float2 *in, float2 *out, int n, int lines, const Bank *bank, int k, const LineArgs &A,
                   hipStream_t s) {
     const FftPlan &P = fft_plan(n, s);
+    Bank dummy{};
+    if (P.blue_m) {
+        const FftPlan &M = fft_plan_m(P.blue_m, s);
+        const size_t lds = 2 * (size_t)lp_len<4>(P.blue_m) * sizeof(float2);
+        static std::once_flag once;
+        std::call_once(once, [] {
+            (void)hipFuncSetAttribute((const void *)k_lines_blue<MODE, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      160 * 1024);
+        });
+        const BlueArgs B{args_of(M), M.tw, P.chirp, P.chirp_spec};
+        hipLaunchKernelGGL((k_lines_blue<MODE, 4>), dim3(lines), dim3(1024), lds, s, in, out, n, B,
+                           bank ? *bank : dummy, k, A);
+        LAUNCHCHECK();
+        return;
+    }
     const size_t lds = line_lds_bytes(n);
     if (lds > 65536) {
         static std::once_flag once;
@@ -241,7 +375,6 @@ void launch_lines(const float2 *in, float2 *out, int n, int lines, const Bank *b
                                       160 * 1024);
         });
     }
-    Bank dummy{};
     hipLaunchKernelGGL((k_lines<MODE, 4>), dim3(lines), dim3(256), lds, s, in, out, args_of(P), P.tw,
                        bank ? *bank : dummy, k, A);
     LAUNCHCHECK();
@@ -278,7 +411,11 @@ void fft_inverse_filtered(const float2 *spec, int hp, int wp, const Bank &bank, 
 
 void fft_release() {
     std::lock_guard<std::mutex> lock(g_plan_mu);
-    for (auto &kv : g_plans) (void)hipFree(kv.second.tw);
+    for (auto &kv : g_plans) {
+        (void)hipFree(kv.second.tw);
+        (void)hipFree(kv.second.chirp);
+        (void)hipFree(kv.second.chirp_spec);
+    }
     g_plans.clear();
 }
 

@@ -99,7 +99,13 @@ struct FftPlan {
     int nfac = 0;
     int fac[MAX_FACTORS] = {0};
     float2 *tw = nullptr;  // device: exp(-2 pi i e / n), e in [0, n)
+    // Lengths with a prime factor of at least BLUE_MIN_PRIME are transformed as a chirp-z
+    // convolution of length blue_m (0: direct passes).
+    int blue_m = 0;
+    float2 *chirp = nullptr;  // device: exp(i pi j^2 / n), j in [0, n)
+    float2 *chirp_spec = nullptr;  // device: length-blue_m DFT of the wrapped chirp, divided by blue_m
 };
+constexpr int BLUE_MIN_PRIME = 67;
 
 const FftPlan &fft_plan(int n, hipStream_t s);  // cached per length, device tables never freed
 void fft_release();

@@ -135,6 +135,22 @@ def test_pair_where_a_block_appears(eng):
     assert e["raw_ux"] <= 1e-4 and e["raw_uy"] <= 1e-4 and e["raw_max"] <= 1e-2
 
 
+@pytest.mark.parametrize("h,w", [(131, 163), (99, 70)])
+def test_pair_with_chirp_z_lengths(eng, h, w):
+    """Frame sizes whose padded lengths (side + 64) have a prime factor of 67 or more go through the
+    chirp-z form of the line transform: 163 + 64 = 227 and 99 + 64 = 163 are prime, 70 + 64 =
+    2 x 67. GPU against the oracle (whose transforms are direct double-precision DFTs)."""
+    from optflow_soil_amd import synth
+    pat = synth.make_pattern(h, w, seed=h + w)
+    f = synth.affine_field(0.004, -0.003, 0.002, 0.3, (1.7, -0.9), ((w - 1) / 2.0, (h - 1) / 2.0))
+    a = synth.as_float(synth.render(pat, None, noise=0.005, seed=1))
+    b = synth.as_float(synth.render(pat, f, noise=0.005, seed=2))
+    print(f"chirp-z pair {w}x{h}", flush=True)
+    r = eng.track_pair(a, b, Config())
+    o = O.track_pair(f64(a), f64(b), Config())
+    compare_pair(r, o, f"chirp-z pair {w}x{h}")
+
+
 def test_pair_fixture(eng):
     a, b, roi = load("pair_a"), load("pair_b"), load("pair_roi") > 0.5
     r = eng.track_pair(f32(a), f32(b), Config(), mask=roi)
