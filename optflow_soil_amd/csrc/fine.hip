@@ -195,6 +195,17 @@ __global__ void k_refpack(const float2 *__restrict__ r, const float2 *__restrict
 constexpr int ACC_T = 16;
 constexpr int ACC_BOX = 1600;  // nodes per staged box (40 x 40)
 
+// Workgroups are dealt to the chip's 8 XCDs in turn (dispatch index L runs on XCD L mod 8) and
+// each XCD has its own L2. Each row of tiles is split into 8 chunks of c = ceil(gx / 8) adjacent
+// tiles, chunk x going to XCD x: neighbouring tiles, whose staged boxes overlap and share cache
+// lines, meet in the same L2, while all XCDs still walk the same rows of the frame together.
+// The grid has 8 c slots per row; slots past the last column return at once.
+__device__ __forceinline__ void xcd_tile(int L, int gx, int &tx, int &ty) {
+    const int x = L & 7, s = L >> 3, c = (gx + 7) >> 3;
+    ty = s / c;
+    tx = x * c + (s - ty * c);
+}
+
 __global__ void __launch_bounds__(256) k_accumulate_tiled(const double *__restrict__ ux, const double *__restrict__ uy,
                                                           const float *__restrict__ sx, const float *__restrict__ sy,
                                                           const float *__restrict__ csx, const float *__restrict__ csy,
@@ -205,7 +216,10 @@ __global__ void __launch_bounds__(256) k_accumulate_tiled(const double *__restri
     __shared__ int red[4][8];
     __shared__ int geom[4];
     const int lx = threadIdx.x & (ACC_T - 1), ly = threadIdx.x >> 4;
-    const int x = blockIdx.x * ACC_T + lx, y = blockIdx.y * ACC_T + ly;
+    int tx, ty;
+    xcd_tile(blockIdx.x, (w + ACC_T - 1) / ACC_T, tx, ty);
+    if (tx * ACC_T >= w) return;  // a padding slot: the whole workgroup leaves before any barrier
+    const int x = tx * ACC_T + lx, y = ty * ACC_T + ly;
     const bool inside = x < w && y < h;
     const size_t n = (size_t)w * h, np = (size_t)wp * hp;
     const size_t i = inside ? (size_t)y * w + x : 0;
@@ -1453,7 +1467,8 @@ void accumulate(const double *ux, const double *uy, const float *sx, const float
                 const float *csy, const float2 *resp, const float4 *refp, double *sums, int w, int h, int F,
                 hipStream_t s) {
     const int wp = w + 2 * PAD, hp = h + 2 * PAD;
-    hipLaunchKernelGGL(k_accumulate_tiled, dim3((w + ACC_T - 1) / ACC_T, (h + ACC_T - 1) / ACC_T), dim3(256), 0, s,
+    const int gx = (w + ACC_T - 1) / ACC_T, gy = (h + ACC_T - 1) / ACC_T;
+    hipLaunchKernelGGL(k_accumulate_tiled, dim3(8 * ((gx + 7) / 8) * gy), dim3(256), 0, s,
                        ux, uy, sx, sy, csx, csy, resp, refp, sums, w, h, wp, hp, F);
     LAUNCHCHECK();
 }
