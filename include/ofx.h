@@ -190,6 +190,12 @@ int ofx_track_sequence(ofx_engine *e, const float *frames, int n, int w, int h,
 int ofx_tavg_window(ofx_engine *e, const float *frames, int nf, int centre, int w, int h, int align,
                     const uint8_t *mask, int keep_unaligned, const ofx_config *cfg, float *out, int *n_used);
 
+/* numpy.median of n doubles (host array), as the engine takes it of the per-pixel energy every
+ * inner iteration (rust/crates/optflow-core/src/select.rs:19-42): NaN if any value is NaN or n is
+ * 0; for an even n the mean of the two middle order statistics, the sign of zero kept as a sort by
+ * total order keeps it. */
+int ofx_median(ofx_engine *e, const double *values, size_t n, double *out);
+
 /* Frame conversion (rust/crates/optflow-cli/src/io.rs:44-52, :259-281; python/optflow_ref/io.py:15-54):
  * pixels (h x w x channels, channels 1, 3 or 4, 8 or 16 bit, host) -> grey levels in [0, 1]
  * (v / 255 or v / 65535; colour 0.299 R + 0.587 G + 0.114 B), box downscaled by factor (the

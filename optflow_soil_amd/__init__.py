@@ -22,7 +22,7 @@ OFX_OK, OFX_EINVAL, OFX_ETOOLARGE, OFX_EDEVICE, OFX_ENODEVICE, OFX_EBUSY, OFX_EU
 SYMBOLS = ("ofx_config_default", "ofx_create", "ofx_destroy", "ofx_last_error", "ofx_lanes", "ofx_returns_slope",
            "ofx_check_window", "ofx_release_caches", "ofx_refine_raw", "ofx_coarse_flow", "ofx_track_pair",
            "ofx_track_pair_device", "ofx_track_pairs", "ofx_seq_begin", "ofx_seq_step", "ofx_seq_step_device", "ofx_seq_end",
-           "ofx_tavg_window", "ofx_frame_to_gray",
+           "ofx_tavg_window", "ofx_frame_to_gray", "ofx_median",
            "ofx_track_sequence", "ofx_profile", "ofx_profile_read", "ofx_profile_stage")
 
 
@@ -337,6 +337,15 @@ class Engine:
         _check(lib().ofx_frame_to_gray(self._h, a.ctypes.data_as(ctypes.c_void_p), 8 * a.itemsize, ch, w, h,
                                        int(factor), _p(out)))
         return out
+
+    def median(self, values):
+        """select.rs:19-42 on the GPU: numpy.median of a float64 array (NaN if any value is NaN or
+        the array is empty)."""
+        a = np.ascontiguousarray(values, dtype=np.float64).ravel()
+        out = ctypes.c_double(0.0)
+        _check(lib().ofx_median(self._h, a.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(a.size),
+                                ctypes.byref(out)))
+        return out.value
 
     def sequence(self, frame0, cfg, mask=None):
         """Opens a streaming sequence on frame 0 (the Sequence iterator of track.rs:1112-1225): a

@@ -1571,6 +1571,25 @@ int ofx_frame_to_gray(ofx_engine *e, const void *pixels, int bits, int channels,
     });
 }
 
+int ofx_median(ofx_engine *e, const double *values, size_t n, double *out) {
+    return guarded([&] {
+        if (!out || (n && !values)) throw Error(OFX_EINVAL, "values or out is NULL");
+        if (n == 0) {
+            *out = NAN;  // numpy.median of nothing
+            return;
+        }
+        LaneGuard g(e);
+        Workspace &ws = *g.lane->ws;
+        hipStream_t s = g.lane->stream;
+        double *dv = B<double>(ws.benergy, n);
+        double *med = B<double>(ws.bmed, 4);
+        copy_h2d(ws.pin, dv, values, n * sizeof(double), s);
+        median(dv, n, ws.bmedscr.get<char>(median_scratch_bytes()), med, s);
+        copy_d2h(ws.pin, out, med, sizeof(double), s);
+        wait_stream(s);
+    });
+}
+
 int ofx_track_pair_device(ofx_engine *e, const float *d_i1, const float *d_i2, int w, int h, const uint8_t *d_mask,
                           const float *d_init_ux, const float *d_init_uy, const ofx_config *cfg, ofx_result *d_out) {
     return guarded([&] {

@@ -97,16 +97,6 @@ __global__ void k_blur_y(const double *__restrict__ in, double *__restrict__ out
     out[(size_t)y * w + x] = s;
 }
 
-__global__ void k_blur_x(const double *__restrict__ in, float *__restrict__ out, int w, const BT bt, int r) {
-    const double *k = bt.k;
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x >= w) return;
-    const double *row = in + (size_t)y * w;
-    double s = 0;
-    for (int j = -r; j <= r; j++) s += k[j + r] * row[min(max(x + j, 0), w - 1)];
-    out[(size_t)y * w + x] = (float)s;
-}
-
 // Register-blocked y pass for a compile-time radius R (sigma_s = 4 gives R = 16): one thread per
 // (column, BLUR_ROWS output rows); each of the BLUR_ROWS + 2R input rows is read once and added
 // into every output row it reaches, taps in the order of the sequential sum.
@@ -1279,58 +1269,103 @@ __global__ void __launch_bounds__(1024) k_msel_final(const double *__restrict__ 
 // Radix select, 8 bits per pass (the form every 2048^2 benchmark run completed with): histograms
 // privatised in LDS; state: prefix (known high bits), rank still to find, NaN count.
 struct SelectState {
-    unsigned long long prefix;
-    unsigned long long rank;
+    unsigned long long prefix[9];  // key digits fixed before pass p (prefix[8]: the selected key)
+    unsigned long long rank[9];    // rank left within them
     unsigned int nan_count;
     unsigned int pad;
     unsigned long long below_count;  // elements with key < the selected key
     unsigned long long below_max;    // largest key below it
+    unsigned int hist[8][256];       // one 8-bit digit histogram per pass
 };
 
-__global__ void k_select_init(SelectState *st, unsigned int *hist, unsigned long long rank) {
-    st->prefix = 0;
-    st->rank = rank;
-    st->nan_count = 0;
-    st->below_count = 0;
-    st->below_max = 0;
-    for (int i = threadIdx.x; i < 256; i += blockDim.x) hist[i] = 0;
+__global__ void k_select_init(SelectState *st, unsigned long long rank) {
+    if (threadIdx.x == 0) {
+        st->prefix[0] = 0;
+        st->rank[0] = rank;
+        st->nan_count = 0;
+        st->below_count = 0;
+        st->below_max = 0;
+    }
+    for (int i = threadIdx.x; i < 8 * 256; i += blockDim.x) st->hist[i >> 8][i & 255] = 0;
 }
 
-__global__ void __launch_bounds__(256) k_select_hist(const double *__restrict__ v, size_t n, int shift,
-                                                     SelectState *st, unsigned int *hist) {
+// The digit of pass p - 1 and the rank left after it, from that pass's histogram: the smallest
+// digit d whose cumulative count exceeds the rank (255 if none does). Computed by one whole
+// wavefront (every lane gets the result): 4 bins per lane, a shuffle scan over the lanes.
+__device__ __forceinline__ void select_pick(const SelectState *st, int p, unsigned long long &prefix,
+                                            unsigned long long &rank) {
+    const int lane = threadIdx.x & 63;
+    const unsigned long long r = st->rank[p - 1];
+    const unsigned int *h = st->hist[p - 1];
+    const unsigned long long c0 = h[4 * lane], c1 = h[4 * lane + 1], c2 = h[4 * lane + 2], c3 = h[4 * lane + 3];
+    unsigned long long incl = c0 + c1 + c2 + c3;
+    for (int off = 1; off < 64; off <<= 1) {
+        const unsigned long long o = __shfl_up(incl, off);
+        if (lane >= off) incl += o;
+    }
+    const unsigned long long excl = incl - (c0 + c1 + c2 + c3);
+    const unsigned long long hit = __ballot(incl > r);
+    int d = 255;
+    unsigned long long below = 0;
+    if (hit) {
+        const int l = __ffsll((long long)hit) - 1;
+        // inside lane l's four bins
+        unsigned long long cum = __shfl(excl, l);
+        const unsigned long long b0 = __shfl(c0, l), b1 = __shfl(c1, l), b2 = __shfl(c2, l);
+        int j = 0;
+        if (cum + b0 > r) j = 0;
+        else if (cum + b0 + b1 > r) { j = 1; cum += b0; }
+        else if (cum + b0 + b1 + b2 > r) { j = 2; cum += b0 + b1; }
+        else { j = 3; cum += b0 + b1 + b2; }
+        d = 4 * l + j;
+        below = cum;
+    } else {
+        below = __shfl(incl, 63);  // no bin reaches the rank: digit 255, every count below it
+    }
+    const int shift = 64 - 8 * p;
+    prefix = st->prefix[p - 1] | ((unsigned long long)d << shift);
+    rank = r - below;
+}
+
+// Pass p (digit bits 63 - 8p .. 56 - 8p): every workgroup first derives the digit of pass p - 1
+// from its histogram (block 0 records it), then histograms the keys that share the prefix.
+__global__ void __launch_bounds__(256) k_select_hist(const double *__restrict__ v, size_t n, int p,
+                                                     SelectState *st) {
     __shared__ unsigned int lh[256];
     lh[threadIdx.x] = 0;
+    unsigned long long prefix = 0;
+    if (p > 0) {
+        unsigned long long rank;
+        select_pick(st, p, prefix, rank);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            st->prefix[p] = prefix;
+            st->rank[p] = rank;
+        }
+    }
     __syncthreads();
-    const unsigned long long prefix = st->prefix;
+    const int shift = 56 - 8 * p;
     const int hi = shift + 8;  // bits above the current digit must equal the prefix
     unsigned int nan_local = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const double x = v[i];
-        if (shift == 56 && isnan(x)) nan_local++;
+        if (p == 0 && isnan(x)) nan_local++;
         const unsigned long long k = order_key(x);
         if (hi >= 64 || (k >> hi) == (prefix >> hi)) atomicAdd(&lh[(k >> shift) & 255], 1u);
     }
     if (nan_local) atomicAdd(&st->nan_count, nan_local);
     __syncthreads();
-    if (lh[threadIdx.x]) atomicAdd(&hist[threadIdx.x], lh[threadIdx.x]);
+    if (lh[threadIdx.x]) atomicAdd(&st->hist[p][threadIdx.x], lh[threadIdx.x]);
 }
 
-__global__ void k_select_pick(SelectState *st, unsigned int *hist, int shift) {
-    if (threadIdx.x != 0) return;
-    unsigned long long rank = st->rank, cum = 0;
-    int d = 0;
-    for (; d < 256; d++) {
-        if (cum + hist[d] > rank) break;
-        cum += hist[d];
-    }
-    if (d == 256) d = 255;
-    st->prefix |= (unsigned long long)d << shift;
-    st->rank = rank - cum;
-    for (int i = 0; i < 256; i++) hist[i] = 0;
+// The selected key, from the last histogram (the first wavefront of workgroup 0 records it)
+__device__ __forceinline__ unsigned long long select_key(SelectState *st) {
+    unsigned long long key, rank;
+    select_pick(st, 8, key, rank);
+    return key;
 }
 
 __global__ void __launch_bounds__(256) k_select_below(const double *__restrict__ v, size_t n, SelectState *st) {
-    const unsigned long long key = st->prefix;
+    const unsigned long long key = select_key(st);
     unsigned long long cnt = 0, mx = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
         const unsigned long long k = order_key(v[i]);
@@ -1361,13 +1396,15 @@ __global__ void __launch_bounds__(256) k_select_below(const double *__restrict__
     }
 }
 
-__global__ void k_select_finish(const SelectState *st, unsigned long long k, int even, double *out) {
+__global__ void k_select_finish(SelectState *st, unsigned long long k, int even, double *out) {
+    const unsigned long long key = select_key(st);
     if (threadIdx.x != 0) return;
+    st->prefix[8] = key;
     if (st->nan_count) {
         out[0] = NAN;
         return;
     }
-    const double upper = from_key(st->prefix);
+    const double upper = from_key(key);
     if (!even) {
         out[0] = upper;
         return;
@@ -1409,10 +1446,9 @@ __global__ void k_confidence(const double *__restrict__ energy, const double *__
 // ------------------------------------------------------------------ host wrappers
 static dim3 grid2(int w, int h) { return dim3((w + 255) / 256, h); }
 
-// The kernels added after the last completed 2048^2 benchmark run (the histogram-compaction median
-// and the register-blocked blur) run only with OFX_STAGED_KERNELS=1: a benchmark process that ran
-// them stopped making progress and the cause is not found yet (their results match the earlier
-// kernels in every parity test).
+// OFX_STAGED_KERNELS=1 selects the histogram-compaction median (12- and 10-bit histograms, a
+// compaction, a one-workgroup select) in place of the 8-bit radix select. Both give np.median
+// exactly; the radix select is the faster at the frame sizes measured.
 bool staged_kernels() {
     static const bool on = [] {
         const char *v = std::getenv("OFX_STAGED_KERNELS");
@@ -1437,17 +1473,12 @@ void blur(const double *in, double *tmp, float *out, int w, int h, const double 
     if (r > 64) throw Error(OFX_EUNSUPPORTED, "sigma_s above 16 px is not supported by the GPU blur");
     BT bt;
     for (int j = 0; j <= 2 * r; j++) bt.k[j] = taps_host[j];
-    if (staged_kernels()) {
-        if (r == 16)
-            hipLaunchKernelGGL(k_blur_y_rb<16>, dim3((w + 255) / 256, (h + BLUR_ROWS - 1) / BLUR_ROWS), dim3(256), 0,
-                               s, in, tmp, w, h, bt);
-        else
-            hipLaunchKernelGGL(k_blur_y, grid2(w, h), dim3(256), 0, s, in, tmp, w, h, bt, r);
-        hipLaunchKernelGGL(k_blur_x_lds, grid2(w, h), dim3(256), 0, s, tmp, out, w, bt, r);
-    } else {
+    if (r == 16)
+        hipLaunchKernelGGL(k_blur_y_rb<16>, dim3((w + 255) / 256, (h + BLUR_ROWS - 1) / BLUR_ROWS), dim3(256), 0, s,
+                           in, tmp, w, h, bt);
+    else
         hipLaunchKernelGGL(k_blur_y, grid2(w, h), dim3(256), 0, s, in, tmp, w, h, bt, r);
-        hipLaunchKernelGGL(k_blur_x, grid2(w, h), dim3(256), 0, s, tmp, out, w, bt, r);
-    }
+    hipLaunchKernelGGL(k_blur_x_lds, grid2(w, h), dim3(256), 0, s, tmp, out, w, bt, r);
     LAUNCHCHECK();
 }
 
@@ -1524,7 +1555,7 @@ void energy_reset(const double *energy, const double *med, const double *ux_in, 
     LAUNCHCHECK();
 }
 
-size_t median_scratch_bytes() { return std::max(sizeof(MedState), sizeof(SelectState) + 256 * sizeof(unsigned int)); }
+size_t median_scratch_bytes() { return std::max(sizeof(MedState), sizeof(SelectState)); }
 
 
 // numpy.median of n values (NaN if any is NaN), for an even n the mean of the n/2 - 1 and n/2 order
@@ -1546,13 +1577,9 @@ void median(const double *v, size_t n, void *scratch, double *out, hipStream_t s
         return;
     }
     SelectState *st = (SelectState *)scratch;
-    unsigned int *hist = (unsigned int *)((char *)scratch + sizeof(SelectState));
-    hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, hist, k);
+    hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, k);
     const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
-    for (int shift = 56; shift >= 0; shift -= 8) {
-        hipLaunchKernelGGL(k_select_hist, dim3(blocks), dim3(256), 0, s, v, n, shift, st, hist);
-        hipLaunchKernelGGL(k_select_pick, dim3(1), dim3(64), 0, s, st, hist, shift);
-    }
+    for (int p = 0; p < 8; p++) hipLaunchKernelGGL(k_select_hist, dim3(blocks), dim3(256), 0, s, v, n, p, st);
     if (even) hipLaunchKernelGGL(k_select_below, dim3(blocks), dim3(256), 0, s, v, n, st);
     hipLaunchKernelGGL(k_select_finish, dim3(1), dim3(64), 0, s, st, k, even, out);
     LAUNCHCHECK();
