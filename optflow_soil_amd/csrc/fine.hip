@@ -1024,21 +1024,8 @@ __global__ void __launch_bounds__(64) k_solve2(const double *__restrict__ mom, c
 }
 
 // ------------------------------------------------------------------ exact median (np.median)
-// Selection over the order-preserving 64-bit keys of the doubles: histograms of the top 12 and the
-// next 10 key bits (privatised in LDS), a compaction of the keys that share the selected 22-bit
-// prefix, and a one-workgroup select among those.
-constexpr int MS_B0 = 12, MS_B1 = 10;
-constexpr unsigned int MS_CAP = 8192;
-
-struct MedState {
-    unsigned int hist0[1 << MS_B0];  // digit 0: key bits 63 .. 52 (sign and exponent)
-    unsigned int hist1[1 << MS_B1];  // digit 1: key bits 51 .. 42, within the selected digit 0
-    unsigned int nan_count;
-    unsigned int cand;               // candidates: keys sharing the selected 22-bit prefix
-    unsigned long long below_max;    // largest key below that prefix (0: none)
-    unsigned long long cand_keys[MS_CAP];
-};
-
+// Order-preserving 64-bit keys of the doubles (a sort by the keys is a sort by total order: -0.0
+// before 0.0, NaN of either sign at the ends).
 __device__ __forceinline__ unsigned long long order_key(double v) {
     unsigned long long b = (unsigned long long)__double_as_longlong(v);
     return (b >> 63) ? ~b : (b | (1ull << 63));
@@ -1048,226 +1035,8 @@ __device__ __forceinline__ double from_key(unsigned long long k) {
     return __longlong_as_double((long long)b);
 }
 
-__global__ void k_msel_init(MedState *st) {
-    for (int i = threadIdx.x; i < (1 << MS_B0); i += blockDim.x) st->hist0[i] = 0;
-    for (int i = threadIdx.x; i < (1 << MS_B1); i += blockDim.x) st->hist1[i] = 0;
-    if (threadIdx.x == 0) {
-        st->nan_count = 0;
-        st->cand = 0;
-        st->below_max = 0;
-    }
-}
-
-// The bin of a global histogram (nb bins, a multiple of the workgroup size, which is at most
-// 1024) holding rank `rank` and the rank left inside it, found by every thread of the workgroup:
-// per-thread partial sums in LDS, then one thread walks them. Every thread must call it.
-__device__ void find_bin(const unsigned int *hist, int nb, unsigned long long rank, int *bin_out,
-                         unsigned long long *rank_out) {
-    __shared__ unsigned long long part[1024];
-    __shared__ int sel_bin;
-    __shared__ unsigned long long sel_rank;
-    const int nt = blockDim.x, per = nb / nt;
-    unsigned long long p = 0;
-    for (int j = 0; j < per; j++) p += hist[threadIdx.x * per + j];
-    part[threadIdx.x] = p;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long cum = 0;
-        int t = 0;
-        for (; t < nt - 1; t++) {
-            if (cum + part[t] > rank) break;
-            cum += part[t];
-        }
-        int b = t * per;
-        for (int j = 0; j < per - 1; j++, b++) {
-            const unsigned int c = hist[b];
-            if (cum + c > rank) break;
-            cum += c;
-        }
-        sel_bin = b;
-        sel_rank = rank - cum;
-    }
-    __syncthreads();
-    *bin_out = sel_bin;
-    *rank_out = sel_rank;
-}
-
-// digit 0 histogram (and the NaN count)
-__global__ void __launch_bounds__(256) k_msel_hist0(const double *__restrict__ v, size_t n, MedState *st) {
-    __shared__ unsigned int lh[1 << MS_B0];
-    for (int i = threadIdx.x; i < (1 << MS_B0); i += blockDim.x) lh[i] = 0;
-    __syncthreads();
-    unsigned int nan_local = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const double x = v[i];
-        nan_local += isnan(x) ? 1u : 0u;
-        atomicAdd(&lh[order_key(x) >> 52], 1u);
-    }
-    if (nan_local) atomicAdd(&st->nan_count, nan_local);
-    __syncthreads();
-    for (int i = threadIdx.x; i < (1 << MS_B0); i += blockDim.x)
-        if (lh[i]) atomicAdd(&st->hist0[i], lh[i]);
-}
-
-// digit 1 histogram of the keys in the selected digit-0 bin
-__global__ void __launch_bounds__(256) k_msel_hist1(const double *__restrict__ v, size_t n, unsigned long long k,
-                                                    MedState *st) {
-    __shared__ unsigned int lh[1 << MS_B1];
-    int b0;
-    unsigned long long r0;
-    find_bin(st->hist0, 1 << MS_B0, k, &b0, &r0);
-    for (int i = threadIdx.x; i < (1 << MS_B1); i += blockDim.x) lh[i] = 0;
-    __syncthreads();
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const unsigned long long key = order_key(v[i]);
-        if ((int)(key >> 52) == b0) atomicAdd(&lh[(key >> 42) & ((1 << MS_B1) - 1)], 1u);
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < (1 << MS_B1); i += blockDim.x)
-        if (lh[i]) atomicAdd(&st->hist1[i], lh[i]);
-}
-
-// the keys sharing the selected 22-bit prefix go to the candidate list (at most MS_CAP kept, all
-// counted); the largest key below the prefix is kept for the even-count median
-__global__ void __launch_bounds__(256) k_msel_compact(const double *__restrict__ v, size_t n, unsigned long long k,
-                                                      MedState *st) {
-    int b0, b1;
-    unsigned long long r0, r1;
-    find_bin(st->hist0, 1 << MS_B0, k, &b0, &r0);
-    find_bin(st->hist1, 1 << MS_B1, r0, &b1, &r1);
-    const unsigned long long prefix = ((unsigned long long)b0 << MS_B1) | (unsigned long long)b1;
-    unsigned long long mx = 0;
-    const int lane = threadIdx.x & 63;
-    for (size_t base = (size_t)blockIdx.x * blockDim.x; base < n; base += (size_t)gridDim.x * blockDim.x) {
-        const size_t i = base + threadIdx.x;
-        unsigned long long key = 0;
-        bool cand = false;
-        if (i < n) {
-            key = order_key(v[i]);
-            const unsigned long long p = key >> 42;
-            cand = p == prefix;
-            if (p < prefix && key > mx) mx = key;
-        }
-        const unsigned long long ball = __ballot(cand);
-        if (ball) {
-            unsigned int slot = 0;
-            if (lane == 0) slot = atomicAdd(&st->cand, (unsigned int)__popcll(ball));
-            slot = __shfl(slot, 0);
-            if (cand) {
-                const unsigned int at = slot + (unsigned int)__popcll(ball & ((1ull << lane) - 1ull));
-                if (at < MS_CAP) st->cand_keys[at] = key;
-            }
-        }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_xor(mx, off);
-        mx = o > mx ? o : mx;
-    }
-    if (lane == 0 && mx) atomicMax(&st->below_max, mx);
-}
-
-// Final selection by one workgroup of 1024 threads: the key of rank r1 among the candidates (or,
-// if more than MS_CAP share the prefix, among the keys of the whole array with that prefix),
-// 7-bit digits over the 42 remaining bits with an LDS histogram; then the (k-1)-th for an even
-// count: the largest key below it if exactly r1 candidates lie below, else the key itself.
-__global__ void __launch_bounds__(1024) k_msel_final(const double *__restrict__ v, size_t n, unsigned long long k,
-                                                     int even, MedState *st, double *out) {
-    __shared__ unsigned int lh[128];
-    __shared__ unsigned long long s_prefix, s_rank;
-    __shared__ unsigned long long red_c[16], red_m[16];
-    int b0, b1;
-    unsigned long long r0, r1;
-    __shared__ int s_b0, s_b1;
-    __shared__ unsigned long long s_r1;
-    find_bin(st->hist0, 1 << MS_B0, k, &b0, &r0);
-    find_bin(st->hist1, 1 << MS_B1, r0, &b1, &r1);
-    if (threadIdx.x == 0) {
-        s_b0 = b0;
-        s_b1 = b1;
-        s_r1 = r1;
-    }
-    __syncthreads();
-    if (st->nan_count) {
-        if (threadIdx.x == 0) out[0] = NAN;
-        return;
-    }
-    const unsigned long long prefix22 = ((unsigned long long)s_b0 << MS_B1) | (unsigned long long)s_b1;
-    const unsigned int m = st->cand;
-    const bool listed = m <= MS_CAP;
-    const size_t count = listed ? m : n;
-    auto key_at = [&](size_t i, bool &ok) {
-        unsigned long long key = listed ? st->cand_keys[i] : order_key(v[i]);
-        ok = (key >> 42) == prefix22;
-        return key;
-    };
-    if (threadIdx.x == 0) {
-        s_prefix = prefix22 << 42;
-        s_rank = s_r1;
-    }
-    __syncthreads();
-    for (int shift = 35; shift >= 0; shift -= 7) {
-        if (threadIdx.x < 128) lh[threadIdx.x] = 0;
-        __syncthreads();
-        const unsigned long long pre = s_prefix;
-        const int hi = shift + 7;
-        for (size_t i = threadIdx.x; i < count; i += blockDim.x) {
-            bool ok;
-            const unsigned long long key = key_at(i, ok);
-            if (ok && (key >> hi) == (pre >> hi)) atomicAdd(&lh[(key >> shift) & 127], 1u);
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            unsigned long long cum = 0, rank = s_rank;
-            int d = 0;
-            for (; d < 127; d++) {
-                if (cum + lh[d] > rank) break;
-                cum += lh[d];
-            }
-            s_prefix = pre | ((unsigned long long)d << shift);
-            s_rank = rank - cum;
-        }
-        __syncthreads();
-    }
-    const unsigned long long K = s_prefix;
-    if (!even) {
-        if (threadIdx.x == 0) out[0] = from_key(K);
-        return;
-    }
-    // candidates below K: their count and largest key
-    unsigned long long c = 0, mx = 0;
-    for (size_t i = threadIdx.x; i < count; i += blockDim.x) {
-        bool ok;
-        const unsigned long long key = key_at(i, ok);
-        if (ok && key < K) {
-            c++;
-            mx = key > mx ? key : mx;
-        }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-        c += __shfl_xor(c, off);
-        const unsigned long long o = __shfl_xor(mx, off);
-        mx = o > mx ? o : mx;
-    }
-    if ((threadIdx.x & 63) == 0) {
-        red_c[threadIdx.x >> 6] = c;
-        red_m[threadIdx.x >> 6] = mx;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned long long cc = 0, mm = 0;
-        for (int j = 0; j < (int)(blockDim.x >> 6); j++) {
-            cc += red_c[j];
-            mm = red_m[j] > mm ? red_m[j] : mm;
-        }
-        const double upper = from_key(K);
-        double lower = upper;
-        if (cc == s_r1) lower = from_key(cc > 0 ? mm : st->below_max);
-        out[0] = (lower + upper) / 2.0;
-    }
-}
-
-// Radix select, 8 bits per pass (the form every 2048^2 benchmark run completed with): histograms
-// privatised in LDS; state: prefix (known high bits), rank still to find, NaN count.
+// Radix select over the keys, 8 bits per pass, with histograms privatised in LDS. Each pass
+// derives the digit of the pass before from its histogram (select.rs:79-153 in one pass per digit).
 struct SelectState {
     unsigned long long prefix[9];  // key digits fixed before pass p (prefix[8]: the selected key)
     unsigned long long rank[9];    // rank left within them
@@ -1446,18 +1215,6 @@ __global__ void k_confidence(const double *__restrict__ energy, const double *__
 // ------------------------------------------------------------------ host wrappers
 static dim3 grid2(int w, int h) { return dim3((w + 255) / 256, h); }
 
-// OFX_STAGED_KERNELS=1 selects the histogram-compaction median (12- and 10-bit histograms, a
-// compaction, a one-workgroup select) in place of the 8-bit radix select. Both give np.median
-// exactly; the radix select is the faster at the frame sizes measured.
-bool staged_kernels() {
-    static const bool on = [] {
-        const char *v = std::getenv("OFX_STAGED_KERNELS");
-        return v && v[0] == '1';
-    }();
-    return on;
-}
-
-
 void prefilter_real(float *c, float *tmp, int w, int h, hipStream_t s) {
     if (h > 1) {
         const dim3 g((w + 255) / 256, (h + PF_SEG - 1) / PF_SEG);
@@ -1555,7 +1312,7 @@ void energy_reset(const double *energy, const double *med, const double *ux_in, 
     LAUNCHCHECK();
 }
 
-size_t median_scratch_bytes() { return std::max(sizeof(MedState), sizeof(SelectState)); }
+size_t median_scratch_bytes() { return sizeof(SelectState); }
 
 
 // numpy.median of n values (NaN if any is NaN), for an even n the mean of the n/2 - 1 and n/2 order
@@ -1563,19 +1320,6 @@ size_t median_scratch_bytes() { return std::max(sizeof(MedState), sizeof(SelectS
 void median(const double *v, size_t n, void *scratch, double *out, hipStream_t s) {
     const unsigned long long k = n / 2;
     const int even = (n % 2 == 0) ? 1 : 0;
-    if (staged_kernels()) {
-        // a 12-bit and a 10-bit histogram pass over the order-preserving keys, a compaction of the
-        // keys sharing the selected 22-bit prefix, and a one-workgroup select among them
-        MedState *st = (MedState *)scratch;
-        const int blocks = (int)std::min<size_t>(512, (n + 255) / 256);
-        hipLaunchKernelGGL(k_msel_init, dim3(1), dim3(1024), 0, s, st);
-        hipLaunchKernelGGL(k_msel_hist0, dim3(blocks), dim3(256), 0, s, v, n, st);
-        hipLaunchKernelGGL(k_msel_hist1, dim3(blocks), dim3(256), 0, s, v, n, k, st);
-        hipLaunchKernelGGL(k_msel_compact, dim3(blocks), dim3(256), 0, s, v, n, k, st);
-        hipLaunchKernelGGL(k_msel_final, dim3(1), dim3(1024), 0, s, v, n, k, even, st, out);
-        LAUNCHCHECK();
-        return;
-    }
     SelectState *st = (SelectState *)scratch;
     hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, k);
     const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
