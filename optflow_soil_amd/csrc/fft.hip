@@ -190,15 +190,13 @@ __device__ __forceinline__ void line_body(const float2 *__restrict__ in, float2 
 // Direct passes: the twiddle table is staged in LDS behind the two line buffers. Three workgroups
 // share a CU (LDS): the register budget is capped at three wavefronts per SIMD.
 template <int MODE, int SH>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) k_lines(const float2 *__restrict__ in, float2 *__restrict__ out, PlanArgs P,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_lines(const float2 *__restrict__ in, float2 *__restrict__ out, PlanArgs P,
                                                const float2 *__restrict__ tw_g, Bank bank, int k, LineArgs A) {
     extern __shared__ float2 lds[];
     const int n = P.n;
     const int ln = lp_len<SH>(n);
-    float2 *a = lds, *b = lds + ln, *tw = lds + 2 * ln;
-    for (int i = threadIdx.x; i < n; i += blockDim.x) tw[i] = tw_g[i];
-    __syncthreads();
-    line_body<MODE, SH>(in, out, n, bank, k, A, a, b, DirectCore<SH>{P, tw});
+    float2 *a = lds, *b = lds + ln;
+    line_body<MODE, SH>(in, out, n, bank, k, A, a, b, DirectCore<SH>{P, tw_g});
 }
 
 // The chirp-z form: the two line buffers of length m (m <= 8192: at most 136 KiB) leave room for
@@ -346,7 +344,7 @@ Bank make_bank(const ofx_config &c) {
 namespace {
 // LDS of a line workgroup: two padded line buffers and the twiddle table (n <= 4096: at most
 // 102 KiB, within the 160 KiB a CDNA4 workgroup may hold).
-size_t line_lds_bytes(int n) { return (2 * (size_t)lp_len<4>(n) + (size_t)n) * sizeof(float2); }
+size_t line_lds_bytes(int n) { return 2 * (size_t)lp_len<4>(n) * sizeof(float2); }
 
 template <int MODE>
 void launch_lines(const float2 *in, float2 *out, int n, int lines, const Bank *bank, int k, const LineArgs &A,
