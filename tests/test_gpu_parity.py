@@ -397,3 +397,30 @@ def test_engine_errors_are_the_reference_messages(eng):
         eng.track_pair(a, a, Config(poc_win=48))
     with pytest.raises(ValueError, match="no frames"):
         list(eng.iter_sequence(iter([]), Config()))
+
+
+@pytest.mark.timeout(170)
+@pytest.mark.parametrize("workload", ["bowl_and_band_2048", "large_motion_4000x3000"])
+def test_benchmark_pairs_at_full_size(eng, workload):
+    """The benchmark's pairs at their full sizes, GPU against the oracle over the whole frame (all
+    host cores), to the same bar, and the end-point error of both against the synthetic truth:
+    the 2048 x 2048 settlement bowl and shear band with its 8-gon ROI, and the 4000 x 3000 pair
+    with six Gaussian bumps up to 40 px (max_disp 48, two POC levels, chirp-z transforms)."""
+    from optflow_soil_amd import synth
+    if workload == "bowl_and_band_2048":
+        a, b, roi, (tx, ty) = synth.workload_c2(2048)
+        cfg = Config()
+    else:
+        a, b, (tx, ty) = synth.workload_c5()
+        roi, cfg = None, Config(max_disp=48.0)
+    A, Bm = synth.as_float(a), synth.as_float(b)
+    print(f"{workload}: rendered {A.shape}", flush=True)
+    r = eng.track_pair(A, Bm, cfg, mask=roi)
+    print(f"{workload}: GPU done; oracle", flush=True)
+    o = O.track_pair(f64(A), f64(Bm), cfg, mask=roi)
+    compare_pair(r, o, workload)
+    for name, ux, uy in (("gpu", r.ux.astype(np.float64), r.uy.astype(np.float64)), ("oracle", o["ux"], o["uy"])):
+        fin = np.isfinite(ux)
+        epe = float(np.sqrt(np.mean((ux[fin] - tx[fin]) ** 2 + (uy[fin] - ty[fin]) ** 2)))
+        print(f"{workload}: {name} EPE vs truth {epe:.5f} px over {fin.mean():.4f} of the frame", flush=True)
+        assert epe < 0.06
