@@ -445,19 +445,20 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
     const Bank bank = make_bank(c);
     const int F = bank.F;
     set_taps(ws, c, s);
-    float2 *spec = B<float2>(ws.bspec, np), *tmp = B<float2>(ws.btmp, np), *tmp2 = B<float2>(ws.btmp2, np);
-    float2 *resp = B<float2>(ws.bresp, std::max<size_t>(F, 3) * np);
+    // F planes of scratch for the batched inverses; 3F response planes for the reference pack
+    float2 *spec = B<float2>(ws.bspec, np), *tmp = B<float2>(ws.btmp, F * np), *tmp2 = B<float2>(ws.btmp2, F * np);
+    float2 *resp = B<float2>(ws.bresp, 3 * F * np);
     float4 *refp = B<float4>(ws.brefp, F * n);
     step_mark("fine: reference pack");
-    // reference pack of image 1 (phaseflow.py:100-111): three inverses per filter
+    // reference pack of image 1 (phaseflow.py:100-111): three inverses per filter, each kind
+    // batched over the filters
     ws.prof.mark(ST_REFERENCE, s);
     fft_forward_padded(i1, w, h, spec, tmp, tmp2, s);
-    for (int k = 0; k < F; k++) {
-        fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G, false, tmp, tmp2, resp, s);
-        fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G_IFX, false, tmp, tmp2, resp + np, s);
-        fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G_IFY, false, tmp, tmp2, resp + 2 * np, s);
-        refpack(resp, resp + np, resp + 2 * np, refp + k * n, w, h, wp, s);
-    }
+    fft_inverse_filtered(spec, hp, wp, bank, 0, F, MUL_G, false, tmp, tmp2, resp, s);
+    fft_inverse_filtered(spec, hp, wp, bank, 0, F, MUL_G_IFX, false, tmp, tmp2, resp + F * np, s);
+    fft_inverse_filtered(spec, hp, wp, bank, 0, F, MUL_G_IFY, false, tmp, tmp2, resp + 2 * F * np, s);
+    for (int k = 0; k < F; k++)
+        refpack(resp + k * np, resp + (F + k) * np, resp + (2 * F + k) * np, refp + k * n, w, h, wp, s);
     // spline of image 2
     ws.prof.mark(ST_WARP, s);
     const size_t n2 = (size_t)w2 * h2;
@@ -515,7 +516,7 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
         warp(c2, w2, h2, shx, shy, sx, sy, warped, w, h, s);
         ws.prof.mark(ST_RESPONSES, s);
         fft_forward_padded(warped, w, h, spec, tmp, tmp2, s);
-        for (int k = 0; k < F; k++) fft_inverse_filtered(spec, hp, wp, bank, k, MUL_G, true, tmp, tmp2, resp + k * np, s);
+        fft_inverse_filtered(spec, hp, wp, bank, 0, F, MUL_G, true, tmp, tmp2, resp, s);
         ws.prof.mark(ST_SMOOTH, s);
         dcopy(csx, sx, n * sizeof(float), s);
         dcopy(csy, sy, n * sizeof(float), s);

@@ -88,6 +88,9 @@ struct LineArgs {
     int mul;
     int prefilter;
     double scale;
+    // a batch of planes (blockIdx.y): element offsets between consecutive input / output planes;
+    // plane j of an inverse uses filter k + j
+    size_t in_plane, out_plane;
 };
 
 // The transform of one line whose element e is load(e), by direct Stockham passes over LDS buffers
@@ -196,7 +199,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     const int n = P.n;
     const int ln = lp_len<SH>(n);
     float2 *a = lds, *b = lds + ln;
-    line_body<MODE, SH>(in, out, n, bank, k, A, a, b, DirectCore<SH>{P, tw_g});
+    line_body<MODE, SH>(in + blockIdx.y * A.in_plane, out + blockIdx.y * A.out_plane, n, bank, k + (int)blockIdx.y, A,
+                        a, b, DirectCore<SH>{P, tw_g});
 }
 
 // The chirp-z form: the two line buffers of length m (m <= 8192: at most 136 KiB) leave room for
@@ -206,7 +210,8 @@ __global__ void __launch_bounds__(1024) k_lines_blue(const float2 *__restrict__ 
                                                      BlueArgs B, Bank bank, int k, LineArgs A) {
     extern __shared__ float2 lds[];
     const int lm = lp_len<SH>(B.M.n);
-    line_body<MODE, SH>(in, out, n, bank, k, A, lds, lds + lm, BlueCore<SH>{B, n});
+    line_body<MODE, SH>(in + blockIdx.y * A.in_plane, out + blockIdx.y * A.out_plane, n, bank, k + (int)blockIdx.y, A,
+                        lds, lds + lm, BlueCore<SH>{B, n});
 }
 
 // b_j = exp(i pi (j^2 mod 2n) / n) from the double angle (the product j^2 is exact in 64 bits)
@@ -236,11 +241,14 @@ __global__ void k_chirp_spectrum(float2 *__restrict__ C, int n, int m) {
     C[kk] = make_float2((float)(sr / m), (float)(si / m));
 }
 
-// out[c][r] = in[r][c] for an R x C array, 32 x 32 tiles through LDS.
+// out[c][r] = in[r][c] for an R x C array, 32 x 32 tiles through LDS; blockIdx.z: the plane of a
+// batch of such arrays stored one after the other.
 __global__ void __launch_bounds__(256) k_transpose(const float2 *__restrict__ in, float2 *__restrict__ out, int R,
                                                    int C) {
     __shared__ float2 tile[32][33];
     const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    in += (size_t)blockIdx.z * R * C;
+    out += (size_t)blockIdx.z * R * C;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
     for (int j = ty; j < 32; j += 8) {
         const int r = r0 + j, c = c0 + tx;
@@ -347,8 +355,8 @@ namespace {
 size_t line_lds_bytes(int n) { return 2 * (size_t)lp_len<4>(n) * sizeof(float2); }
 
 template <int MODE>
-void launch_lines(const float2 *in, float2 *out, int n, int lines, const Bank *bank, int k, const LineArgs &A,
-                  hipStream_t s) {
+void launch_lines(const float2 *in, float2 *out, int n, int lines, int planes, const Bank *bank, int k,
+                  const LineArgs &A, hipStream_t s) {
     const FftPlan &P = fft_plan(n, s);
     Bank dummy{};
     if (P.blue_m) {
@@ -360,7 +368,7 @@ void launch_lines(const float2 *in, float2 *out, int n, int lines, const Bank *b
                                       160 * 1024);
         });
         const BlueArgs B{args_of(M), M.tw, P.chirp, P.chirp_spec};
-        hipLaunchKernelGGL((k_lines_blue<MODE, 4>), dim3(lines), dim3(1024), lds, s, in, out, n, B,
+        hipLaunchKernelGGL((k_lines_blue<MODE, 4>), dim3(lines, planes), dim3(1024), lds, s, in, out, n, B,
                            bank ? *bank : dummy, k, A);
         LAUNCHCHECK();
         return;
@@ -373,13 +381,13 @@ void launch_lines(const float2 *in, float2 *out, int n, int lines, const Bank *b
                                       160 * 1024);
         });
     }
-    hipLaunchKernelGGL((k_lines<MODE, 4>), dim3(lines), dim3(256), lds, s, in, out, args_of(P), P.tw,
+    hipLaunchKernelGGL((k_lines<MODE, 4>), dim3(lines, planes), dim3(256), lds, s, in, out, args_of(P), P.tw,
                        bank ? *bank : dummy, k, A);
     LAUNCHCHECK();
 }
 
-void transpose(const float2 *in, float2 *out, int R, int C, hipStream_t s) {
-    hipLaunchKernelGGL(k_transpose, dim3((C + 31) / 32, (R + 31) / 32), dim3(256), 0, s, in, out, R, C);
+void transpose(const float2 *in, float2 *out, int R, int C, int planes, hipStream_t s) {
+    hipLaunchKernelGGL(k_transpose, dim3((C + 31) / 32, (R + 31) / 32, planes), dim3(256), 0, s, in, out, R, C);
     LAUNCHCHECK();
 }
 }  // namespace
@@ -390,21 +398,25 @@ void fft_forward_padded(const float *img, int w, int h, float2 *spec, float2 *tm
     A.img = img;
     A.w = w;
     A.h = h;
-    launch_lines<MODE_FWD_REFLECT>(nullptr, tmp, wp, hp, nullptr, 0, A, s);  // [y][kx]
-    transpose(tmp, tmp2, hp, wp, s);                                        // [kx][y]
-    launch_lines<MODE_PLAIN>(tmp2, spec, hp, wp, nullptr, 0, A, s);         // [kx][ky]
+    launch_lines<MODE_FWD_REFLECT>(nullptr, tmp, wp, hp, 1, nullptr, 0, A, s);  // [y][kx]
+    transpose(tmp, tmp2, hp, wp, 1, s);                                        // [kx][y]
+    launch_lines<MODE_PLAIN>(tmp2, spec, hp, wp, 1, nullptr, 0, A, s);         // [kx][ky]
 }
 
-void fft_inverse_filtered(const float2 *spec, int hp, int wp, const Bank &bank, int k, SpecMul mul, bool prefilter,
-                          float2 *tmp, float2 *tmp2, float2 *out, hipStream_t s) {
+void fft_inverse_filtered(const float2 *spec, int hp, int wp, const Bank &bank, int k0, int K, SpecMul mul,
+                          bool prefilter, float2 *tmp, float2 *tmp2, float2 *out, hipStream_t s) {
+    const size_t np = (size_t)hp * wp;
     LineArgs A{};
     A.other = wp;
     A.mul = (int)mul;
     A.prefilter = prefilter ? 1 : 0;
     A.scale = 1.0 / ((double)hp * (double)wp);
-    launch_lines<MODE_INV_FIRST>(spec, tmp, hp, wp, &bank, k, A, s);  // [kx][y]
-    transpose(tmp, tmp2, wp, hp, s);                                 // [y][kx]
-    launch_lines<MODE_INV_SECOND>(tmp2, out, wp, hp, nullptr, 0, A, s);
+    A.in_plane = 0;  // every filter reads the one spectrum
+    A.out_plane = np;
+    launch_lines<MODE_INV_FIRST>(spec, tmp, hp, wp, K, &bank, k0, A, s);  // [kx][y] per filter
+    transpose(tmp, tmp2, wp, hp, K, s);                                  // [y][kx]
+    A.in_plane = np;
+    launch_lines<MODE_INV_SECOND>(tmp2, out, wp, hp, K, nullptr, 0, A, s);
 }
 
 void fft_release() {

@@ -124,10 +124,11 @@ enum SpecMul { MUL_G = 0, MUL_G_IFX = 1, MUL_G_IFY = 2 };
 // Forward 2-D FFT of the reflect-padded real image img (w x h, pitch w) into spec, stored
 // transposed: spec[kx * hp + ky]. tmp, tmp2: hp x wp scratch.
 void fft_forward_padded(const float *img, int w, int h, float2 *spec, float2 *tmp, float2 *tmp2, hipStream_t s);
-// Inverse 2-D FFT of spec * G_k (* i fx or i fy) into out (hp x wp, natural layout). With
-// prefilter, the cubic B-spline prefilter is applied along both axes (out holds spline
-// coefficients).
-void fft_inverse_filtered(const float2 *spec, int hp, int wp, const Bank &bank, int k, SpecMul mul,
+// Inverse 2-D FFTs of spec * G_k (* i fx or i fy) for the K filters k = k0 .. k0 + K - 1 into the
+// K planes of out (hp x wp each, natural layout), batched in one launch per pass so that the
+// lines of every filter share the GPU. With prefilter, the cubic B-spline prefilter is applied
+// along both axes (out holds spline coefficients). tmp, tmp2: K planes of scratch each.
+void fft_inverse_filtered(const float2 *spec, int hp, int wp, const Bank &bank, int k0, int K, SpecMul mul,
                           bool prefilter, float2 *tmp, float2 *tmp2, float2 *out, hipStream_t s);
 
 // ---------------------------------------------------------------- fine stage and friends
