@@ -185,6 +185,11 @@ __global__ void k_refpack(const float2 *__restrict__ r, const float2 *__restrict
 constexpr int ACC_T = 16;
 constexpr int ACC_BOX = 1600;  // nodes per staged box (40 x 40)
 
+// The reference amplitude sqrt(|r|^2 + 1e-12), which the reference stores as f32 (gabor.rs:260-269),
+// here in f32 arithmetic (within 2 ulp of that stored value); the weight is its product with the
+// sample's modulus, formed in f64 as the reference forms it (phaseflow.rs:1268-1277).
+__device__ __forceinline__ float amp_of(const float4 &p) { return sqrtf(fmaf(p.x, p.x, fmaf(p.y, p.y, 1e-12f))); }
+
 // Workgroups are dealt to the chip's 8 XCDs in turn (dispatch index L runs on XCD L mod 8) and
 // each XCD has its own L2. Each row of tiles is split into 8 chunks of c = ceil(gx / 8) adjacent
 // tiles, chunk x going to XCD x: neighbouring tiles, whose staged boxes overlap and share cache
@@ -324,8 +329,7 @@ __global__ void __launch_bounds__(256) k_accumulate_tiled(const double *__restri
                 const float re = p.x * r2x + p.y * r2y;
                 const float im = p.y * r2x - p.x * r2y;
                 const double dphi = (double)atan2f(im, re);
-                const double a1 = sqrt((double)p.x * p.x + (double)p.y * p.y + 1e-12);
-                const double wk = a1 * (double)hypotf(r2x, r2y);
+                const double wk = (double)amp_of(p) * (double)sqrtf(fmaf(r2x, r2x, r2y * r2y));
                 const double gx = p.z, gy = p.w;
                 const double c = dphi + gx * u + gy * v;
                 const double wgx = wk * gx, wgy = wk * gy;
@@ -354,8 +358,7 @@ __global__ void __launch_bounds__(256) k_accumulate_tiled(const double *__restri
             const float re = p.x * r2.x + p.y * r2.y;
             const float im = p.y * r2.x - p.x * r2.y;
             const double dphi = (double)atan2f(im, re);
-            const double a1 = sqrt((double)p.x * p.x + (double)p.y * p.y + 1e-12);
-            const double wk = a1 * (double)hypotf(r2.x, r2.y);
+            const double wk = (double)amp_of(p) * (double)sqrtf(fmaf(r2.x, r2.x, r2.y * r2.y));
             const double gx = p.z, gy = p.w;
             const double c = dphi + gx * u + gy * v;
             const double wgx = wk * gx, wgy = wk * gy;
@@ -632,14 +635,18 @@ __global__ void __launch_bounds__(256) k_moments_y_tiled(const double *__restric
     }
 }
 
+// RC > 0: the radius fixed at compile time (r = RC), so that the tap loop unrolls and every LDS
+// read takes an immediate offset; RC = 0: any r <= 32 from the argument.
+template <int RC>
 __global__ void __launch_bounds__(256) k_moments_x_solve_tiled(const double *__restrict__ ym,
                                                                const double *__restrict__ ux_in,
                                                                const double *__restrict__ uy_in,
                                                                double *__restrict__ ux_out, double *__restrict__ uy_out,
                                                                double *__restrict__ energy, double *__restrict__ resid,
                                                                double *__restrict__ slope, int w, int h, const KT kt,
-                                                               int r) {
+                                                               int r_arg) {
     extern __shared__ double seg[];  // 15 planes x (256 + 2r)
+    const int r = RC > 0 ? RC : r_arg;
     const int x0 = blockIdx.x * 256, y = blockIdx.y;
     const int cols = 256 + 2 * r, klen = 2 * r + 1;
     const size_t n = (size_t)w * h;
@@ -665,6 +672,7 @@ __global__ void __launch_bounds__(256) k_moments_x_solve_tiled(const double *__r
 #pragma unroll
             for (int b = 0; b < 2; b++) mr[p][a][b] = 0;
     const double *s0 = seg + threadIdx.x;  // column x - r
+#pragma unroll (RC > 0 ? 11 : 1)
     for (int j = 0; j < klen; j++) {
         const double k0 = kt.k[0][j], k1 = kt.k[1][j], k2 = kt.k[2][j];
 #pragma unroll
@@ -1272,7 +1280,7 @@ void moments_solve(const double *sums, double *ym, const double *ux_in, const do
         if (r == 16) {
             hipLaunchKernelGGL(k_moments_y_rb<16>, dim3((w + 255) / 256, (h + RB_ROWS - 1) / RB_ROWS, 7), dim3(256), 0,
                                s, sums, ym, w, h, kt_arg);
-            hipLaunchKernelGGL(k_moments_x_solve_tiled, dim3((w + 255) / 256, h), dim3(256),
+            hipLaunchKernelGGL(k_moments_x_solve_tiled<16>, dim3((w + 255) / 256, h), dim3(256),
                                (size_t)15 * (256 + 2 * r) * sizeof(double), s, ym, ux_in, uy_in, ux_out, uy_out, energy,
                                resid, slope, w, h, kt_arg, r);
             LAUNCHCHECK();
@@ -1280,7 +1288,7 @@ void moments_solve(const double *sums, double *ym, const double *ux_in, const do
         }
         hipLaunchKernelGGL(k_moments_y_tiled, dim3((w + MT_TW - 1) / MT_TW, (h + MT_TH - 1) / MT_TH), dim3(256),
                            (size_t)(MT_TH + 2 * r) * MT_TW * sizeof(double), s, sums, ym, w, h, kt_arg, r);
-        hipLaunchKernelGGL(k_moments_x_solve_tiled, dim3((w + 255) / 256, h), dim3(256),
+        hipLaunchKernelGGL(k_moments_x_solve_tiled<0>, dim3((w + 255) / 256, h), dim3(256),
                            (size_t)15 * (256 + 2 * r) * sizeof(double), s, ym, ux_in, uy_in, ux_out, uy_out, energy,
                            resid, slope, w, h, kt_arg, r);
     } else {
