@@ -294,10 +294,22 @@ struct Workspace {
         bcax, bcay, bca_adj, bgate0, bgate, bhcnt, bgscr, bhlow, bhux, bhuy, bfb;
     CoarseWs *coarse = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    // the tiled fine stage: the workspaces and streams of the tile groups that run beside the
+    // lane's own stream, and the events that fork them from it and join them back (first use)
+    std::vector<std::unique_ptr<Workspace>> sub;
+    std::vector<hipStream_t> sub_streams;
+    std::vector<hipEvent_t> sub_done;
+    hipEvent_t fork = nullptr;
     ~Workspace() {
         coarse_release(coarse);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (hipStream_t t : sub_streams) {
+            (void)hipStreamSynchronize(t);
+            (void)hipStreamDestroy(t);
+        }
+        for (hipEvent_t e : sub_done) (void)hipEventDestroy(e);
+        if (fork) (void)hipEventDestroy(fork);
     }
 };
 
@@ -614,6 +626,10 @@ __global__ void k_write_core(const double *__restrict__ tux, const double *__res
     jr[i] = tres[l];
 }
 
+// Tile groups refined side by side in the tiled fine stage: the lane's stream and two more, which
+// with the stream of the caller keeps to the four hardware queues a process gets by default.
+constexpr int TILE_GROUPS = 3;
+
 // tile.rs:80-95
 int tile_margin(const ofx_config &c) {
     return std::max({64, (int)std::ceil(4.0 * c.sigma_w), (int)std::ceil(4.0 * c.sigma_s),
@@ -658,25 +674,28 @@ void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, c
     for (size_t k = 0; k < ts.size(); k++) order[k] = k;
     std::stable_sort(order.begin(), order.end(),
                      [&](size_t a, size_t b) { return ts[a].second.area() > ts[b].second.area(); });
-    const int nblk = 256;
-    double *part = B<double>(ws.bpart, 4 * nblk);
-    std::vector<double> hpart(4 * nblk);
-    for (size_t k : order) {
-        step_mark("fine: a tile");
-        const TRect core = ts[k].first, ext = ts[k].second;
-        const int ew = ext.x1 - ext.x0, eh = ext.y1 - ext.y0;
-        const size_t ne = (size_t)ew * eh;
-        // tile.rs:112-146: the extension moved by the finite range of its (f32) start field
+    // tile.rs:112-146: each extension moved by the finite range of its (f32) start field; the
+    // ranges of every tile come from the start field alone, so they are found first, in one pass
+    const int nblk = 256, nt = (int)ts.size();
+    double *part = B<double>(ws.bpart, (size_t)4 * nblk * nt);
+    for (int k = 0; k < nt; k++) {
+        const TRect ext = ts[k].second;
         hipLaunchKernelGGL(k_rect_minmax, dim3(nblk), dim3(256), 0, s, (const double *)fsx, (const double *)fsy, w,
-                           ext.x0, ext.y0, ew, eh, part);
+                           ext.x0, ext.y0, ext.x1 - ext.x0, ext.y1 - ext.y0, part + (size_t)4 * nblk * k);
         LAUNCHCHECK();
-        copy_d2h(ws.pin, hpart.data(), part, hpart.size() * sizeof(double), s);
+    }
+    std::vector<double> hpart((size_t)4 * nblk * nt);
+    copy_d2h(ws.pin, hpart.data(), part, hpart.size() * sizeof(double), s);
+    std::vector<TRect> wins(nt);
+    for (int k = 0; k < nt; k++) {
+        const TRect ext = ts[k].second;
+        const double *hp = hpart.data() + (size_t)4 * nblk * k;
         double lox = INFINITY, hix = -INFINITY, loy = INFINITY, hiy = -INFINITY;
         for (int b = 0; b < nblk; b++) {
-            lox = std::min(lox, hpart[4 * b]);
-            hix = std::max(hix, hpart[4 * b + 1]);
-            loy = std::min(loy, hpart[4 * b + 2]);
-            hiy = std::max(hiy, hpart[4 * b + 3]);
+            lox = std::min(lox, hp[4 * b]);
+            hix = std::max(hix, hp[4 * b + 1]);
+            loy = std::min(loy, hp[4 * b + 2]);
+            hiy = std::max(hiy, hp[4 * b + 3]);
         }
         if (lox > hix) lox = hix = 0.0;
         if (loy > hiy) loy = hiy = 0.0;
@@ -687,26 +706,72 @@ void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, c
             fb = std::min(std::max(fb, (double)(a + 1)), (double)nn);
             b = (int)fb;
         };
-        TRect win;
-        span(ext.x0, ext.x1, lox, hix, w, win.x0, win.x1);
-        span(ext.y0, ext.y1, loy, hiy, h, win.y0, win.y1);
+        span(ext.x0, ext.x1, lox, hix, w, wins[k].x0, wins[k].x1);
+        span(ext.y0, ext.y1, loy, hiy, h, wins[k].y0, wins[k].y1);
+    }
+    // The tiles are independent refines (tile.rs:225-330): they run in TILE_GROUPS groups, the
+    // group g taking the tiles order[g], order[g + G], ... (largest first), group 0 on the lane's
+    // stream and workspace, the others on streams and workspaces of their own that fork from the
+    // lane's stream here and join it before the joined planes are read. Each group's workspace
+    // is used by one stream only; the cores the tiles write into jx .. jr are disjoint.
+    const int G = std::min(TILE_GROUPS, nt);
+    while ((int)ws.sub.size() < G - 1) {
+        ws.sub.emplace_back(new Workspace);
+        hipStream_t t;
+        HIPCHECK(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+        ws.sub_streams.push_back(t);
+        hipEvent_t e;
+        HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        ws.sub_done.push_back(e);
+    }
+    if (!ws.fork) HIPCHECK(hipEventCreateWithFlags(&ws.fork, hipEventDisableTiming));
+    HIPCHECK(hipEventRecord(ws.fork, s));
+    for (int g = 1; g < G; g++) {
+        HIPCHECK(hipStreamWaitEvent(ws.sub_streams[g - 1], ws.fork, 0));
+        ws.sub[g - 1]->prof.on = ws.prof.on;
+    }
+    for (int q = 0; q < nt; q++) {
+        step_mark("fine: a tile");
+        const int g = q % G;
+        Workspace &tw = g == 0 ? ws : *ws.sub[g - 1];
+        hipStream_t ts_ = g == 0 ? s : ws.sub_streams[g - 1];
+        const size_t k = order[q];
+        const TRect core = ts[k].first, ext = ts[k].second, win = wins[k];
+        const int ew = ext.x1 - ext.x0, eh = ext.y1 - ext.y0;
+        const size_t ne = (size_t)ew * eh;
         const int ww = win.x1 - win.x0, wh = win.y1 - win.y0;
-        float *t1 = B<float>(ws.btile1, ne), *t2 = B<float>(ws.btile2, (size_t)ww * wh);
-        dcopy2d(t1, ew * sizeof(float), i1 + (size_t)ext.y0 * w + ext.x0, w * sizeof(float),
-                                  ew * sizeof(float), eh, s);
-        dcopy2d(t2, ww * sizeof(float), i2 + (size_t)win.y0 * w + win.x0, w * sizeof(float),
-                                  ww * sizeof(float), wh, s);
-        double *sx = B<double>(ws.bux, ne), *sy = B<double>(ws.buy, ne);
+        float *t1 = B<float>(tw.btile1, ne), *t2 = B<float>(tw.btile2, (size_t)ww * wh);
+        dcopy2d(t1, ew * sizeof(float), i1 + (size_t)ext.y0 * w + ext.x0, w * sizeof(float), ew * sizeof(float), eh,
+                ts_);
+        dcopy2d(t2, ww * sizeof(float), i2 + (size_t)win.y0 * w + win.x0, w * sizeof(float), ww * sizeof(float), wh,
+                ts_);
+        double *sx = B<double>(tw.bux, ne), *sy = B<double>(tw.buy, ne);
         dcopy2d(sx, ew * sizeof(double), fsx + (size_t)ext.y0 * w + ext.x0, w * sizeof(double),
-                                  ew * sizeof(double), eh, s);
+                ew * sizeof(double), eh, ts_);
         dcopy2d(sy, ew * sizeof(double), fsy + (size_t)ext.y0 * w + ext.x0, w * sizeof(double),
-                                  ew * sizeof(double), eh, s);
-        refine_device(ws, t1, t2, ew, eh, ww, wh, ext.x0 - win.x0, ext.y0 - win.y0, c, s);
+                ew * sizeof(double), eh, ts_);
+        refine_device(tw, t1, t2, ew, eh, ww, wh, ext.x0 - win.x0, ext.y0 - win.y0, c, ts_);
         const int cw = core.x1 - core.x0;
-        hipLaunchKernelGGL(k_write_core, dim3((cw + 255) / 256, core.y1 - core.y0), dim3(256), 0, s,
-                           (const double *)ws.bux.p, (const double *)ws.buy.p, (const double *)ws.benergy.p,
-                           (const double *)ws.bresid.p, ew, ext.x0, ext.y0, core.x0, core.y0, cw, w, jx, jy, je, jr);
+        hipLaunchKernelGGL(k_write_core, dim3((cw + 255) / 256, core.y1 - core.y0), dim3(256), 0, ts_,
+                           (const double *)tw.bux.p, (const double *)tw.buy.p, (const double *)tw.benergy.p,
+                           (const double *)tw.bresid.p, ew, ext.x0, ext.y0, core.x0, core.y0, cw, w, jx, jy, je, jr);
         LAUNCHCHECK();
+    }
+    for (int g = 1; g < G; g++) {
+        HIPCHECK(hipEventRecord(ws.sub_done[g - 1], ws.sub_streams[g - 1]));
+        HIPCHECK(hipStreamWaitEvent(s, ws.sub_done[g - 1], 0));
+    }
+    if (ws.prof.on) {
+        // the stage times of the other groups, measured on their streams, join the lane's
+        for (int g = 1; g < G; g++) {
+            Profiler &sp = ws.sub[g - 1]->prof;
+            sp.flush();
+            for (int i = 0; i < ST_COUNT; i++) {
+                ws.prof.ms[i] += sp.ms[i];
+                ws.prof.calls[i] += sp.calls[i];
+            }
+            sp.reset();
+        }
     }
     double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
     double *energy = B<double>(ws.benergy, n), *resid = B<double>(ws.bresid, n);

@@ -295,8 +295,9 @@ def test_batched_pairs_match_single_pairs(eng):
 
 def test_tiled_pair(eng):
     """Config.tile: the reference's tiled fine stage (tile.rs:225-500) on the GPU, 2 x 3 tiles of
-    128 px: against the oracle's tiled pair at the pair bar, and against the GPU's whole-frame pair
-    to the reference's tiling bar, 1e-3 px."""
+    128 px in three groups that run side by side: against the oracle's tiled pair at the pair bar,
+    against the GPU's whole-frame pair to the reference's tiling bar, 1e-3 px, and bit for bit
+    against a second run (each tile is refined alone, whatever runs beside it)."""
     from optflow_soil_amd import synth
     h, w = 224, 288
     pat = synth.make_pattern(h, w, seed=4)
@@ -307,6 +308,9 @@ def test_tiled_pair(eng):
     r = eng.track_pair(a, b, Config(tile=128))
     o = O.track_pair(f64(a), f64(b), Config(tile=128))
     compare_pair(r, o, "tiled pair")
+    again = eng.track_pair(a, b, Config(tile=128))
+    for name in ("ux", "uy", "conf", "raw_ux", "raw_uy"):
+        np.testing.assert_array_equal(getattr(r, name), getattr(again, name), err_msg=name)
     whole = eng.track_pair(a, b, Config())
     fin = np.isfinite(whole.ux) & np.isfinite(r.ux)
     print(f"tiled vs whole frame: max ux {np.abs(whole.ux - r.ux)[fin].max():.2e}", flush=True)
