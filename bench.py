@@ -33,6 +33,24 @@ def accumulate_bytes(w, h, F):
     return n * (16 + 8 + 8 + 16 * F + 56) + 8 * F * np_
 
 
+def pipeline_bytes(w, h, tile=0):
+    """Algorithmic HBM bytes of one whole pair with the default bank (F = 12 filters, 8 outer and
+    2 inner iterations), each intermediate read once and written once: 6388 B per frame pixel
+    plus 9120 B per padded-plane node of the fine stage, and about 100 B per pixel for the
+    coarse stage, guard and strain. Tiled, the planes are those of the tile extensions."""
+    if tile and max(w, h) > tile:
+        m = 64
+        n, np_ = 0, 0
+        for ky in range(-(-h // tile)):
+            for kx in range(-(-w // tile)):
+                ew = min(w, (kx + 1) * tile + m) - max(0, kx * tile - m)
+                eh = min(h, (ky + 1) * tile + m) - max(0, ky * tile - m)
+                n += ew * eh
+                np_ += (ew + 64) * (eh + 64)
+        return 6388 * n + 9120 * np_ + 100 * w * h
+    return 6388 * w * h + 9120 * (w + 64) * (h + 64) + 100 * w * h
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -161,6 +179,10 @@ def main():
                      "algorithmic_bytes_per_launch": acc_bytes, "avg_launch_ms": round(acc_avg_s * 1e3, 4)},
         "stage_ms_per_step": stages,
         "epe_px": round(epe, 5), "valid_frac_roi": round(valid_frac, 5),
+        "pipeline": {"bound": "hbm", "bytes_per_step": pipeline_bytes(w, h, cfg.tile),
+                     "achieved": round(pipeline_bytes(w, h, cfg.tile) / dt / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s",
+                     "frac": round(pipeline_bytes(w, h, cfg.tile) / dt / 1e9 / HBM_PEAK_GBS, 4)},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
