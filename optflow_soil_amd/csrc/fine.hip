@@ -754,88 +754,6 @@ __global__ void __launch_bounds__(256) k_moments_y_rb(const double *__restrict__
     }
 }
 
-// x pass for R = 16 with 2 adjacent pixels per thread from the LDS row segment: each loaded value
-// serves both pixels, halving the LDS reads of the one-pixel form.
-__global__ void __launch_bounds__(128) k_moments_x_solve_tiled2(const double *__restrict__ ym,
-                                                                const double *__restrict__ ux_in,
-                                                                const double *__restrict__ uy_in,
-                                                                double *__restrict__ ux_out,
-                                                                double *__restrict__ uy_out,
-                                                                double *__restrict__ energy,
-                                                                double *__restrict__ resid, double *__restrict__ slope,
-                                                                int w, int h, const KT kt) {
-    constexpr int R = 16;
-    __shared__ double seg[15 * (256 + 2 * R)];
-    const int x0 = blockIdx.x * 256, y = blockIdx.y;
-    constexpr int cols = 256 + 2 * R;
-    const size_t n = (size_t)w * h;
-    // column c of plane q at q * cols + (c & 1) * cols / 2 + c / 2: even and odd columns apart, so
-    // that the lanes reading columns 2 t + j touch consecutive addresses
-    for (int idx = threadIdx.x; idx < 15 * cols; idx += blockDim.x) {
-        const int q = idx / cols, c = idx - q * cols;
-        const int xx = min(max(x0 - R + c, 0), w - 1);
-        seg[q * cols + (c & 1) * (cols / 2) + (c >> 1)] = ym[(size_t)q * n + (size_t)y * w + xx];
-    }
-    __syncthreads();
-    const int xa = x0 + 2 * threadIdx.x;
-    if (xa >= w) return;
-    double mp[2][3][3][3], mr[2][2][2][2], mcc[2] = {0, 0}, mw[2] = {0, 0};
-#pragma unroll
-    for (int e = 0; e < 2; e++) {
-#pragma unroll
-        for (int p = 0; p < 3; p++)
-#pragma unroll
-            for (int a = 0; a < 3; a++)
-#pragma unroll
-                for (int b = 0; b < 3; b++) mp[e][p][a][b] = 0;
-#pragma unroll
-        for (int p = 0; p < 2; p++)
-#pragma unroll
-            for (int a = 0; a < 2; a++)
-#pragma unroll
-                for (int b = 0; b < 2; b++) mr[e][p][a][b] = 0;
-    }
-    const double *se = seg + threadIdx.x, *so = seg + cols / 2 + threadIdx.x;  // columns xa - R (+1)
-    for (int jj = 0; jj < 1 + R; jj++) {
-#pragma unroll
-      for (int par = 0; par < 2; par++) {
-        const int j = 2 * jj + par;
-        double v[15];
-#pragma unroll
-        for (int q = 0; q < 15; q++) v[q] = (par ? so : se)[q * cols + jj];
-#pragma unroll
-        for (int e = 0; e < 2; e++) {
-            const int t = j - e;
-            if (t >= 0 && t <= 2 * R) {
-                const double k0 = kt.k[0][t], k1 = kt.k[1][t], k2 = kt.k[2][t];
-#pragma unroll
-                for (int p = 0; p < 3; p++) {
-                    mp[e][p][0][0] += k0 * v[3 * p];
-                    mp[e][p][1][0] += k1 * v[3 * p];
-                    mp[e][p][2][0] += k2 * v[3 * p];
-                    mp[e][p][0][1] += k0 * v[3 * p + 1];
-                    mp[e][p][1][1] += k1 * v[3 * p + 1];
-                    mp[e][p][0][2] += k0 * v[3 * p + 2];
-                }
-#pragma unroll
-                for (int p = 0; p < 2; p++) {
-                    mr[e][p][0][0] += k0 * v[9 + 2 * p];
-                    mr[e][p][1][0] += k1 * v[9 + 2 * p];
-                    mr[e][p][0][1] += k0 * v[10 + 2 * p];
-                }
-                mcc[e] += k0 * v[13];
-                mw[e] += k0 * v[14];
-            }
-        }
-      }
-    }
-#pragma unroll
-    for (int e = 0; e < 2; e++)
-        if (xa + e < w)
-            solve_pixel(mp[e], mr[e], mcc[e], mw[e], (size_t)y * w + xa + e, ux_in, uy_in, ux_out, uy_out, energy,
-                        resid, slope);
-}
-
 // Energy threshold (phaseflow.py:149): pixels with energy <= 1e-9 median + 1e-30 (or NaN) keep the
 // input displacement and get resid = pi.
 __global__ void k_energy_reset(const double *__restrict__ energy, const double *__restrict__ med,
