@@ -391,6 +391,25 @@ class Sequence:
         _check(lib().ofx_seq_step(self._h, _p(f), ctypes.byref(r)))
         return _to_result(r, arrs, valid)
 
+    def step_device(self, d_frame, d_out):
+        """ofx_seq_step_device: the next frame from a device pointer (an integer, e.g. torch's
+        data_ptr()), the results into the device arrays of d_out (result name -> pointer, "fb"
+        for the hybrid counter). Returns the timing and sequence figures."""
+        if not self._h:
+            raise RuntimeError("sequence is closed")
+        r = OfxResult()
+        for k in _RES_F32:
+            setattr(r, k, ctypes.cast(ctypes.c_void_p(d_out.get(k, 0) or None), _FP))
+        r.valid = ctypes.cast(ctypes.c_void_p(d_out.get("valid", 0) or None), ctypes.POINTER(ctypes.c_uint8))
+        r.fb = ctypes.cast(ctypes.c_void_p(d_out.get("fb", 0) or None), ctypes.POINTER(ctypes.c_int32))
+        _check(lib().ofx_seq_step_device(self._h, ctypes.c_void_p(d_frame), ctypes.byref(r)))
+        t = {"coarse": r.t_coarse, "fine": r.t_fine, "strain": r.t_strain, "reference": r.reference,
+             "mean_conf": r.mean_conf}
+        if self._hybrid:
+            t.update(n_fb_max=r.n_fb_max, fallback_frac=r.fallback_frac, fb_gate_rejected=r.fb_gate_rejected,
+                     reacquired=r.reacquired)
+        return t
+
     def close(self):
         if self._h:
             lib().ofx_seq_end(self._h)

@@ -273,7 +273,11 @@ struct Workspace {
     Profiler prof;
     int w = 0, h = 0, F = 0;
     DevBuf bi1, bi2, bmask, binx, biny;                 // staging of host inputs
-    DevBuf bc2, bspec, btmp, btmp2, bresp, brefp;
+    DevBuf bc2, bspec, btmp, btmp2, bresp, brefp, bpack;
+    // the reference pack of a sequence's reference frame (refine_device, cache_i1)
+    bool pack_valid = false;
+    int pack_w = 0, pack_h = 0;
+    Bank pack_bank{};
     DevBuf bux, buy, bnux, bnuy, bsx, bsy, bcsx, bcsy, bwarped, bfx, bfy, bbt;
     DevBuf bsums, bym, benergy, bresid, bslope, bmed, bmedscr, bmom2;
     DevBuf bconf, bconf64, blow, bkept, bwithin, bok, bcold2, bcolrow, bcnt;
@@ -450,8 +454,15 @@ void set_taps(Workspace &ws, const ofx_config &c, hipStream_t s) {
 // displaces it. The start field is in ws.bux / ws.buy (double); the result replaces it, and energy,
 // resid, the energy median (ws.bmed[0]) and the final confidence (ws.bconf as float, ws.bconf64 as
 // double) are left in the workspace.
+bool same_bank(const Bank &a, const Bank &b) {
+    if (a.F != b.F || a.s_hp2_half != b.s_hp2_half) return false;
+    for (int k = 0; k < a.F; k++)
+        if (a.w0x[k] != b.w0x[k] || a.w0y[k] != b.w0y[k] || a.inv2s2[k] != b.inv2s2[k]) return false;
+    return true;
+}
+
 void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h, int w2, int h2, long shx, long shy,
-                   const ofx_config &c, hipStream_t s) {
+                   const ofx_config &c, hipStream_t s, bool cache_i1) {
     const int wp = w + 2 * PAD, hp = h + 2 * PAD;
     const size_t n = (size_t)w * h, np = (size_t)wp * hp;
     const Bank bank = make_bank(c);
@@ -460,17 +471,29 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
     // F planes of scratch for the batched inverses; 3F response planes for the reference pack
     float2 *spec = B<float2>(ws.bspec, np), *tmp = B<float2>(ws.btmp, F * np), *tmp2 = B<float2>(ws.btmp2, F * np);
     float2 *resp = B<float2>(ws.bresp, 3 * F * np);
-    float4 *refp = B<float4>(ws.brefp, F * n);
+    // A sequence's reference frame is image 1 of the direct leg at every step until the reference
+    // moves: its pack is kept in ws.bpack and computed again only after the sequence marks it
+    // stale (ws.pack_valid), or when the frame size or the filter bank changed.
+    const bool cached = cache_i1 && ws.pack_valid && ws.pack_w == w && ws.pack_h == h && same_bank(ws.pack_bank, bank);
+    float4 *refp = cache_i1 ? B<float4>(ws.bpack, F * n) : B<float4>(ws.brefp, F * n);
     step_mark("fine: reference pack");
-    // reference pack of image 1 (phaseflow.py:100-111): three inverses per filter, each kind
-    // batched over the filters
     ws.prof.mark(ST_REFERENCE, s);
-    fft_forward_padded(i1, w, h, spec, tmp, tmp2, s);
-    fft_inverse_filtered(spec, hp, wp, bank, 0, F, MUL_G, false, tmp, tmp2, resp, s);
-    fft_inverse_filtered(spec, hp, wp, bank, 0, F, MUL_G_IFX, false, tmp, tmp2, resp + F * np, s);
-    fft_inverse_filtered(spec, hp, wp, bank, 0, F, MUL_G_IFY, false, tmp, tmp2, resp + 2 * F * np, s);
-    for (int k = 0; k < F; k++)
-        refpack(resp + k * np, resp + (F + k) * np, resp + (2 * F + k) * np, refp + k * n, w, h, wp, s);
+    if (!cached) {
+        // reference pack of image 1 (phaseflow.py:100-111): three inverses per filter, each kind
+        // batched over the filters
+        fft_forward_padded(i1, w, h, spec, tmp, tmp2, s);
+        fft_inverse_filtered(spec, hp, wp, bank, 0, F, MUL_G, false, tmp, tmp2, resp, s);
+        fft_inverse_filtered(spec, hp, wp, bank, 0, F, MUL_G_IFX, false, tmp, tmp2, resp + F * np, s);
+        fft_inverse_filtered(spec, hp, wp, bank, 0, F, MUL_G_IFY, false, tmp, tmp2, resp + 2 * F * np, s);
+        for (int k = 0; k < F; k++)
+            refpack(resp + k * np, resp + (F + k) * np, resp + (2 * F + k) * np, refp + k * n, w, h, wp, s);
+        if (cache_i1) {
+            ws.pack_valid = true;
+            ws.pack_w = w;
+            ws.pack_h = h;
+            ws.pack_bank = bank;
+        }
+    }
     // spline of image 2
     ws.prof.mark(ST_WARP, s);
     const size_t n2 = (size_t)w2 * h2;
@@ -639,9 +662,10 @@ int tile_margin(const ofx_config &c) {
 // The fine stage of a pair whose dense start is in ws.bux / ws.buy (w x h): the whole frame, or
 // tiled when the frame has a side over c.tile (track.rs:580-598). Leaves the field in ws.bux /
 // ws.buy, energy and residual in ws.benergy / ws.bresid, the confidence in ws.bconf / ws.bconf64.
-void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, const ofx_config &c, hipStream_t s) {
+void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, const ofx_config &c, hipStream_t s,
+                bool cache_i1) {
     if (c.tile <= 0 || std::max(w, h) <= c.tile) {
-        refine_device(ws, i1, i2, w, h, w, h, 0, 0, c, s);
+        refine_device(ws, i1, i2, w, h, w, h, 0, 0, c, s, cache_i1);
         return;
     }
     const size_t n = (size_t)w * h;
@@ -750,7 +774,7 @@ void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, c
                 ew * sizeof(double), eh, ts_);
         dcopy2d(sy, ew * sizeof(double), fsy + (size_t)ext.y0 * w + ext.x0, w * sizeof(double),
                 ew * sizeof(double), eh, ts_);
-        refine_device(tw, t1, t2, ew, eh, ww, wh, ext.x0 - win.x0, ext.y0 - win.y0, c, ts_);
+        refine_device(tw, t1, t2, ew, eh, ww, wh, ext.x0 - win.x0, ext.y0 - win.y0, c, ts_, false);
         const int cw = core.x1 - core.x0;
         hipLaunchKernelGGL(k_write_core, dim3((cw + 255) / 256, core.y1 - core.y0), dim3(256), 0, ts_,
                            (const double *)tw.bux.p, (const double *)tw.buy.p, (const double *)tw.benergy.p,
@@ -876,7 +900,7 @@ void pair_device(Workspace &ws, const float *i1, const float *i2, int w, int h, 
     coarse_dense(i1, i2, w, h, c, init_x, init_y, ux, uy, ws.coarse, s);
     HIPCHECK(hipEventRecord(ws.ev[1], s));
     step_mark("pair: fine stage");
-    fine_stage(ws, i1, i2, w, h, c, s);
+    fine_stage(ws, i1, i2, w, h, c, s, false);
     step_mark("pair: guard and strain");
     HIPCHECK(hipEventRecord(ws.ev[2], s));
     ux = (double *)ws.bux.p;
@@ -1034,8 +1058,9 @@ struct Seq {
 
     // one leg a -> b from the start (ix, iy) or the coarse stage alone; its raw field and
     // confidence go to (ox, oy, oc); returns the coarse and fine seconds
+    // (cache_a: a is the reference frame, whose pack the workspace may keep between steps)
     std::pair<double, double> run_leg_to(const float *a, const float *b, const double *ix, const double *iy, double *ox,
-                                         double *oy, double *oc) {
+                                         double *oy, double *oc, bool cache_a = false) {
         for (auto &ev : ws.ev)
             if (!ev) HIPCHECK(hipEventCreate(&ev));
         HIPCHECK(hipEventRecord(ws.ev[0], s));
@@ -1045,7 +1070,7 @@ struct Seq {
         coarse_dense(a, b, w, h, c, ix, iy, ux, uy, ws.coarse, s);
         step_mark("sequence: fine stage of a leg");
         HIPCHECK(hipEventRecord(ws.ev[1], s));
-        fine_stage(ws, a, b, w, h, c, s);
+        fine_stage(ws, a, b, w, h, c, s, cache_a);
         HIPCHECK(hipEventRecord(ws.ev[2], s));
         dcopy(ox, ws.bux.p, n * sizeof(double), s);
         dcopy(oy, ws.buy.p, n * sizeof(double), s);
@@ -1059,7 +1084,7 @@ struct Seq {
         return run_leg_to((const float *)ws.bframe_ref.p, (const float *)ws.bi2.p,
                           with_init ? (const double *)ws.bprev_x.p : nullptr,
                           with_init ? (const double *)ws.bprev_y.p : nullptr, (double *)ws.bstart_x.p,
-                          (double *)ws.bstart_y.p, (double *)ws.bleg_c.p);
+                          (double *)ws.bstart_y.p, (double *)ws.bleg_c.p, true);
     }
     // mean link confidence of the direct leg over the reportable ROI pixels (track.rs:986-1023)
     double leg_mean() {
@@ -1079,6 +1104,7 @@ struct Seq {
         }
         float *fref = B<float>(ws.bframe_ref, n), *fprev = B<float>(ws.bframe_prev, n);
         (void)B<float>(ws.bi2, n);
+        ws.pack_valid = false;  // a new reference frame
         if (device_frame)
             dcopy(fref, frame0, n * sizeof(float), s);
         else
@@ -1156,6 +1182,7 @@ struct Seq {
             dcopy(u0x, cx, n * sizeof(double), s);
             dcopy(u0y, cy, n * sizeof(double), s);
             dcopy(fref, fprev, n * sizeof(float), s);
+            ws.pack_valid = false;  // the reference moved
             ref_idx = k - 1;
             tl = run_leg(false);
             cm = leg_mean();
@@ -1229,6 +1256,7 @@ struct Seq {
             dcopy(u0x, upx, n * sizeof(double), s);
             dcopy(u0y, upy, n * sizeof(double), s);
             dcopy(fref, fprev, n * sizeof(float), s);
+            ws.pack_valid = false;  // the reference moved
             ref_idx = k - 1;
             tl = run_leg(false);
             cm = leg_mean();
@@ -1485,7 +1513,7 @@ int ofx_refine_raw(ofx_engine *e, const float *i1, int w, int h, const float *i2
         copy_h2d(ws.pin, dy, uy, n * sizeof(float), s);
         to_f64(dx, B<double>(ws.bux, n), n, s);
         to_f64(dy, B<double>(ws.buy, n), n, s);
-        refine_device(ws, d1, d2, w, h, w2, h2, (long)shift_x, (long)shift_y, *cfg, s);
+        refine_device(ws, d1, d2, w, h, w2, h2, (long)shift_x, (long)shift_y, *cfg, s, false);
         if (out_ux) copy_d2h(ws.pin, out_ux, ws.bux.p, n * sizeof(double), s);
         if (out_uy) copy_d2h(ws.pin, out_uy, ws.buy.p, n * sizeof(double), s);
         if (out_energy) copy_d2h(ws.pin, out_energy, ws.benergy.p, n * sizeof(double), s);
