@@ -488,3 +488,32 @@ def test_hybrid_sequence_with_temporal_averaging(eng):
     # the oracle's own drift here is 0.145 px: the last frame's mean is a one-sided window, which
     # the dense alignment brings back only to within its own error
     assert drift < 0.3
+
+
+@pytest.mark.timeout(170)
+def test_batched_uneven_pairs_at_workload_sizes():
+    """The batched workload's five size classes (512^2, 768 x 1024, 1024^2, 1536^2, 1024 x 2048), one
+    pair each with its affine field and polygon ROI, through ofx_track_pairs on 4 lanes, against
+    the oracle pair by pair at the pair bar."""
+    import optflow_soil_amd as ofx
+    from optflow_soil_amd import synth
+    rng = np.random.default_rng(5)
+    pairs = []
+    for k, (h, w) in enumerate(synth.C4_SIZES):
+        pat = synth.make_pattern(h, w, seed=300 + k)
+        f = synth.affine_field(*rng.uniform(-0.02, 0.02, 3), rng.uniform(-1.0, 1.0), rng.uniform(-8.0, 8.0, 2),
+                               ((w - 1) / 2.0, (h - 1) / 2.0))
+        a = synth.as_float(synth.render(pat, None, noise=0.005, seed=2 * k))
+        b = synth.as_float(synth.render(pat, f, noise=0.005, seed=2 * k + 1))
+        roi = synth.convex_polygon_mask(h, w, seed=k, coverage=float(rng.uniform(0.4, 0.9)), search_step=8)
+        pairs.append((a, b, roi))
+    e4 = ofx.Engine(0, lanes=4)
+    try:
+        print("batched: the five size classes on 4 lanes", flush=True)
+        got = e4.track_pairs(pairs, Config())
+    finally:
+        e4.close()
+    for (a, b, roi), r in zip(pairs, got):
+        print(f"batched pair {a.shape}: oracle", flush=True)
+        o = O.track_pair(f64(a), f64(b), Config(), mask=roi)
+        compare_pair(r, o, f"batched {a.shape[1]}x{a.shape[0]}")
