@@ -69,7 +69,9 @@ typedef struct ofx_config {
     double fb_tol;            /* 0.5 */
     int prewarp_freeze_at;    /* 0: off */
     int tile;                 /* 0: whole frame; N >= 64: N x N tiles for frames with a side over N
-                                 (the reference's tiled fine stage with N = 1024, tile.rs:56-98) */
+                                 (the reference's tiled fine stage with N = 1024, tile.rs:56-98);
+                                 the tiles run in three groups side by side, on the lane's stream
+                                 and two more streams of the engine */
 } ofx_config;
 
 typedef struct ofx_engine ofx_engine;
