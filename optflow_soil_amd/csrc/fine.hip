@@ -185,10 +185,13 @@ __global__ void k_refpack(const float2 *__restrict__ r, const float2 *__restrict
 constexpr int ACC_T = 16;
 constexpr int ACC_BOX = 1600;  // nodes per staged box (40 x 40)
 
-// The reference amplitude sqrt(|r|^2 + 1e-12), which the reference stores as f32 (gabor.rs:260-269),
-// here in f32 arithmetic (within 2 ulp of that stored value); the weight is its product with the
-// sample's modulus, formed in f64 as the reference forms it (phaseflow.rs:1268-1277).
-__device__ __forceinline__ float amp_of(const float4 &p) { return sqrtf(fmaf(p.x, p.x, fmaf(p.y, p.y, 1e-12f))); }
+// The weight of a filter: the reference amplitude sqrt(|r|^2 + 1e-12) (stored as f32,
+// gabor.rs:260-269) times the sample's modulus |s| (phaseflow.rs:1268-1277). With re + i im =
+// r conj(s), re^2 + im^2 = |r|^2 |s|^2, so the product is one square root,
+// sqrt(re^2 + im^2 + 1e-12 |s|^2), in f32 (a few ulp from the product of the two f32 roots).
+__device__ __forceinline__ float weight_of(float re, float im, float s2) {
+    return sqrtf(fmaf(re, re, fmaf(im, im, 1e-12f * s2)));
+}
 
 // Workgroups are dealt to the chip's 8 XCDs in turn (dispatch index L runs on XCD L mod 8) and
 // each XCD has its own L2. Each row of tiles is split into 8 chunks of c = ceil(gx / 8) adjacent
@@ -329,7 +332,7 @@ __global__ void __launch_bounds__(256) k_accumulate_tiled(const double *__restri
                 const float re = p.x * r2x + p.y * r2y;
                 const float im = p.y * r2x - p.x * r2y;
                 const double dphi = (double)atan2f(im, re);
-                const double wk = (double)amp_of(p) * (double)sqrtf(fmaf(r2x, r2x, r2y * r2y));
+                const double wk = (double)weight_of(re, im, fmaf(r2x, r2x, r2y * r2y));
                 const double gx = p.z, gy = p.w;
                 const double c = dphi + gx * u + gy * v;
                 const double wgx = wk * gx, wgy = wk * gy;
@@ -358,7 +361,7 @@ __global__ void __launch_bounds__(256) k_accumulate_tiled(const double *__restri
             const float re = p.x * r2.x + p.y * r2.y;
             const float im = p.y * r2.x - p.x * r2.y;
             const double dphi = (double)atan2f(im, re);
-            const double wk = (double)amp_of(p) * (double)sqrtf(fmaf(r2.x, r2.x, r2.y * r2.y));
+            const double wk = (double)weight_of(re, im, fmaf(r2.x, r2.x, r2.y * r2.y));
             const double gx = p.z, gy = p.w;
             const double c = dphi + gx * u + gy * v;
             const double wgx = wk * gx, wgy = wk * gy;
