@@ -705,7 +705,16 @@ __global__ void __launch_bounds__(256) k_moments_x_solve_tiled(const double *__r
 // y pass: one thread per (column, 16 consecutive output rows, input plane); the 16 + 2R input rows
 // are read once straight from global memory (coalesced across the wavefront) and each is added
 // into every output row it reaches.
+//
+// The weights stay in scalar registers: the Gaussian taps k_t = kt.k[0][t] are symmetric
+// (k_t = k_{2R-t} exactly), so R + 1 values serve all 2R + 1 taps, and the weights k_t (t - R)^p of
+// the powers p = 1, 2 are not loaded at all. With a_j = j - R - C for input row j of the block and
+// c_q = q - C for output row q (C = 8), t - R = a_j - c_q, so with T_p = sum_j k_{j-q} a_j^p v_j,
+//   sum_t k_t (t - R) v = T1 - c_q T0,   sum_t k_t (t - R)^2 v = T2 - 2 c_q T1 + c_q^2 T0.
+// The row factors a_j^p multiply each input value once; the multiply-adds per output are as many
+// as with the three weight rows, which need 6R + 3 scalar register pairs and spill to vector lanes.
 constexpr int RB_ROWS = 16;
+constexpr int RB_C = 8;
 
 template <int R, int NP>
 __device__ __forceinline__ void y_block(const double *__restrict__ src, double *__restrict__ out, size_t n, int w,
@@ -719,12 +728,18 @@ __device__ __forceinline__ void y_block(const double *__restrict__ src, double *
     for (int j = 0; j < RB_ROWS + 2 * R; j++) {
         const int yy = min(max(y0 - R + j, 0), h - 1);
         const double v = src[(size_t)yy * w + x];
+        const double a = (double)(j - R - RB_C);
+        double vp[NP];
+        vp[0] = v;
+        if constexpr (NP > 1) vp[1] = a * v;
+        if constexpr (NP > 2) vp[2] = (a * a) * v;
 #pragma unroll
         for (int q = 0; q < RB_ROWS; q++) {
             const int t = j - q;  // tap index 0 .. 2R of output row y0 + q
             if (t >= 0 && t <= 2 * R) {
+                const double k = kt.k[0][t <= R ? t : 2 * R - t];
 #pragma unroll
-                for (int p = 0; p < NP; p++) acc[q][p] += kt.k[p][t] * v;
+                for (int p = 0; p < NP; p++) acc[q][p] = fma(k, vp[p], acc[q][p]);
             }
         }
     }
@@ -732,8 +747,11 @@ __device__ __forceinline__ void y_block(const double *__restrict__ src, double *
     for (int q = 0; q < RB_ROWS; q++) {
         const int y = y0 + q;
         if (y < h) {
-#pragma unroll
-            for (int p = 0; p < NP; p++) out[(size_t)p * n + (size_t)y * w + x] = acc[q][p];
+            const double c = (double)(q - RB_C);
+            const size_t i = (size_t)y * w + x;
+            out[i] = acc[q][0];
+            if constexpr (NP > 1) out[n + i] = fma(-c, acc[q][0], acc[q][1]);
+            if constexpr (NP > 2) out[2 * n + i] = fma(c * c, acc[q][0], fma(-2 * c, acc[q][1], acc[q][2]));
         }
     }
 }
