@@ -568,8 +568,15 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
             moments_solve(sums, ym, ux, uy, nux, nuy, energy, resid, slope, w, h, (const double *)ws.btaps_w.p,
                           ws.taps_w_host.data(), ws.taps_w_r, s);
             ws.prof.mark(ST_SOLVE, s);
-            median(energy, n, medscr, med, s);
-            energy_reset(energy, med, ux, uy, nux, nuy, resid, slope, n, s);
+            // the median itself is read only after an outer iteration's last inner one (the
+            // confidence); before that the threshold's bounds decide the reset where they can.
+            // OFX_EXACT_ENERGY_RESET set in the environment takes the exact median every time.
+            if (in + 1 == c.inner_iters || std::getenv("OFX_EXACT_ENERGY_RESET")) {
+                median(energy, n, medscr, med, s);
+                energy_reset(energy, med, ux, uy, nux, nuy, resid, slope, n, s);
+            } else {
+                median_energy_reset(energy, n, medscr, med, ux, uy, nux, nuy, resid, slope, s);
+            }
             std::swap(ux, nux);
             std::swap(uy, nuy);
         }

@@ -779,9 +779,9 @@ __global__ void __launch_bounds__(256) k_moments_y_rb(const double *__restrict__
 // input displacement and get resid = pi.
 __global__ void k_energy_reset(const double *__restrict__ energy, const double *__restrict__ med,
                                const double *__restrict__ ux_in, const double *__restrict__ uy_in, double *ux,
-                               double *uy, double *resid, double *slope, size_t n) {
+                               double *uy, double *resid, double *slope, size_t n, const unsigned int *gate) {
     const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= n || (gate && !*gate)) return;
     const double thr = 1e-9 * med[0] + 1e-30;
     if (!(energy[i] > thr)) {
         ux[i] = ux_in[i];
@@ -988,7 +988,7 @@ struct SelectState {
     unsigned long long prefix[9];  // key digits fixed before pass p (prefix[8]: the selected key)
     unsigned long long rank[9];    // rank left within them
     unsigned int nan_count;
-    unsigned int pad;
+    unsigned int undecided;          // bounded reset: some pixel needs the exact median
     unsigned long long below_count;  // elements with key < the selected key
     unsigned long long below_max;    // largest key below it
     unsigned int hist[8][256];       // one 8-bit digit histogram per pass
@@ -999,6 +999,7 @@ __global__ void k_select_init(SelectState *st, unsigned long long rank) {
         st->prefix[0] = 0;
         st->rank[0] = rank;
         st->nan_count = 0;
+        st->undecided = 0;
         st->below_count = 0;
         st->below_max = 0;
     }
@@ -1046,7 +1047,8 @@ __device__ __forceinline__ void select_pick(const SelectState *st, int p, unsign
 // Pass p (digit bits 63 - 8p .. 56 - 8p): every workgroup first derives the digit of pass p - 1
 // from its histogram (block 0 records it), then histograms the keys that share the prefix.
 __global__ void __launch_bounds__(256) k_select_hist(const double *__restrict__ v, size_t n, int p,
-                                                     SelectState *st) {
+                                                     SelectState *st, int gated) {
+    if (gated && !st->undecided) return;  // the bounded reset decided every pixel
     __shared__ unsigned int lh[256];
     lh[threadIdx.x] = 0;
     unsigned long long prefix = 0;
@@ -1080,7 +1082,9 @@ __device__ __forceinline__ unsigned long long select_key(SelectState *st) {
     return key;
 }
 
-__global__ void __launch_bounds__(256) k_select_below(const double *__restrict__ v, size_t n, SelectState *st) {
+__global__ void __launch_bounds__(256) k_select_below(const double *__restrict__ v, size_t n, SelectState *st,
+                                                      int gated) {
+    if (gated && !st->undecided) return;
     const unsigned long long key = select_key(st);
     unsigned long long cnt = 0, mx = 0;
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
@@ -1112,7 +1116,8 @@ __global__ void __launch_bounds__(256) k_select_below(const double *__restrict__
     }
 }
 
-__global__ void k_select_finish(SelectState *st, unsigned long long k, int even, double *out) {
+__global__ void k_select_finish(SelectState *st, unsigned long long k, int even, double *out, int gated) {
+    if (gated && !st->undecided) return;
     const unsigned long long key = select_key(st);
     if (threadIdx.x != 0) return;
     st->prefix[8] = key;
@@ -1255,7 +1260,7 @@ void moments_solve2(const double *sums, double *scratch, const double *ux_in, co
 void energy_reset(const double *energy, const double *med, const double *ux_in, const double *uy_in, double *ux,
                   double *uy, double *resid, double *slope, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(k_energy_reset, dim3((n + 255) / 256), dim3(256), 0, s, energy, med, ux_in, uy_in, ux, uy,
-                       resid, slope, n);
+                       resid, slope, n, (const unsigned int *)nullptr);
     LAUNCHCHECK();
 }
 
@@ -1270,9 +1275,75 @@ void median(const double *v, size_t n, void *scratch, double *out, hipStream_t s
     SelectState *st = (SelectState *)scratch;
     hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, k);
     const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
-    for (int p = 0; p < 8; p++) hipLaunchKernelGGL(k_select_hist, dim3(blocks), dim3(256), 0, s, v, n, p, st);
-    if (even) hipLaunchKernelGGL(k_select_below, dim3(blocks), dim3(256), 0, s, v, n, st);
-    hipLaunchKernelGGL(k_select_finish, dim3(1), dim3(64), 0, s, st, k, even, out);
+    for (int p = 0; p < 8; p++) hipLaunchKernelGGL(k_select_hist, dim3(blocks), dim3(256), 0, s, v, n, p, st, 0);
+    if (even) hipLaunchKernelGGL(k_select_below, dim3(blocks), dim3(256), 0, s, v, n, st, 0);
+    hipLaunchKernelGGL(k_select_finish, dim3(1), dim3(64), 0, s, st, k, even, out, 0);
+    LAUNCHCHECK();
+}
+
+// The energy threshold without the exact median where the pixels allow it. After two digit passes
+// the median's key shares 16 bits (sign, exponent and 4 mantissa bits) with [lo, hi], the keys of
+// its bucket, and thr(M) = 1e-9 M + 1e-30 is nondecreasing in M, so thr(lo) <= thr(M) <= thr(hi):
+// energies above thr(hi) keep the solve, energies not above thr(lo) (NaN included) are reset, as
+// with the exact median. Only an energy in (thr(lo), thr(hi)] needs M itself; such a pixel, a NaN
+// in the values, a NaN bound or (n even) a lower middle element outside the bucket sets
+// `undecided`, and the remaining passes, the exact median and the plain reset then run; otherwise
+// they return at once and `out` is not written.
+__global__ void __launch_bounds__(256) k_energy_reset_bounded(const double *__restrict__ energy, SelectState *st,
+                                                              int even, const double *__restrict__ ux_in,
+                                                              const double *__restrict__ uy_in, double *ux, double *uy,
+                                                              double *resid, double *slope, size_t n) {
+    __shared__ double thr_lo, thr_hi;
+    __shared__ int ok;
+    if (threadIdx.x < 64) {
+        unsigned long long prefix, rank;
+        select_pick(st, 2, prefix, rank);
+        if (threadIdx.x == 0) {
+            const double lo = from_key(prefix), hi = from_key(prefix | 0xFFFFFFFFFFFFull);
+            // rank: elements of the bucket below the upper middle one; with one or more, the lower
+            // middle element (n even) is in the bucket too
+            ok = st->nan_count == 0 && (!even || rank >= 1) && !isnan(lo) && !isnan(hi);
+            thr_lo = 1e-9 * lo + 1e-30;
+            thr_hi = 1e-9 * hi + 1e-30;
+        }
+    }
+    __syncthreads();
+    if (!ok) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) st->undecided = 1;
+        return;
+    }
+    const double tl = thr_lo, th = thr_hi;
+    bool open = false;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const double e = energy[i];
+        if (e > th) continue;
+        if (!(e > tl)) {
+            ux[i] = ux_in[i];
+            uy[i] = uy_in[i];
+            resid[i] = M_PI;
+            slope[4 * i] = slope[4 * i + 1] = slope[4 * i + 2] = slope[4 * i + 3] = 0;
+        } else {
+            open = true;
+        }
+    }
+    if (open) st->undecided = 1;
+}
+
+void median_energy_reset(const double *energy, size_t n, void *scratch, double *med, const double *ux_in,
+                         const double *uy_in, double *ux, double *uy, double *resid, double *slope, hipStream_t s) {
+    const unsigned long long k = n / 2;
+    const int even = (n % 2 == 0) ? 1 : 0;
+    SelectState *st = (SelectState *)scratch;
+    hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, k);
+    const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
+    for (int p = 0; p < 2; p++) hipLaunchKernelGGL(k_select_hist, dim3(blocks), dim3(256), 0, s, energy, n, p, st, 0);
+    hipLaunchKernelGGL(k_energy_reset_bounded, dim3(blocks), dim3(256), 0, s, energy, st, even, ux_in, uy_in, ux, uy,
+                       resid, slope, n);
+    for (int p = 2; p < 8; p++) hipLaunchKernelGGL(k_select_hist, dim3(blocks), dim3(256), 0, s, energy, n, p, st, 1);
+    if (even) hipLaunchKernelGGL(k_select_below, dim3(blocks), dim3(256), 0, s, energy, n, st, 1);
+    hipLaunchKernelGGL(k_select_finish, dim3(1), dim3(64), 0, s, st, k, even, med, 1);
+    hipLaunchKernelGGL(k_energy_reset, dim3((n + 255) / 256), dim3(256), 0, s, energy, (const double *)med, ux_in,
+                       uy_in, ux, uy, resid, slope, n, (const unsigned int *)&st->undecided);
     LAUNCHCHECK();
 }
 

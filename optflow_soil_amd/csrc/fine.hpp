@@ -29,6 +29,11 @@ void energy_reset(const double *energy, const double *med, const double *ux_in, 
                   double *uy, double *resid, double *slope, size_t n, hipStream_t s);
 size_t median_scratch_bytes();
 void median(const double *v, size_t n, void *scratch, double *out, hipStream_t s);
+// energy_reset with the threshold from two digit passes where every pixel is decided by them; the
+// exact median (written to med) and the plain reset run only when some pixel is not. The results
+// equal median + energy_reset; med is left as it was when the exact median was not needed.
+void median_energy_reset(const double *energy, size_t n, void *scratch, double *med, const double *ux_in,
+                         const double *uy_in, double *ux, double *uy, double *resid, double *slope, hipStream_t s);
 void confidence(const double *energy, const double *resid, const double *med, const double *ux, const double *uy,
                 double phase_tol, double conf_min, double B, int w, int h, uint8_t *low, float *conf_out,
                 double *conf64, hipStream_t s);

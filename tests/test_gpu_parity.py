@@ -543,3 +543,29 @@ def test_batched_tiled_pairs_match_single_tiled_pairs(eng):
         one = eng.track_pair(p[0], p[1], cfg)
         for name in ("ux", "uy", "conf", "raw_ux", "raw_uy"):
             np.testing.assert_array_equal(getattr(one, name), getattr(r, name), err_msg=f"pair {k} {name}")
+
+
+@pytest.mark.parametrize("flat", [False, True])
+def test_bounded_energy_reset_matches_the_exact_median(eng, flat, monkeypatch):
+    """The first inner iteration of each outer one resets low-energy pixels from bounds on the
+    median (two digit passes) and takes the exact median only for pixels those bounds leave open;
+    OFX_EXACT_ENERGY_RESET forces the exact median. Both give bit-identical results, on a textured
+    pair and on one with flat blocks (energy near zero: reset pixels next to kept ones)."""
+    from optflow_soil_amd import synth
+    h, w = 300, 340
+    pat = synth.make_pattern(h, w, seed=77)
+    f = synth.affine_field(0.003, -0.002, 0.001, 0.2, (1.3, -0.7), ((w - 1) / 2.0, (h - 1) / 2.0))
+    a = synth.as_float(synth.render(pat))
+    b = synth.as_float(synth.render(pat, f, noise=0.005, seed=3))
+    if flat:
+        for arr in (a, b):
+            arr[40:120, 60:150] = 0.5
+            arr[200:260, 220:330] = 0.25
+    cfg = Config()
+    monkeypatch.delenv("OFX_EXACT_ENERGY_RESET", raising=False)
+    print(f"bounded energy reset: flat={flat}", flush=True)
+    got = eng.track_pair(a, b, cfg)
+    monkeypatch.setenv("OFX_EXACT_ENERGY_RESET", "1")
+    ref = eng.track_pair(a, b, cfg)
+    for name in ("ux", "uy", "conf", "raw_ux", "raw_uy"):
+        np.testing.assert_array_equal(getattr(got, name), getattr(ref, name), err_msg=name)
