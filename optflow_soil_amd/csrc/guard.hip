@@ -86,6 +86,33 @@ __global__ void k_within(const int *__restrict__ col_d2, int w, int h, double th
     out[(size_t)y * w + x] = best >= 0 && sqrt((double)best) <= thr;
 }
 
+// The same test with the row segment a workgroup reaches (its 256 pixels and T on each side)
+// staged in LDS once, instead of 2T + 1 global loads per pixel; positions outside the row read as
+// -1 (no column point), which the loop skips as k_within skips them. For T <= WITHIN_MAXT.
+constexpr int WITHIN_MAXT = 256;
+__global__ void __launch_bounds__(256) k_within_lds(const int *__restrict__ col_d2, int w, double thr, int T,
+                                                    uint8_t *__restrict__ out) {
+    __shared__ int seg[256 + 2 * WITHIN_MAXT];
+    const int x0 = blockIdx.x * 256, y = blockIdx.y;
+    const int *row = col_d2 + (size_t)y * w;
+    for (int c = threadIdx.x; c < 256 + 2 * T; c += 256) {
+        const int xx = x0 - T + c;
+        seg[c] = (xx >= 0 && xx < w) ? row[xx] : -1;
+    }
+    __syncthreads();
+    const int x = x0 + threadIdx.x;
+    if (x >= w) return;
+    const int *g0 = seg + threadIdx.x + T;  // g0[dx] is column x + dx
+    long long best = -1;
+    for (int dx = -T; dx <= T; dx++) {
+        const int g = g0[dx];
+        if (g < 0) continue;
+        const long long d2 = (long long)dx * dx + g;
+        if (best < 0 || d2 < best) best = d2;
+    }
+    out[(size_t)y * w + x] = best >= 0 && sqrt((double)best) <= thr;
+}
+
 __global__ void k_zero_counters(Counters *cnt) {
     cnt->n_low = 0;
     cnt->n_ok = 0;
@@ -457,7 +484,10 @@ size_t column_summary_ints(int w, int h) { return 2 * (size_t)((h + CHUNK - 1) /
 
 void within_distance(const int *col_d2, int w, int h, double thr, uint8_t *within, hipStream_t s) {
     const int T = thr >= 0 ? (int)std::floor(thr) : -1;
-    hipLaunchKernelGGL(k_within, grid2(w, h), dim3(256), 0, s, col_d2, w, h, thr, T, within);
+    if (T <= WITHIN_MAXT)
+        hipLaunchKernelGGL(k_within_lds, dim3((w + 255) / 256, h), dim3(256), 0, s, col_d2, w, thr, T, within);
+    else
+        hipLaunchKernelGGL(k_within, grid2(w, h), dim3(256), 0, s, col_d2, w, h, thr, T, within);
     LAUNCHCHECK();
 }
 
