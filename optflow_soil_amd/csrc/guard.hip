@@ -393,13 +393,29 @@ __device__ bool solve6(double m[6][6], double rhs[6][2], double c[6][2]) {
     return true;
 }
 
+// LDS = true: the 11 planes' row segments this workgroup reaches (its 256 columns and r on each
+// side) are staged in LDS once, and the tap loop reads them there instead of making 11 (2r + 1)
+// global loads per pixel (same values, same order of sums).
+template <bool LDS>
 __global__ void __launch_bounds__(256) k_strain_x(const double *__restrict__ ym, const double *__restrict__ mx,
                                                   const double *__restrict__ my, int w, int h, int r, int win,
                                                   float *ux_out, float *uy_out, uint8_t *valid, float *exx, float *eyy,
                                                   float *exy, float *gmax, float *vol) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x >= w) return;
     const size_t n = (size_t)w * h;
+    extern __shared__ double seg[];  // LDS: 11 planes x (256 + 2r)
+    const int cols = 256 + 2 * r;
+    if constexpr (LDS) {
+        const int x0 = blockIdx.x * blockDim.x;
+        for (int idx = threadIdx.x; idx < 11 * cols; idx += blockDim.x) {
+            const int q = idx / cols, c = idx - q * cols;
+            const int xx = x0 - r + c;
+            seg[idx] = (xx >= 0 && xx < w) ? ym[(size_t)q * n + (size_t)y * w + xx] : 0.0;
+        }
+        __syncthreads();
+    }
+    if (x >= w) return;
+    const double *s0 = seg + threadIdx.x + r;  // LDS: s0[q * cols + j] is plane q at column x + j
     double mm[5][5], mu[3][3], mv[3][3];
 #pragma unroll
     for (int a = 0; a < 5; a++)
@@ -417,13 +433,14 @@ __global__ void __launch_bounds__(256) k_strain_x(const double *__restrict__ ym,
         const double tp[5] = {1.0, t, t2, t2 * t, t2 * t2};
 #pragma unroll
         for (int b = 0; b < 5; b++) {
-            const double v = row[b * n + xx];
+            const double v = LDS ? s0[b * cols + j] : row[b * n + xx];
 #pragma unroll
             for (int a = 0; a + b < 5; a++) mm[a][b] += tp[a] * v;
         }
 #pragma unroll
         for (int b = 0; b < 3; b++) {
-            const double vu = row[(5 + b) * n + xx], vv = row[(8 + b) * n + xx];
+            const double vu = LDS ? s0[(5 + b) * cols + j] : row[(5 + b) * n + xx];
+            const double vv = LDS ? s0[(8 + b) * cols + j] : row[(8 + b) * n + xx];
 #pragma unroll
             for (int a = 0; a + b < 3; a++) {
                 mu[a][b] += tp[a] * vu;
@@ -540,8 +557,13 @@ void strain(const double *mx, const double *my, int w, int h, int win, double *y
             uint8_t *valid, float *exx, float *eyy, float *exy, float *gmax, float *vol, hipStream_t s) {
     const int r = win / 2;
     hipLaunchKernelGGL(k_strain_y, grid2(w, h), dim3(256), 0, s, mx, my, w, h, r, ytmp);
-    hipLaunchKernelGGL(k_strain_x, grid2(w, h), dim3(256), 0, s, ytmp, mx, my, w, h, r, win, ux_out, uy_out, valid,
-                       exx, eyy, exy, gmax, vol);
+    const size_t lds = (size_t)11 * (256 + 2 * r) * sizeof(double);
+    if (lds <= 65536)
+        hipLaunchKernelGGL(k_strain_x<true>, grid2(w, h), dim3(256), lds, s, ytmp, mx, my, w, h, r, win, ux_out, uy_out,
+                           valid, exx, eyy, exy, gmax, vol);
+    else
+        hipLaunchKernelGGL(k_strain_x<false>, grid2(w, h), dim3(256), 0, s, ytmp, mx, my, w, h, r, win, ux_out, uy_out,
+                           valid, exx, eyy, exy, gmax, vol);
     LAUNCHCHECK();
 }
 
