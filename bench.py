@@ -10,6 +10,7 @@ synthetic truth and a CPU baseline (the FP64 oracle on a 1024 x 1024 crop of the
 the host's cores).
 """
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -170,9 +171,10 @@ def main():
             traffic = json.load(fh).get("hbm_bytes_per_launch")
     stages = {k: round(v[0] / args.steps, 3) for k, v in prof.items() if v[1]}
 
-    cpu = None
+    cpu = err = None
     if not args.no_cpu:
-        cpu = cpu_baseline(a, b, roi, cfg)
+        cpu, crop_info = cpu_baseline(a, b, roi, cfg)
+        err = err_vs_oracle(eng, a, b, crop_info)
 
     line = {
         "metric": metric, "value": round(value, 3), "unit": "Mpix/s", "n_gpus": 1, "steps": args.steps,
@@ -183,7 +185,7 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": acc_bytes, "avg_launch_ms": round(acc_avg_s * 1e3, 4)},
         "stage_ms_per_step": stages,
-        "epe_px": round(epe, 5), "valid_frac_roi": round(valid_frac, 5),
+        "epe_px": round(epe, 5), "valid_frac_roi": round(valid_frac, 5), "err_vs_oracle": err,
         "pipeline": {"bound": "hbm", "bytes_per_step": pipeline_bytes(w, h, cfg.tile),
                      "achieved": round(pipeline_bytes(w, h, cfg.tile) / dt / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s",
@@ -364,23 +366,75 @@ def bench_c4(args):
     print(json.dumps(line))
 
 
-def cpu_baseline(a, b, roi, cfg):
-    """The FP64 oracle (C++/OpenMP restatement of the reference CPU path) on the central 1024 x 1024
-    crop of the same pair, all host cores up to 16, one timed run."""
+def host_threads():
+    """Threads the CPU leg may use: OMP_NUM_THREADS when set (the host's CPU share for this
+    process), otherwise every core this process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(a, b, roi, cfg, crop=1024, runs=3):
+    """The FP64 oracle (this repository's C++/OpenMP restatement of the reference CPU path: the
+    same algorithm and constants, f64 storage and arithmetic) on the central crop x crop window of
+    the same pair, tiled as the reference CLI tiles (Config.tile = 1024: a 1024 x 1024 crop is one
+    of the four tiles of the 2048 x 2048 frame), one untimed warm-up, then the minimum of `runs`
+    timed runs, as the reference's bench_pair (examples/bench_pair.rs:12-17). Returns the
+    baseline record and the oracle's result of the crop (for the error of the GPU against it)."""
     import oracle
     from optflow_soil_amd import synth
-    cores = min(os.cpu_count() or 1, 16)
+    cores = host_threads()
     oracle.set_threads(cores)
-    c = 1024
-    y0, x0 = (a.shape[0] - c) // 2, (a.shape[1] - c) // 2
-    sl = np.s_[y0:y0 + c, x0:x0 + c]
+    y0, x0 = (a.shape[0] - crop) // 2, (a.shape[1] - crop) // 2
+    sl = np.s_[y0:y0 + crop, x0:x0 + crop]
     A = synth.as_float(a[sl]).astype(np.float64)
     Bm = synth.as_float(b[sl]).astype(np.float64)
-    t = time.perf_counter()
-    oracle.track_pair(A, Bm, cfg, mask=None if roi is None else roi[sl])
-    dt = time.perf_counter() - t
-    return {"value": round(c * c / 1e6 / dt, 4), "unit": "Mpix/s", "cores": cores, "kind": "port",
-            "sample": f"central {c}x{c} crop of the benchmark pair (same ROI), one pair, {dt:.2f} s"}
+    m = None if roi is None else np.ascontiguousarray(roi[sl])
+    ccfg = dataclasses.replace(cfg, tile=1024)
+    times = []
+    o = None
+    for k in range(runs + 1):
+        t = time.perf_counter()
+        o = oracle.track_pair(A, Bm, ccfg, mask=m)
+        dt = time.perf_counter() - t
+        if k:
+            times.append(dt)
+        print(f"# cpu baseline run {k}/{runs}{' (warm-up)' if k == 0 else ''}: {dt:.2f} s", file=sys.stderr,
+              flush=True)
+    best = min(times)
+    rec = {"value": round(crop * crop / 1e6 / best, 4), "unit": "Mpix/s", "cores": cores, "kind": "port",
+           "host_cpus": os.cpu_count(),
+           "sample": (f"central {crop}x{crop} crop of the benchmark pair (same ROI; one tile of the reference "
+                      f"CLI's 1024 px tiling), FP64 oracle, {cores} OpenMP threads, min of {runs} runs after a "
+                      f"warm-up: " + ", ".join(f"{t:.2f}" for t in times) + " s"),
+           "published_reference_cpu": "0.49-0.67 Mpix/s, Rust CPU path on 10 Apple M4 threads (rust/README.md:166-171)"}
+    return rec, (sl, m, ccfg, o)
+
+
+def err_vs_oracle(eng, a, b, crop_info):
+    """The GPU's result on the CPU leg's crop against the oracle's: ux / uy RMS and largest
+    difference over pixels valid on both sides, valid-mask agreement over the crop's ROI pixels,
+    and the strain planes' RMS."""
+    from optflow_soil_amd import synth
+    sl, m, ccfg, o = crop_info
+    r = eng.track_pair(synth.as_float(a[sl]), synth.as_float(b[sl]), ccfg, mask=m)
+    vg, vo = np.isfinite(r.ux), np.isfinite(o["ux"])
+    both = vg & vo
+    region = np.ones(vg.shape, bool) if m is None else m.astype(bool)
+    dx = r.ux.astype(np.float64)[both] - o["ux"][both]
+    dy = r.uy.astype(np.float64)[both] - o["uy"][both]
+    fin = np.isfinite(r.strain["Exx"]) & np.isfinite(o["Exx"])
+    out = {"sample": "the CPU leg's crop, same settings (tiled at 1024)",
+           "ux_rms": float(np.sqrt(np.mean(dx ** 2))), "uy_rms": float(np.sqrt(np.mean(dy ** 2))),
+           "disp_max": float(max(np.abs(dx).max(), np.abs(dy).max())),
+           "valid_agree_roi": float((vg == vo)[region].mean())}
+    for p in ("Exx", "Eyy", "Exy", "gmax", "vol"):
+        out[p + "_rms"] = float(np.sqrt(np.mean((r.strain[p].astype(np.float64)[fin] - o[p][fin]) ** 2)))
+    return {k: (float(f"{v:.3g}") if isinstance(v, float) else v) for k, v in out.items()}
 
 
 if __name__ == "__main__":

@@ -21,6 +21,7 @@
 #ifndef OFX_H
 #define OFX_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus

@@ -398,13 +398,30 @@ void validate(const ofx_config *cfg) {
                     "poc_win <= 1024 required");
 }
 
+// tile.rs:80-95
+int tile_margin(const ofx_config &c) {
+    return std::max({64, (int)std::ceil(4.0 * c.sigma_w), (int)std::ceil(4.0 * c.sigma_s),
+                     (int)std::ceil(std::max(edge_guard(c), 0.0)), c.strain_win});
+}
+
+// The largest window the fine stage refines for a w x h frame must fit the line transforms (one
+// padded line per workgroup's LDS): the whole frame, or with tiling (c.tile > 0) a tile's
+// extension, at most c.tile + 2 * tile_margin a side (the reference checks
+// check_window(min(w, side), min(h, side)), track.rs:754-762). The other stages have no line
+// limit; a tile's window of image 2 is checked against MAX_I2_SIDE when the tiles are laid out.
 void check_window_impl(int w, int h, const ofx_config &c) {
     if (w < 1 || h < 1) throw Error(OFX_EINVAL, "empty window");
     if (w <= PAD || h <= PAD) throw Error(OFX_EINVAL, "image smaller than the 32 px reflect pad");
-    if (w + 2 * PAD > 4096 || h + 2 * PAD > 4096)
-        throw Error(OFX_ETOOLARGE, "window " + std::to_string(w) + "x" + std::to_string(h) +
-                                       " exceeds 4032 px per side (one padded line must fit a workgroup's LDS)");
-    (void)c;
+    int fw = w, fh = h;
+    if (c.tile > 0) {
+        const int side = c.tile + 2 * tile_margin(c);
+        fw = std::min(w, side);
+        fh = std::min(h, side);
+    }
+    if (fw + 2 * PAD > 4096 || fh + 2 * PAD > 4096)
+        throw Error(OFX_ETOOLARGE, "window " + std::to_string(fw) + "x" + std::to_string(fh) +
+                                       " exceeds 4032 px per side (one padded line must fit a workgroup's LDS)" +
+                                       (c.tile > 0 ? "" : "; a tile size refines larger frames tile by tile"));
 }
 
 template <class T>
@@ -660,12 +677,6 @@ __global__ void k_write_core(const double *__restrict__ tux, const double *__res
 // with the stream of the caller keeps to the four hardware queues a process gets by default.
 constexpr int TILE_GROUPS = 3;
 
-// tile.rs:80-95
-int tile_margin(const ofx_config &c) {
-    return std::max({64, (int)std::ceil(4.0 * c.sigma_w), (int)std::ceil(4.0 * c.sigma_s),
-                     (int)std::ceil(std::max(edge_guard(c), 0.0)), c.strain_win});
-}
-
 // The fine stage of a pair whose dense start is in ws.bux / ws.buy (w x h): the whole frame, or
 // tiled when the frame has a side over c.tile (track.rs:580-598). Leaves the field in ws.bux /
 // ws.buy, energy and residual in ws.benergy / ws.bresid, the confidence in ws.bconf / ws.bconf64.
@@ -739,6 +750,11 @@ void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, c
         };
         span(ext.x0, ext.x1, lox, hix, w, wins[k].x0, wins[k].x1);
         span(ext.y0, ext.y1, loy, hiy, h, wins[k].y0, wins[k].y1);
+        const int ww = wins[k].x1 - wins[k].x0, wh = wins[k].y1 - wins[k].y0;
+        if (ww > MAX_I2_SIDE || wh > MAX_I2_SIDE)
+            throw Error(OFX_ETOOLARGE, "the image 2 window of a tile, " + std::to_string(ww) + "x" + std::to_string(wh) +
+                                           ", exceeds the " + std::to_string(MAX_I2_SIDE) +
+                                           " px side the spline rows hold (the start field spans too far)");
     }
     // The tiles are independent refines (tile.rs:225-330): they run in TILE_GROUPS groups, the
     // group g taking the tiles order[g], order[g + G], ... (largest first), group 0 on the lane's
@@ -1613,6 +1629,11 @@ int ofx_tavg_window(ofx_engine *e, const float *frames, int nf, int centre, int 
             if (!cfg) throw Error(OFX_EINVAL, "aligning needs a configuration");
             validate(cfg);
             if (std::min(w, h) >= cfg->poc_win) check_window_impl(w, h, *cfg);
+            // the aligned frames are resampled through a B-spline of the whole frame
+            if (w > MAX_I2_SIDE || h > MAX_I2_SIDE)
+                throw Error(OFX_ETOOLARGE, "aligning frames of " + std::to_string(w) + "x" + std::to_string(h) +
+                                               " exceeds the " + std::to_string(MAX_I2_SIDE) +
+                                               " px side the spline rows hold");
         }
         LaneGuard g(e);
         Workspace &ws = *g.lane->ws;

@@ -35,23 +35,42 @@ def f64(a):
     return np.asarray(a, dtype=np.float32).astype(np.float64)
 
 
-def compare_pair(r, o, label):
+STRAIN_PLANES = ("Exx", "Eyy", "Exy", "gmax", "vol")
+# ux / uy of pixels valid on both sides: RMS and the 99.99th percentile of |GPU - oracle| within
+# the displacement bar, and no pixel further off than DISP_MAX_TOL (a pixel whose solve is
+# near-degenerate amplifies the float-vs-double difference of its sums).
+DISP_P9999_TOL = 1e-3
+DISP_MAX_TOL = 1e-2
+
+
+def compare_pair(r, o, label, roi=None):
+    """Every output plane of a pair against the oracle. Valid-mask agreement is taken over ROI
+    pixels when an ROI is given (outside it both sides are NaN by construction)."""
     valid_g, valid_o = np.isfinite(r.ux), np.isfinite(o["ux"])
-    agree = float((valid_g == valid_o).mean())
+    region = np.ones(valid_g.shape, bool) if roi is None else np.asarray(roi, bool)
+    agree = float((valid_g == valid_o)[region].mean())
     both = valid_g & valid_o
     d = {"agree": agree, "n_valid": int(both.sum()),
          "ux": rms(r.ux.astype(np.float64), o["ux"], both), "uy": rms(r.uy.astype(np.float64), o["uy"], both),
          "conf": rms(r.conf.astype(np.float64), o["conf"], np.ones(both.shape, bool))}
+    if both.any():
+        dev = np.maximum(np.abs(r.ux.astype(np.float64) - o["ux"])[both], np.abs(r.uy.astype(np.float64) - o["uy"])[both])
+        d["disp_p9999"] = float(np.quantile(dev, 0.9999))
+        d["disp_max"] = float(dev.max())
+    else:
+        d["disp_p9999"] = d["disp_max"] = 0.0
     # strain is NaN wherever fewer than half of the window's pixels are valid; a frame with
     # too few valid pixels has no strain to compare, and then both sides must say so
     fin = np.isfinite(r.strain["Exx"]) & np.isfinite(o["Exx"])
-    d["strain_agree"] = float((np.isfinite(r.strain["Exx"]) == np.isfinite(o["Exx"])).mean())
-    d["Exx"] = rms(r.strain["Exx"].astype(np.float64), o["Exx"], fin) if fin.any() else 0.0
-    d["gmax"] = rms(r.strain["gmax"].astype(np.float64), o["gmax"], fin) if fin.any() else 0.0
+    d["strain_agree"] = float((np.isfinite(r.strain["Exx"]) == np.isfinite(o["Exx"]))[region].mean())
+    for p in STRAIN_PLANES:
+        d[p] = rms(r.strain[p].astype(np.float64), o[p], fin) if fin.any() else 0.0
     print(label, " ".join(f"{k}={v:.3g}" for k, v in d.items()), flush=True)
     assert agree >= VALID_AGREE and d["strain_agree"] >= VALID_AGREE
     assert d["ux"] <= DISP_TOL and d["uy"] <= DISP_TOL
-    assert d["Exx"] <= STRAIN_TOL and d["gmax"] <= STRAIN_TOL
+    assert d["disp_p9999"] <= DISP_P9999_TOL and d["disp_max"] <= DISP_MAX_TOL
+    for p in STRAIN_PLANES:
+        assert d[p] <= STRAIN_TOL, p
     assert d["conf"] <= CONF_TOL
     return d
 
@@ -155,7 +174,7 @@ def test_pair_fixture(eng):
     a, b, roi = load("pair_a"), load("pair_b"), load("pair_roi") > 0.5
     r = eng.track_pair(f32(a), f32(b), Config(), mask=roi)
     o = O.track_pair(f64(a), f64(b), Config(), mask=roi)
-    compare_pair(r, o, "pair")
+    compare_pair(r, o, "pair", roi=roi)
     # and against the reference's own fixture
     assert (np.isfinite(r.ux) == np.isfinite(load("pair_ux"))).mean() >= VALID_AGREE
     assert rms(r.ux.astype(np.float64), load("pair_ux")) <= DISP_TOL
@@ -317,6 +336,28 @@ def test_tiled_pair(eng):
     assert np.abs(whole.ux - r.ux)[fin].max() < 1e-3 and np.abs(whole.uy - r.uy)[fin].max() < 1e-3
 
 
+def test_tiled_pair_wider_than_one_line_transform(eng):
+    """A frame with a side over 4032 px runs when tiled: the line limit applies to the largest
+    window the fine stage refines, a tile's extension (the reference's check_window(min(w, side),
+    min(h, side)), track.rs:754-762), not to the whole frame. 4160 x 192 in tiles of 1024 against
+    the oracle's tiled pair; untiled, the same frame is refused as too large before any work."""
+    from optflow_soil_amd import synth
+    h, w = 192, 4160
+    pat = synth.make_pattern(h, w, seed=11)
+    f = synth.stretch_translate(0.004, (1.2, -0.6), (w / 2, h / 2))
+    a = synth.as_float(synth.render(pat))
+    b = synth.as_float(synth.render(pat, f, noise=0.005, seed=5))
+    assert eng.check_window(w, h, Config(tile=1024)) is None
+    with pytest.raises(ValueError, match="exceeds 4032 px"):
+        eng.check_window(w, h, Config())
+    with pytest.raises(ValueError, match="exceeds 4032 px"):
+        eng.track_pair(a, b, Config())
+    print(f"tiled pair {w}x{h}, tile 1024", flush=True)
+    r = eng.track_pair(a, b, Config(tile=1024))
+    o = O.track_pair(f64(a), f64(b), Config(tile=1024))
+    compare_pair(r, o, f"tiled pair {w}x{h}")
+
+
 @pytest.mark.parametrize("order", [1, 2])
 def test_refine_stripe_runs_the_degenerate_paths(eng, order):
     """The reference's stripe fixture (texture along one axis only): the aperture test marks the
@@ -385,7 +426,7 @@ def test_pair_settings(eng, variant):
     print(f"settings {variant}: {h}x{w}", flush=True)
     r = eng.track_pair(a, b, cfg, mask=roi)
     o = O.track_pair(f64(a), f64(b), cfg, mask=roi)
-    compare_pair(r, o, f"settings {variant}")
+    compare_pair(r, o, f"settings {variant}", roi=roi)
 
 
 def test_engine_errors_are_the_reference_messages(eng):
@@ -423,7 +464,7 @@ def test_benchmark_pairs_at_full_size(eng, workload):
     r = eng.track_pair(A, Bm, cfg, mask=roi)
     print(f"{workload}: GPU done; oracle", flush=True)
     o = O.track_pair(f64(A), f64(Bm), cfg, mask=roi)
-    compare_pair(r, o, workload)
+    compare_pair(r, o, workload, roi=roi)
     for name, ux, uy in (("gpu", r.ux.astype(np.float64), r.uy.astype(np.float64)), ("oracle", o["ux"], o["uy"])):
         fin = np.isfinite(ux)
         epe = float(np.sqrt(np.mean((ux[fin] - tx[fin]) ** 2 + (uy[fin] - ty[fin]) ** 2)))
@@ -517,7 +558,7 @@ def test_batched_uneven_pairs_at_workload_sizes():
     for (a, b, roi), r in zip(pairs, got):
         print(f"batched pair {a.shape}: oracle", flush=True)
         o = O.track_pair(f64(a), f64(b), Config(), mask=roi)
-        compare_pair(r, o, f"batched {a.shape[1]}x{a.shape[0]}")
+        compare_pair(r, o, f"batched {a.shape[1]}x{a.shape[0]}", roi=roi)
 
 
 def test_batched_tiled_pairs_match_single_tiled_pairs(eng):
