@@ -199,6 +199,15 @@ int ofx_tavg_window(ofx_engine *e, const float *frames, int nf, int centre, int 
  * total order keeps it. */
 int ofx_median(ofx_engine *e, const double *values, size_t n, double *out);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): ofx_median of n
+ * doubles already in device memory (d_values), the result written to d_out (one double, device
+ * memory). The selection's histograms live in a fixed array of the library (16 KiB per GPU, held
+ * until the process ends), so the call allocates nothing. At most 4294967295 values (the
+ * histogram bins count in 32 bits; OFX_ETOOLARGE above). Every call on a GPU shares that GPU's
+ * array: calls that overlap on one GPU race, so run them one at a time per GPU. */
+int ofx_median_device_noalloc(ofx_engine *e, const double *d_values, size_t n, double *d_out);
+
 /* Frame conversion (rust/crates/optflow-cli/src/io.rs:44-52, :259-281; python/optflow_ref/io.py:15-54):
  * pixels (h x w x channels, channels 1, 3 or 4, 8 or 16 bit, host) -> grey levels in [0, 1]
  * (v / 255 or v / 65535; colour 0.299 R + 0.587 G + 0.114 B), box downscaled by factor (the

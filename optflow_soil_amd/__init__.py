@@ -22,7 +22,7 @@ OFX_OK, OFX_EINVAL, OFX_ETOOLARGE, OFX_EDEVICE, OFX_ENODEVICE, OFX_EBUSY, OFX_EU
 SYMBOLS = ("ofx_config_default", "ofx_create", "ofx_destroy", "ofx_last_error", "ofx_lanes", "ofx_returns_slope",
            "ofx_check_window", "ofx_release_caches", "ofx_refine_raw", "ofx_coarse_flow", "ofx_track_pair",
            "ofx_track_pair_device", "ofx_track_pairs", "ofx_seq_begin", "ofx_seq_step", "ofx_seq_step_device", "ofx_seq_end",
-           "ofx_tavg_window", "ofx_frame_to_gray", "ofx_median",
+           "ofx_tavg_window", "ofx_frame_to_gray", "ofx_median", "ofx_median_device_noalloc",
            "ofx_track_sequence", "ofx_profile", "ofx_profile_read", "ofx_profile_stage")
 
 
@@ -346,6 +346,13 @@ class Engine:
         _check(lib().ofx_median(self._h, a.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(a.size),
                                 ctypes.byref(out)))
         return out.value
+
+    def median_device_noalloc(self, d_values, n, d_out):
+        """ofx_median_device_noalloc: the median of n float64 values at device address d_values
+        written to the device address d_out, with no device allocation (integers from e.g.
+        torch's data_ptr())."""
+        _check(lib().ofx_median_device_noalloc(self._h, ctypes.c_void_p(d_values or None), ctypes.c_size_t(int(n)),
+                                               ctypes.c_void_p(d_out or None)))
 
     def sequence(self, frame0, cfg, mask=None):
         """Opens a streaming sequence on frame 0 (the Sequence iterator of track.rs:1112-1225): a

@@ -1375,6 +1375,44 @@ struct Seq {
 
 using namespace ofx;
 
+// ---------------------------------------------------------------- allocation-free entry point
+// The scratch of ofx_median_device_noalloc: one fixed array per GPU, in the library's code object,
+// held until the process ends. Host code never uses its name as an address (on the host it names
+// a stand-in, not device memory): each device's address comes from hipGetSymbolAddress, taken once
+// per device at set-up (ofx_create) with that device current, and only that address reaches a
+// kernel. Every call on a GPU shares its array, so calls that overlap on one GPU race.
+constexpr size_t NOALLOC_SCRATCH_BYTES = 16384;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_noalloc_mu;
+std::vector<void *> g_noalloc_ptr;  // per device; nullptr until that device's set-up
+
+// with `device` current
+void noalloc_setup(int device) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if ((int)g_noalloc_ptr.size() <= device) g_noalloc_ptr.resize(device + 1, nullptr);
+    if (g_noalloc_ptr[device]) return;
+    void *p = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) throw Error(OFX_EDEVICE, "hipGetSymbolAddress gave no address for the scratch array");
+    g_noalloc_ptr[device] = p;
+}
+
+void *noalloc_scratch(int device) {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (device < 0 || device >= (int)g_noalloc_ptr.size() || !g_noalloc_ptr[device])
+        throw Error(OFX_EDEVICE, "the scratch array of device " + std::to_string(device) + " was never set up");
+    return g_noalloc_ptr[device];
+}
+
+// The selection's digit histograms count in 32 bits (SelectState in fine.hip).
+void check_median_count(size_t n) {
+    if (n > 0xFFFFFFFFull)
+        throw Error(OFX_ETOOLARGE, "median of " + std::to_string(n) + " values: at most 4294967295 are supported");
+}
+}  // namespace
+
 extern "C" {
 
 void ofx_config_default(ofx_config *c) {
@@ -1424,6 +1462,7 @@ int ofx_create(int device, int lanes, ofx_engine **out) {
         if (lanes < 1 || lanes > 8) throw Error(OFX_EINVAL, "lanes must lie in [1, 8]");
         watchdog_start();
         HIPCHECK(hipSetDevice(device));
+        noalloc_setup(device);
         std::unique_ptr<ofx_engine> e(new ofx_engine);
         e->device = device;
         for (int i = 0; i < lanes; i++) {
@@ -1696,6 +1735,7 @@ int ofx_frame_to_gray(ofx_engine *e, const void *pixels, int bits, int channels,
 int ofx_median(ofx_engine *e, const double *values, size_t n, double *out) {
     return guarded([&] {
         if (!out || (n && !values)) throw Error(OFX_EINVAL, "values or out is NULL");
+        check_median_count(n);
         if (n == 0) {
             *out = NAN;  // numpy.median of nothing
             return;
@@ -1708,6 +1748,27 @@ int ofx_median(ofx_engine *e, const double *values, size_t n, double *out) {
         copy_h2d(ws.pin, dv, values, n * sizeof(double), s);
         median(dv, n, ws.bmedscr.get<char>(median_scratch_bytes()), med, s);
         copy_d2h(ws.pin, out, med, sizeof(double), s);
+        wait_stream(s);
+    });
+}
+
+int ofx_median_device_noalloc(ofx_engine *e, const double *d_values, size_t n, double *d_out) {
+    return guarded([&] {
+        if (!e) throw Error(OFX_EINVAL, "engine is NULL");
+        if (!d_out || (n && !d_values)) throw Error(OFX_EINVAL, "d_values or d_out is NULL");
+        check_median_count(n);
+        static_assert(NOALLOC_SCRATCH_BYTES >= sizeof(double), "scratch too small");
+        void *scr = noalloc_scratch(e->device);
+        if (median_scratch_bytes() > NOALLOC_SCRATCH_BYTES)
+            throw Error(OFX_EUNSUPPORTED, "the selection state exceeds the library's fixed scratch array");
+        LaneGuard g(e);
+        hipStream_t s = g.lane->stream;
+        if (n == 0) {
+            hipLaunchKernelGGL(k_fill, dim3(1), dim3(64), 0, s, d_out, (double)NAN, (size_t)1);  // numpy.median of nothing
+            LAUNCHCHECK();
+        } else {
+            median(d_values, n, scr, d_out, s);
+        }
         wait_stream(s);
     });
 }
