@@ -230,6 +230,48 @@ int ofx_seq_step(ofx_sequence *seq, const float *frame, ofx_result *out);
 int ofx_seq_step_device(ofx_sequence *seq, const float *d_frame, ofx_result *d_out);
 void ofx_seq_end(ofx_sequence *seq);
 
+/* ---------------------------------------------------------------- output folder (host only)
+ * The reference CLI's output side (rust/crates/optflow-cli/src/crop.rs, run.rs:74-110,
+ * :185-207, io.rs:557-621), so that a folder written through this library has the bytes the
+ * reference writes. None of these functions touches the device. */
+
+/* The automatic ROI crop (crop.rs:43-130): the bounding box of the nonzero pixels of mask
+ * (w x h), grown by *margin px, or with margin NULL by the default margin (the largest
+ * displacement + 4 sigma_w + the guard band + 32 + (poc_win / 2 + grid_step) 2^L, doubled when
+ * seq), its origin moved down to a multiple of grid_step 2^L (the frame's coarse grid), clipped to
+ * the frame. Errors as the reference: "ROI mask is empty", "--crop-margin must be >= 0, got m",
+ * "image smaller than poc_win", "the crop WxH is smaller than poc_win". */
+typedef struct ofx_crop {
+    int x0, y0, x1, y1; /* half-open rectangle of the frame */
+    int64_t margin;
+} ofx_crop;
+int ofx_crop_plan(const uint8_t *mask, int w, int h, const ofx_config *cfg, const int64_t *margin, int seq,
+                  ofx_crop *out);
+
+/* crop.json (run.rs:192-207, Crop::json crop.rs:139-144): the rectangle and margin with "mode":
+ * "auto", or {"mode": "off"} for crop NULL, written into dir (created with its parents). */
+int ofx_write_crop_json(const char *dir, const ofx_crop *crop);
+
+/* One result folder (run.rs:74-110 save, io.rs:557-621 save_result_placed): ux, uy, conf, Exx,
+ * Eyy, Exy, gmax, vol as <f4 .npy, valid (|u1, 1 where the written ux is finite), fb (<i4) when
+ * res->fb, raw_ux and raw_uy with raw, and timing.json (Timing of track.rs:157-184 with "device":
+ * "gpu" after "coarse"; reference and mean_conf when seq; the hybrid shares when seq and res->fb).
+ * res holds w x h host maps: the whole frame (crop NULL), or the crop's, each then placed in a
+ * frame_w x frame_h map that is NaN outside the crop (conf and fb 0). The .npy bytes are those of
+ * numpy.save (format 1.0); dir is created with its parents. */
+int ofx_write_result(const char *dir, const ofx_result *res, int w, int h, const ofx_crop *crop, int frame_w,
+                     int frame_h, int raw, int seq);
+
+/* The coarse stage's pyramid (poc.rs:610-666, python/optflow_ref/poc.py:172-195): the number of
+ * levels below the full image for a w x h frame, and the POC window (wy, wx) at a level of w x h
+ * (0, 0 when none fits). */
+int ofx_pyramid_levels(const ofx_config *cfg, int w, int h, int *levels);
+int ofx_poc_level_window(int w, int h, int win, int *wy, int *wx);
+
+/* v as the reference CLI's JSON writer (serde_json) spells it: the shortest digits that read back
+ * to v, "1.0" for integers, 1e-7 style exponents outside [1e-5, 1e16); null when not finite. */
+int ofx_format_json_number(double v, char *buf, size_t cap);
+
 #ifdef __cplusplus
 }
 #endif

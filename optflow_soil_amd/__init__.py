@@ -23,7 +23,9 @@ SYMBOLS = ("ofx_config_default", "ofx_create", "ofx_destroy", "ofx_last_error", 
            "ofx_check_window", "ofx_release_caches", "ofx_refine_raw", "ofx_coarse_flow", "ofx_track_pair",
            "ofx_track_pair_device", "ofx_track_pairs", "ofx_seq_begin", "ofx_seq_step", "ofx_seq_step_device", "ofx_seq_end",
            "ofx_tavg_window", "ofx_frame_to_gray", "ofx_median", "ofx_median_device_noalloc",
-           "ofx_track_sequence", "ofx_profile", "ofx_profile_read", "ofx_profile_stage")
+           "ofx_track_sequence", "ofx_profile", "ofx_profile_read", "ofx_profile_stage", "ofx_crop_plan",
+           "ofx_write_crop_json", "ofx_write_result", "ofx_pyramid_levels", "ofx_poc_level_window",
+           "ofx_format_json_number")
 
 
 class OfxConfig(ctypes.Structure):

@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libofx.so")
-SOURCES = ["copy.hip", "fft.hip", "fine.hip", "guard.hip", "poc.hip", "seq.hip", "hybrid.hip", "tavg.hip", "engine.hip"]
+SOURCES = ["copy.hip", "fft.hip", "fine.hip", "guard.hip", "poc.hip", "seq.hip", "hybrid.hip", "tavg.hip", "engine.hip", "output.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          # packed FP32 instructions stay off: they have given wrong results on this hardware
@@ -17,7 +17,7 @@ def build(jobs=8, verbose=False):
     objs = []
     procs = []
     for src in SOURCES:
-        obj = os.path.join(CSRC, src.replace(".hip", ".o"))
+        obj = os.path.join(CSRC, os.path.splitext(src)[0] + ".o")
         cmd = [HIPCC, *FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
         objs.append(obj)
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))

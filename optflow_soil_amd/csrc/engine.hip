@@ -7,6 +7,7 @@
 // rust/crates/optflow-core/src/track.rs:1112-1225 (chained sequence),
 // rust/crates/optflow-gpu/src/engine.rs:633-1071 (lanes, check_window, errors).
 #include <atomic>
+#include <functional>
 #include <cstdio>
 #include <cstdlib>
 #include <thread>
@@ -1054,6 +1055,14 @@ int guarded(F &&f) {
     }
 }
 
+
+}  // namespace
+
+int guarded_call(const std::function<void()> &f) { return guarded(f); }
+void validate_config(const ofx_config *cfg) { validate(cfg); }
+double guard_band_px(const ofx_config &c) { return edge_guard(c); }
+
+namespace {
 
 // A sequence on one lane (rust/crates/optflow-core/src/track.rs:1112-1225 Sequence::new and
 // step; track.py:260-315 for seq_mode "chain", track/hybrid.rs:262-540 and track.py:383-505 for

@@ -4,6 +4,7 @@
 
 #include <atomic>
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -92,6 +93,18 @@ void copy_d2h(PinnedBuf &pin, void *host, const void *dev, size_t bytes, hipStre
         ::ofx::launch_mark(__FILE__, __LINE__);         \
         HIPCHECK(hipGetLastError());                    \
     } while (0)
+
+// Runs f, turning an exception into the status code it carries and the thread's last-error text
+// (every C entry point goes through it).
+int guarded_call(const std::function<void()> &f);
+// The reference's configuration checks (config.py:166-245) for the keys the engine reads.
+void validate_config(const ofx_config *cfg);
+// The guard band in px: edge_guard, or with -1 the automatic 3 lam max(scales) / (2 pi bandwidth).
+double guard_band_px(const ofx_config &c);
+// The coarse stage's pyramid (poc.rs:610-666): the number of levels below the full image for a
+// w x h image, and the POC window at a level of h x w (false: none fits).
+int pyramid_levels(double max_disp, int win, int h, int w);
+bool poc_window_at(int h, int w, int win, int &wy, int &wx);
 
 // ---------------------------------------------------------------- FFT plans and passes
 struct FftPlan {

@@ -62,6 +62,13 @@ int n_levels(double max_disp, int win, int h, int w) {
     return L;
 }
 
+}  // namespace
+
+int pyramid_levels(double max_disp, int win, int h, int w) { return n_levels(max_disp, win, h, w); }
+bool poc_window_at(int h, int w, int win, int &wy, int &wx) { return level_window(h, w, win, wy, wx); }
+
+namespace {
+
 // rust poc.rs:244-256: (start, count) of the grid positions along an axis of length n
 void grid_axis(int n, int win, int step, int &start, int &count) {
     const int stop = n + 1 - (win - win / 2);

@@ -3,12 +3,14 @@
 - to_gray / read_image / list_images: the reference's image input (python/optflow_ref/io.py:15-97,
   rust/crates/optflow-cli/src/io.rs:44-52, :259-281). Decoding stays on the host (Pillow); the
   conversion to grey levels and the box downscale run on the GPU (ofx_frame_to_gray).
-- averaged: the windowing of rust/crates/optflow-cli/src/tavg.rs:210-352 (io.py:100-162): every
-  sampled frame is the mean of the 2 m + 1 frames around it, clipped to the input; a window that is
-  one-sided because the input ends has its neighbours aligned by the dense coarse model first.
-  The mean itself is ofx_tavg_window (or any function with its signature, for tests).
+- averaged: temporal averaging with the windows of rust/crates/optflow-cli/src/tavg.rs:210-352:
+  every sampled frame is the mean of the 2 m + 1 frames around it, clipped to the input; a window
+  that is one-sided because the input ends has its neighbours aligned by the dense coarse model
+  first. A bounded queue holds the frames later windows still need. The mean itself is
+  ofx_tavg_window (or any function with its signature, for tests).
 - iter_frames: io.py:100-121 on a directory or glob of images.
 """
+import collections
 import dataclasses
 import glob
 import os
@@ -65,55 +67,56 @@ def gpu_average(engine: Engine = None):
 
 def averaged(raw, start=0, stop=None, step=1, m=0, align=False, cfg=None, mask_fn=None, edge_align=True,
              average=None):
-    """tavg.rs:210-352 / io.py:124-162. raw yields the frames with index max(0, start - m), ...;
-    yields the means for the sampled indices start, start + step, ... < stop (the windows may reach
-    past stop and are clipped at the last frame). align: every window aligned (affine or dense as
-    cfg.tavg_model says, fit restricted to mask_fn(shape)); else the one-sided windows at the ends
-    are aligned with the dense model (edge_align), a neighbour whose alignment fails entering
-    unaligned."""
+    """Temporal averaging of a frame stream (the behaviour of tavg.rs:210-352): raw yields the
+    frames with index max(0, start - m), max(0, start - m) + 1, ...; the result is the mean of
+    the frames within m of each sampled index s = start, start + step, ... (s < stop when stop
+    is given), clipped to the frames that exist. A mean is produced as soon as frame s + m has
+    arrived, or at the end of the stream with the window cut at the last frame, and only the
+    frames some later window still needs are held.
+
+    align: every window aligned before averaging (cfg.tavg_model, fit restricted to
+    mask_fn(shape)); otherwise a window made one-sided by the ends of the input is aligned with
+    the dense coarse model (edge_align), a neighbour whose alignment fails entering unaligned.
+    average(frames, centre, model, cfg, mask, keep_unaligned) forms one mean (ofx_tavg_window on
+    the GPU by default)."""
     if step < 1:
         raise ValueError(f"step must be >= 1, got {step}")
     if start < 0:
         raise ValueError(f"start must be >= 0, got {start}")
     if m < 0:
         raise ValueError(f"tavg must be >= 0, got {m}")
-    average = average or gpu_average()
+    mean = average or gpu_average()
     cfg = cfg or Config()
-    edge_cfg = dataclasses.replace(cfg, tavg_model="dense") if edge_align else None
-    buf = {}
-    nxt = start
-    last = None
+    dense = dataclasses.replace(cfg, tavg_model="dense") if edge_align else None
+    held = collections.deque()  # (index, frame), consecutive indices
+    wanted = (lambda s: True) if stop is None else (lambda s: s < stop)
 
-    def emit(s):
-        lo, hi = max(0, s - m), min(s + m, last)
-        fr = [buf[i] for i in range(lo, hi + 1)]
-        if len(fr) == 1:
-            return fr[0]
-        one_sided = s - lo != hi - s
-        # an image smaller than the POC window has no grid to estimate the motion on
-        edge = (one_sided and not align and edge_cfg is not None and min(fr[0].shape) >= edge_cfg.poc_win)
+    def window_mean(s, newest):
+        lo, hi = max(0, s - m), min(s + m, newest)
+        frames = [f for i, f in held if lo <= i <= hi]
+        centre = s - lo
+        if len(frames) == 1:
+            return frames[0]
         if align:
-            mask = mask_fn(fr[0].shape) if mask_fn else None
-            return average(fr, s - lo, cfg.tavg_model, cfg, mask, False)
-        if edge:
-            return average(fr, s - lo, "dense", edge_cfg, None, True)
-        return average(fr, s - lo, None, cfg, None, False)
+            return mean(frames, centre, cfg.tavg_model, cfg, mask_fn(frames[0].shape) if mask_fn else None, False)
+        # a window cut by the ends of the input is centred on s only after alignment; an image
+        # smaller than the POC window has no grid to estimate the motion on
+        if dense is not None and hi - s != centre and min(frames[0].shape) >= dense.poc_win:
+            return mean(frames, centre, "dense", dense, None, True)
+        return mean(frames, centre, None, cfg, None, False)
 
-    def due(ready):
-        return (stop is None or nxt < stop) and ready is not None and nxt <= ready
-
-    read = max(0, start - m)
-    for f in raw:
-        buf[read], last = f, read
-        read += 1
-        while due(last - m if last - m >= 0 else None):
-            yield emit(nxt)
-            nxt += step
-            for j in [j for j in buf if j < nxt - m]:
-                del buf[j]
-    while due(last):
-        yield emit(nxt)
-        nxt += step
+    target = start
+    newest = -1
+    for newest, frame in enumerate(raw, start=max(0, start - m)):
+        held.append((newest, frame))
+        while wanted(target) and target + m <= newest:
+            yield window_mean(target, newest)
+            target += step
+            while held and held[0][0] < target - m:
+                held.popleft()
+    while wanted(target) and target <= newest:
+        yield window_mean(target, newest)
+        target += step
 
 
 def iter_frames(source, start=0, stop=None, step=1, scale=1, tavg=0, align=False, cfg=None, mask_fn=None,

@@ -67,55 +67,144 @@ def test_synthetic_workloads_are_seeded():
     assert 0.55 < m.mean() < 0.65
 
 
-def test_output_folder_has_the_reference_files_and_dtypes(tmp_path):
-    """io.py:203-223: the .npy names and dtypes of a result folder, byte-identical to numpy.save."""
-    from optflow_soil_amd import Result, io
-    h, w = 5, 7
-    rng = np.random.default_rng(0)
+def _result(h, w, seed=0, fb=False):
+    from optflow_soil_amd import Result
+    rng = np.random.default_rng(seed)
     f = lambda: rng.standard_normal((h, w)).astype(np.float32)
     ux = f()
     ux[0, 0] = np.nan
     strain = {k: f() for k in ("Exx", "Eyy", "Exy", "gmax", "vol")}
-    res = Result(ux, f(), f(), strain, f(), f(), np.isfinite(ux), {"coarse": 0.1}, np.zeros((h, w), np.int32))
-    io.save_result(str(tmp_path), res, raw=True)
-    want = {"ux": "<f4", "uy": "<f4", "conf": "<f4", "Exx": "<f4", "Eyy": "<f4", "Exy": "<f4", "gmax": "<f4",
-            "vol": "<f4", "raw_ux": "<f4", "raw_uy": "<f4", "valid": "|u1", "fb": "<i4"}
-    for name, dt in want.items():
-        a = np.load(tmp_path / f"{name}.npy")
-        assert a.dtype.str == dt and a.shape == (h, w), name
-    assert np.load(tmp_path / "valid.npy")[0, 0] == 0
-    # byte-identical to numpy.save of the same array
+    fbm = rng.integers(0, 5, (h, w)).astype(np.int32) if fb else None
+    return Result(ux, f(), f(), strain, f(), f(), np.isfinite(ux), {"coarse": 0.1, "fine": 2.0, "strain": 0.0125}, fbm)
+
+
+def _npy_bytes(a):
     import io as pyio
     buf = pyio.BytesIO()
-    np.save(buf, np.ascontiguousarray(res.strain["gmax"], dtype="<f4"))
-    assert (tmp_path / "gmax.npy").read_bytes() == buf.getvalue()
-    import json
-    assert json.loads((tmp_path / "timing.json").read_text()) == {"coarse": 0.1}
-    assert io.sequence_dir("out", 3).endswith("frame_00003")
+    np.save(buf, a)
+    return buf.getvalue()
 
 
-def test_crop_plan_and_embed():
-    """crop.py:81-103: bounding box + margin, origin on the coarse grid, clipped; embed puts the
-    crop back with NaN / 0 outside."""
-    from optflow_soil_amd import Result, io
-    cfg = Config()
-    mask = np.zeros((600, 800), bool)
-    mask[200:300, 250:400] = True
-    c = io.Crop.plan(mask, cfg, margin=50)
-    A = cfg.grid_step * 2 ** io.n_levels(cfg.max_disp, cfg.poc_win, mask.shape)
-    assert (c.x0, c.y0) == ((250 - 50) // A * A, (200 - 50) // A * A)
-    assert (c.x1, c.y1) == (400 + 50, 300 + 50)
-    assert io.n_levels(32, 64, (2048, 2048)) == 1 and io.level_window((56, 56), 64) == (32, 32)
-    h, w = c.shape
-    z = np.ones((h, w), np.float32)
-    res = Result(z, z, z, {k: z for k in ("Exx", "Eyy", "Exy", "gmax", "vol")}, z, z, np.ones((h, w), bool), {})
-    full = c.embed(res, mask.shape)
-    assert np.isnan(full.ux[0, 0]) and full.conf[0, 0] == 0 and not full.valid[0, 0]
-    assert full.ux[c.y0, c.x0] == 1 and np.isfinite(full.ux).sum() == h * w
-    assert '"mode": "auto"' in c.json()
+def test_output_folder_has_the_reference_files_and_bytes(tmp_path):
+    """A result folder (run.rs:74-110, io.rs:557-621) written by libofx: every map byte-identical
+    to numpy.save of the expected array (<f4 maps, |u1 valid, <i4 fb), and timing.json in the
+    reference CLI's layout ("device" after "coarse", serde_json's number spelling)."""
+    from optflow_soil_amd import io
+    h, w = 5, 7
+    res = _result(h, w)
+    io.write_result(str(tmp_path), res, raw=True)
+    maps = {"ux": res.ux, "uy": res.uy, "conf": res.conf, "raw_ux": res.raw_ux, "raw_uy": res.raw_uy, **res.strain}
+    for name, a in maps.items():
+        assert (tmp_path / f"{name}.npy").read_bytes() == _npy_bytes(np.ascontiguousarray(a, "<f4")), name
+    assert (tmp_path / "valid.npy").read_bytes() == _npy_bytes(np.isfinite(res.ux).astype(np.uint8))
+    assert not (tmp_path / "fb.npy").exists()
+    assert (tmp_path / "timing.json").read_text() == (
+        '{\n  "coarse": 0.1,\n  "device": "gpu",\n  "fine": 2.0,\n  "strain": 0.0125\n}')
+    assert io.frame_folder("out", 3).endswith("frame_00003")
+
+
+def test_output_folder_of_a_crop_is_placed_in_the_frame(tmp_path):
+    """A crop's result is written as full-frame maps: NaN outside the crop (conf and fb 0), valid
+    1 where the placed ux is finite; a hybrid sequence frame adds fb and the timing shares."""
+    from optflow_soil_amd import io
+    H, W = 40, 50
+    crop = io.CropWindow(16, 8, 16 + 21, 8 + 13, 5)
+    res = _result(13, 21, seed=1, fb=True)
+    res.timing.update(reference=2, mean_conf=0.75, n_fb_max=3, fallback_frac=0.125, fb_gate_rejected=1e-6,
+                      reacquired=0.0)
+    io.write_result(str(tmp_path), res, crop=crop, frame_shape=(H, W), sequence=True)
+
+    def placed(a, fill, dtype):
+        full = np.full((H, W), fill, dtype)
+        full[crop.y0:crop.y1, crop.x0:crop.x1] = a
+        return full
+    assert (tmp_path / "ux.npy").read_bytes() == _npy_bytes(placed(res.ux, np.nan, "<f4"))
+    assert (tmp_path / "conf.npy").read_bytes() == _npy_bytes(placed(res.conf, 0.0, "<f4"))
+    assert (tmp_path / "vol.npy").read_bytes() == _npy_bytes(placed(res.strain["vol"], np.nan, "<f4"))
+    assert (tmp_path / "fb.npy").read_bytes() == _npy_bytes(placed(res.fb, 0, "<i4"))
+    valid = np.load(tmp_path / "valid.npy")
+    assert valid.dtype == np.uint8 and (valid == np.isfinite(placed(res.ux, np.nan, "<f4"))).all()
+    assert not (tmp_path / "raw_ux.npy").exists()
+    assert (tmp_path / "timing.json").read_text() == (
+        '{\n  "coarse": 0.1,\n  "device": "gpu",\n  "fine": 2.0,\n  "strain": 0.0125,\n  "reference": 2,\n'
+        '  "mean_conf": 0.75,\n  "n_fb_max": 3,\n  "fallback_frac": 0.125,\n  "fb_gate_rejected": 1e-6,\n'
+        '  "reacquired": 0.0\n}')
     import pytest
-    with pytest.raises(ValueError, match="ROI mask is empty"):
-        io.Crop.plan(np.zeros((100, 100), bool), cfg)
+    with pytest.raises(ValueError, match="does not hold"):
+        io.write_result(str(tmp_path), res, crop=io.CropWindow(40, 8, 61, 21, 5), frame_shape=(H, W))
+
+
+def test_json_numbers_are_spelled_as_the_reference_cli_spells_them():
+    """serde_json's spelling (the shortest digits that read back, ryu's layout): what the
+    reference CLI's timing.json holds."""
+    from optflow_soil_amd import io
+    cases = {0.0: "0.0", 1.0: "1.0", 0.1: "0.1", 12.34: "12.34", 2.5e-5: "0.000025", 1e-5: "0.00001",
+             1e-6: "1e-6", 1.5e-7: "1.5e-7", 123456789.0: "123456789.0", 1e15: "1000000000000000.0", 1e16: "1e16",
+             1e17: "1e17", 1.25e20: "1.25e20", -0.5: "-0.5", 1 / 3: "0.3333333333333333", float("nan"): "null"}
+    for v, want in cases.items():
+        assert io.json_number(v) == want, v
+    rng = np.random.default_rng(3)
+    for v in rng.lognormal(0, 8, 200):
+        assert float(io.json_number(v)) == v
+
+
+def _roi(w, h, xs, ys):
+    m = np.zeros((h, w), bool)
+    m[ys[0]:ys[1], xs[0]:xs[1]] = True
+    return m
+
+
+def test_crop_plan_matches_the_reference_cli_cases():
+    """ofx_crop_plan against the reference's own crop cases (crop.rs:255-397): the default margin
+    (displacement + 4 sigma_w + guard + 32 + the coarse footprint, doubled for a sequence), the
+    origin aligned down to grid_step 2^L, clipping, a given margin, and the error texts."""
+    import pytest
+    from optflow_soil_amd import io
+    cfg = Config()
+    c = io.CropWindow.plan(_roi(768, 768, (300, 400), (300, 400)), cfg)
+    assert (c.x0, c.y0, c.x1, c.y1, c.margin) == (96, 96, 589, 589, 189)
+    assert io.CropWindow.plan(_roi(768, 768, (300, 400), (300, 400)), cfg, seq=True).margin == 2 * 189
+    c = io.CropWindow.plan(_roi(2048, 2048, (1000, 1080), (1000, 1080)), Config(max_disp=128.0))
+    assert c.margin == 573 and (c.x0, c.y0, c.x1, c.y1) == (384, 384, 1653, 1653)
+    assert c.to_json() == ('{\n  "x0": 384,\n  "y0": 384,\n  "x1": 1653,\n  "y1": 1653,\n  "margin": 573,\n'
+                           '  "mode": "auto"\n}')
+    c = io.CropWindow.plan(_roi(768, 768, (300, 400), (300, 400)), Config(max_disp=100.5, sigma_w=6.0, edge_guard=5.0))
+    assert c.margin == 101 + 24 + 5 + 32 + 384
+    m = _roi(400, 300, (350, 400), (0, 80))
+    c = io.CropWindow.plan(m, cfg)
+    assert (c.x0, c.y0, c.x1, c.y1) == (160, 0, 400, 269)
+    c = io.CropWindow.plan(m, cfg, margin=10)
+    assert (c.x0, c.y0, c.x1, c.y1, c.margin) == (320, 0, 400, 90, 10)
+    assert io.CropWindow.plan(m, cfg, seq=True).margin == 378
+    with pytest.raises(ValueError, match="^ROI mask is empty$"):
+        io.CropWindow.plan(np.zeros((300, 400), bool), cfg)
+    with pytest.raises(ValueError, match="^--crop-margin must be >= 0, got -1$"):
+        io.CropWindow.plan(_roi(400, 300, (0, 400), (0, 300)), cfg, margin=-1)
+    with pytest.raises(ValueError, match=r"^the crop 14x14 is smaller than poc_win \(64\): increase --crop-margin "
+                                         r"or use --crop off$"):
+        io.CropWindow.plan(_roi(400, 300, (100, 110), (100, 110)), cfg, margin=0)
+    with pytest.raises(ValueError, match=r"^image smaller than poc_win \(64\): \(32, 40\)$"):
+        io.CropWindow.plan(_roi(40, 32, (0, 40), (0, 32)), cfg)
+    a = np.arange(768 * 768, dtype=np.float32).reshape(768, 768)
+    c = io.CropWindow.plan(_roi(768, 768, (300, 400), (300, 400)), cfg)
+    assert (c.cut(a) == a[96:589, 96:589]).all() and c.shape == (493, 493)
+
+
+def test_pyramid_tables_of_the_engine_match_the_reference(tmp_path):
+    """The engine's pyramid depth and per-level POC window (the tables the coarse stage and the
+    crop use) against the reference's tables (meta.json level_window, n_levels), and crop.json."""
+    from conftest import meta
+    from optflow_soil_amd import io
+    m = meta()
+    for h, w, win in m["level_window"]:
+        assert io.poc_level_window((h, w), 64) == (tuple(win) if win else None), (h, w)
+    for d, h, w, lev in m["n_levels"]:
+        assert io.pyramid_levels(Config(max_disp=float(d)), (h, w)) == lev, (d, h, w)
+    io.CropWindow.write_json(str(tmp_path))
+    assert (tmp_path / "crop.json").read_text() == '{\n  "mode": "off"\n}'
+    c = io.CropWindow(0, 32, 100, 90, 7)
+    io.CropWindow.write_json(str(tmp_path), c)
+    assert (tmp_path / "crop.json").read_text() == c.to_json()
 
 
 def test_sequence_handle_errors_need_no_device():
