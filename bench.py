@@ -5,9 +5,11 @@ speckle pair (seeded sum of Gaussian dots) with a settlement bowl uy = -6 exp(-r
 and a 40 px wide shear band at 35 deg with a 3 px tanh slip, noise sigma 0.005, a seeded convex
 8-gon ROI of ~60 %, default settings. One step = one whole pair (coarse POC, fine stage,
 confidence, guard, strain) with the inputs already resident in HBM, through the device-pointer
-C ABI. Reports the per-stage GPU times, the roofline of the dominant kernel, the EPE against the
-synthetic truth and a CPU baseline (the FP64 oracle on a 1024 x 1024 crop of the same pair, timed on
-the host's cores).
+C ABI. `value` is the plain method (the reference's POC-pyramid coarse stage); `learned_coarse`
+times the same pair with the learned coarse proposal in place of the pyramid's coarse levels.
+Reports the per-stage GPU times, the roofline of the dominant kernel, the EPE against the synthetic
+truth, the error against the FP64 oracle and a CPU baseline (the oracle on a 1024 x 1024 crop of
+the same pair, timed on the host's cores).
 """
 import argparse
 import dataclasses
@@ -59,6 +61,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=2048)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-coarse form of the pair")
     ap.add_argument("--frames", type=int, default=12, help="frames of the --workload c3 sequence")
     ap.add_argument("--workload", choices=("c2", "c3", "c4", "c5"), default="c2",
                     help="c2: one 2048^2 pair (default); c4: 64 uneven pairs through the batched entry point; "
@@ -106,7 +109,7 @@ def main():
         a, b, (tx, ty) = synth.workload_c5()
         roi = None
         cfg = Config(max_disp=48.0, tile=args.tile)
-        metric = "Mpix/s flow+strain at 4000x3000, plain POC-initialised; EPE px vs truth"
+        metric = "Mpix/s flow+strain at 4000x3000, plain GPU vs learned-coarse; EPE px vs truth"
         desc = ("single pair 4000x3000 uint8 speckle, six Gaussian-bump fields up to 40 px, noise 0.01, "
                 "plain POC-initialised method, max_disp 48 (two POC levels), full frame, "
                 + (f"tiles of {args.tile}" if args.tile else "untiled"))
@@ -130,31 +133,47 @@ def main():
     eng = ofx.Engine(0)
     eng.check_window(w, h, cfg)
 
-    def step():
-        return eng.track_pair_device(d_a.data_ptr(), d_b.data_ptr(), w, h, cfg, ptrs,
-                                     d_mask=0 if d_roi is None else d_roi.data_ptr())
+    def timed(c, label):
+        """warm-up, then args.steps timed pairs with the stage profiler on; returns (seconds per
+        pair, stage profile, last step's timing record, EPE, valid fraction)"""
+        def step():
+            return eng.track_pair_device(d_a.data_ptr(), d_b.data_ptr(), w, h, c, ptrs,
+                                         d_mask=0 if d_roi is None else d_roi.data_ptr())
+        for i in range(args.warmup):
+            step()
+            print(f"# {label}: warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        eng.profile(True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            rec = step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        prof = eng.profile_read()
+        eng.profile(False)
+        # accuracy of the last step against the synthetic truth (valid pixels inside the ROI)
+        ux, uy = outs["ux"].cpu().numpy(), outs["uy"].cpu().numpy()
+        fin = np.isfinite(ux) & np.isfinite(uy)
+        epe = float(np.sqrt(np.mean((ux[fin] - tx[fin]) ** 2 + (uy[fin] - ty[fin]) ** 2))) if fin.any() else float("nan")
+        valid_frac = float(fin[roi].mean()) if roi is not None else float(fin.mean())
+        return (t1 - t0) / args.steps, prof, rec, epe, valid_frac
 
-    for i in range(args.warmup):
-        step()
-        print(f"# warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
-    torch.cuda.synchronize()
-    eng.profile(True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for i in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    prof = eng.profile_read()
-    eng.profile(False)
-    dt = (t1 - t0) / args.steps
+    dt, prof, _, epe, valid_frac = timed(cfg, "plain")
     value = w * h / 1e6 / dt
-
-    # accuracy of the last step against the synthetic truth (valid pixels inside the ROI)
-    ux, uy = outs["ux"].cpu().numpy(), outs["uy"].cpu().numpy()
-    fin = np.isfinite(ux) & np.isfinite(uy)
-    epe = float(np.sqrt(np.mean((ux[fin] - tx[fin]) ** 2 + (uy[fin] - ty[fin]) ** 2))) if fin.any() else float("nan")
-    valid_frac = float(fin[roi].mean()) if roi is not None else float(fin.mean())
+    learned = None
+    if not args.no_learned:
+        eng.load_coarse_net()
+        cfg_l = dataclasses.replace(cfg, coarse_model="learned")
+        dt_l, prof_l, rec_l, epe_l, valid_l = timed(cfg_l, "learned coarse")
+        learned = {
+            "value": round(w * h / 1e6 / dt_l, 3), "ms_per_step": round(dt_l * 1e3, 3),
+            "over_plain": round(dt / dt_l, 4), "epe_px": round(epe_l, 5), "valid_frac_roi": round(valid_l, 5),
+            "coarse_ms_per_step": round(prof_l["coarse"][0] / args.steps, 3),
+            "plain_coarse_ms_per_step": round(prof["coarse"][0] / args.steps, 3),
+            "coarse_confirmed": round(rec_l["coarse_confirmed"], 4), "fallback": rec_l["coarse_fallback"],
+            "network": "6 x conv3x3 bf16 (169 -> 32 -> 32 -> 32 -> 32 -> 32 -> 2), fp32 accumulate, "
+                       "optflow_soil_amd/weights/coarse_net.npy"}
 
     # roofline of the accumulate kernel: one launch per inner iteration over the frame, or per tile
     # and inner iteration when tiled (then the frame's bytes per iteration over the tiles' total time)
@@ -186,6 +205,7 @@ def main():
                      "algorithmic_bytes_per_launch": acc_bytes, "avg_launch_ms": round(acc_avg_s * 1e3, 4)},
         "stage_ms_per_step": stages,
         "epe_px": round(epe, 5), "valid_frac_roi": round(valid_frac, 5), "err_vs_oracle": err,
+        "learned_coarse": learned,
         "pipeline": {"bound": "hbm", "bytes_per_step": pipeline_bytes(w, h, cfg.tile),
                      "achieved": round(pipeline_bytes(w, h, cfg.tile) / dt / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s",

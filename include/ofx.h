@@ -73,6 +73,10 @@ typedef struct ofx_config {
                                  (the reference's tiled fine stage with N = 1024, tile.rs:56-98);
                                  the tiles run in three groups side by side, on the lane's stream
                                  and two more streams of the engine */
+    int coarse_learned;       /* 0: the POC pyramid (the reference's coarse stage); 1: the level-0
+                                 POC grid started from the learned coarse proposal
+                                 (ofx_coarse_net_load), with the pyramid as the fallback */
+    double coarse_gate;       /* 0.9: share of the grid points that must confirm the proposal */
 } ofx_config;
 
 typedef struct ofx_engine ofx_engine;
@@ -146,6 +150,11 @@ typedef struct ofx_result {
     double fallback_frac;              /* valid by an accepted fallback */
     double fb_gate_rejected;           /* fallback candidates the gate rejected */
     double reacquired;                 /* re-acquired by the direct leg */
+    /* pairs with coarse_learned: the share of the level-0 grid points that confirmed the learned
+     * proposal, and 1 when that share was below coarse_gate and the POC pyramid ran instead;
+     * -1 and 0 when the pyramid ran by configuration */
+    double coarse_confirmed;
+    int coarse_fallback;
 } ofx_result;
 
 /* track_pair (rust/crates/optflow-core/src/track.rs:688-861; python/optflow_ref/track.py:115-146):
@@ -229,6 +238,49 @@ int ofx_seq_begin(ofx_engine *e, const float *frame0, int w, int h, const uint8_
 int ofx_seq_step(ofx_sequence *seq, const float *frame, ofx_result *out);
 int ofx_seq_step_device(ofx_sequence *seq, const float *d_frame, ofx_result *d_out);
 void ofx_seq_end(ofx_sequence *seq);
+
+/* ---------------------------------------------------------------- learned coarse proposal
+ * No counterpart in the reference: a 6-layer 3 x 3 convolutional network in bfloat16 (32 hidden
+ * channels, float32 accumulation on the matrix cores) that maps the correlation volume of the two
+ * frames at 1/8 resolution (displacements of -6 .. 6 cells, 169 channels) to the displacement of
+ * every 8 x 8 cell. It stands in for the coarse levels of the POC pyramid (poc.rs:694-766): the
+ * level-0 POC grid starts from its proposal and checks it, and the pyramid runs instead when the
+ * check fails (ofx_config.coarse_learned, coarse_gate). Parameters are float32 in a flat layout:
+ * per layer the weights [cout][3 x 3][cin] then the biases [cout], layers 169 -> 32 -> 32 -> 32 ->
+ * 32 -> 32 -> 2 (ReLU after all but the last). */
+size_t ofx_coarse_net_params(void);
+
+/* Loads the network every lane of the engine uses for coarse_learned (n = ofx_coarse_net_params()).
+ * OFX_EBUSY while a call runs on the engine. */
+int ofx_coarse_net_load(ofx_engine *e, const float *params, size_t n);
+
+/* The proposal for frames i1, i2 (w x h, host): out is ((h + 7) / 8) x ((w + 7) / 8) x 2, the
+ * displacement (ux, uy) in pixels of each 8 x 8 cell, with the loaded network. */
+int ofx_coarse_net_propose(ofx_engine *e, const float *i1, const float *i2, int w, int h, float *out);
+
+/* One forward and backward pass of the network with parameters params (n floats, flat layout) on a
+ * batch of nb pairs of cell maps a1, a2 (nb x h8 x w8 each, host; the 8 x 8 means of the frames)
+ * against target (nb x h8 x w8 x 2, cells): the mean Charbonnier end-point error
+ * sqrt(ex^2 + ey^2 + 0.01^2) - 0.01 into *loss, the gradient of every parameter into grad (n, flat
+ * layout), and the prediction (nb x h8 x w8 x 2, cells) into pred if not NULL. */
+int ofx_coarse_net_gradient(ofx_engine *e, const float *params, size_t n, const float *a1, const float *a2, int nb,
+                            int w8, int h8, const float *target, double *loss, float *grad, float *pred);
+
+/* Training on pairs from the renderer, on the GPU: each step renders a batch of pairs (crops of
+ * crop x crop px of one of the n_pat speckle patterns (pw x ph each, host, values in [0, 1]) under
+ * a field of 6 Gaussian bumps over a 4000 x 3000 px frame whose largest |u| is uniform in
+ * [max_disp / 2, max_disp] px, 8-bit frames with noise of sd `noise`),
+ * runs forward and backward and takes an Adam step (learning rate from lr down to lr_final by a
+ * cosine). Sample k of the n_pairs distinct pairs is step k / batch of an epoch; the samples are a
+ * function of seed. params: initial parameters in, trained parameters out (flat layout); loss:
+ * the loss of every step (steps doubles), may be NULL. */
+typedef struct ofx_net_train {
+    int steps, batch, crop, n_pairs;
+    double lr, lr_final, max_disp, noise;
+    uint64_t seed;
+} ofx_net_train;
+int ofx_coarse_net_train(ofx_engine *e, const ofx_net_train *t, const float *patterns, int n_pat, int pw, int ph,
+                         float *params, double *loss);
 
 /* ---------------------------------------------------------------- output folder (host only)
  * The reference CLI's output side (rust/crates/optflow-cli/src/crop.rs, run.rs:74-110,

@@ -25,7 +25,8 @@ SYMBOLS = ("ofx_config_default", "ofx_create", "ofx_destroy", "ofx_last_error", 
            "ofx_tavg_window", "ofx_frame_to_gray", "ofx_median", "ofx_median_device_noalloc",
            "ofx_track_sequence", "ofx_profile", "ofx_profile_read", "ofx_profile_stage", "ofx_crop_plan",
            "ofx_write_crop_json", "ofx_write_result", "ofx_pyramid_levels", "ofx_poc_level_window",
-           "ofx_format_json_number")
+           "ofx_format_json_number", "ofx_coarse_net_params", "ofx_coarse_net_load", "ofx_coarse_net_propose",
+           "ofx_coarse_net_gradient", "ofx_coarse_net_train")
 
 
 class OfxConfig(ctypes.Structure):
@@ -39,7 +40,7 @@ class OfxConfig(ctypes.Structure):
                 ("strain_win", ctypes.c_int), ("ref_update_conf", ctypes.c_double), ("seq_hybrid", ctypes.c_int),
                 ("fb_max", ctypes.c_int), ("fb_radius", ctypes.c_double), ("fb_min_support", ctypes.c_int),
                 ("fb_tol", ctypes.c_double), ("prewarp_freeze_at", ctypes.c_int),
-                ("tile", ctypes.c_int)]
+                ("tile", ctypes.c_int), ("coarse_learned", ctypes.c_int), ("coarse_gate", ctypes.c_double)]
 
 
 _FP = ctypes.POINTER(ctypes.c_float)
@@ -51,7 +52,8 @@ class OfxResult(ctypes.Structure):
         ("valid", ctypes.POINTER(ctypes.c_uint8)), ("fb", ctypes.POINTER(ctypes.c_int32)),
         ("t_coarse", ctypes.c_double), ("t_fine", ctypes.c_double), ("t_strain", ctypes.c_double),
         ("reference", ctypes.c_int), ("mean_conf", ctypes.c_double), ("n_fb_max", ctypes.c_int),
-        ("fallback_frac", ctypes.c_double), ("fb_gate_rejected", ctypes.c_double), ("reacquired", ctypes.c_double)]
+        ("fallback_frac", ctypes.c_double), ("fb_gate_rejected", ctypes.c_double), ("reacquired", ctypes.c_double),
+        ("coarse_confirmed", ctypes.c_double), ("coarse_fallback", ctypes.c_int)]
 
 
 _lib = None
@@ -67,6 +69,7 @@ def lib():
         _lib = ctypes.CDLL(LIB_PATH)
         _lib.ofx_last_error.restype = ctypes.c_char_p
         _lib.ofx_profile_stage.restype = ctypes.c_char_p
+        _lib.ofx_coarse_net_params.restype = ctypes.c_size_t
     return _lib
 
 
@@ -74,13 +77,14 @@ def to_c(cfg: Config) -> OfxConfig:
     cfg.validate()
     c = OfxConfig()
     for name, _ in OfxConfig._fields_:
-        if name in ("n_scales", "scales", "seq_hybrid"):
+        if name in ("n_scales", "scales", "seq_hybrid", "coarse_learned"):
             continue
         setattr(c, name, getattr(cfg, name))
     c.n_scales = len(cfg.scales)
     for i, s in enumerate(cfg.scales):
         c.scales[i] = s
     c.seq_hybrid = 1 if cfg.seq_mode == "hybrid" else 0
+    c.coarse_learned = 1 if cfg.coarse_model == "learned" else 0
     return c
 
 
@@ -118,6 +122,20 @@ class Result:
     fb: np.ndarray = None  # hybrid sequences: consecutive frames carried by the adjacent leg
 
 
+class OfxNetTrain(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_int), ("batch", ctypes.c_int), ("crop", ctypes.c_int), ("n_pairs", ctypes.c_int),
+                ("lr", ctypes.c_double), ("lr_final", ctypes.c_double), ("max_disp", ctypes.c_double),
+                ("noise", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+COARSE_NET_FILE = os.path.join(_HERE, "weights", "coarse_net.npy")
+
+
+def coarse_net_params():
+    """Number of parameters of the learned coarse network (flat layout of include/ofx.h)."""
+    return int(lib().ofx_coarse_net_params())
+
+
 class OfxPair(ctypes.Structure):
     _fields_ = [("i1", ctypes.c_void_p), ("i2", ctypes.c_void_p), ("w", ctypes.c_int), ("h", ctypes.c_int),
                 ("mask", ctypes.c_void_p), ("init_ux", ctypes.c_void_p), ("init_uy", ctypes.c_void_p)]
@@ -141,6 +159,8 @@ def _to_result(r, arrs, valid):
     strain = {"Exx": arrs["exx"], "Eyy": arrs["eyy"], "Exy": arrs["exy"], "gmax": arrs["gmax"], "vol": arrs["vol"]}
     timing = {"coarse": r.t_coarse, "fine": r.t_fine, "strain": r.t_strain, "reference": r.reference,
               "mean_conf": r.mean_conf, "device": "gpu"}
+    if r.coarse_confirmed >= 0:
+        timing.update(coarse_confirmed=r.coarse_confirmed, coarse_fallback=bool(r.coarse_fallback))
     fb = arrs.get("fb")
     if fb is not None:
         timing.update(n_fb_max=r.n_fb_max, fallback_frac=r.fallback_frac, fb_gate_rejected=r.fb_gate_rejected,
@@ -281,7 +301,8 @@ class Engine:
         _check(lib().ofx_track_pair_device(self._h, ctypes.c_void_p(d_i1), ctypes.c_void_p(d_i2), int(w), int(h),
                                            ctypes.c_void_p(d_mask or None), ctypes.c_void_p(d_init[0] or None),
                                            ctypes.c_void_p(d_init[1] or None), ctypes.byref(c), ctypes.byref(r)))
-        return {"coarse": r.t_coarse, "fine": r.t_fine, "strain": r.t_strain}
+        return {"coarse": r.t_coarse, "fine": r.t_fine, "strain": r.t_strain, "coarse_confirmed": r.coarse_confirmed,
+                "coarse_fallback": bool(r.coarse_fallback)}
 
     def track_sequence(self, frames, cfg, mask=None):
         """track.py:260-315 (seq_mode chain) and track.py:383-505 (hybrid; results carry fb and the
@@ -355,6 +376,59 @@ class Engine:
         torch's data_ptr())."""
         _check(lib().ofx_median_device_noalloc(self._h, ctypes.c_void_p(d_values or None), ctypes.c_size_t(int(n)),
                                                ctypes.c_void_p(d_out or None)))
+
+    def load_coarse_net(self, params=None):
+        """Loads the learned coarse network (coarse_model "learned"): a flat float32 parameter
+        vector, or by default the trained one shipped with the package (weights/coarse_net.npy)."""
+        if params is None:
+            if not os.path.exists(COARSE_NET_FILE):
+                raise RuntimeError(f"{COARSE_NET_FILE} is missing: train the network with "
+                                   "`python -m optflow_soil_amd.train_coarse_net`")
+            params = np.load(COARSE_NET_FILE, allow_pickle=False)
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        _check(lib().ofx_coarse_net_load(self._h, p.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(p.size)))
+
+    def coarse_net_propose(self, i1, i2):
+        """The loaded network's proposal for a pair: ((h + 7) // 8, (w + 7) // 8, 2), pixels."""
+        i1, i2 = _f32(i1), _f32(i2)
+        h, w = i1.shape
+        if i2.shape != (h, w):
+            raise ValueError("images differ in size")
+        out = np.empty(((h + 7) // 8, (w + 7) // 8, 2), np.float32)
+        _check(lib().ofx_coarse_net_propose(self._h, _p(i1), _p(i2), w, h, _p(out)))
+        return out
+
+    def coarse_net_gradient(self, params, a1, a2, target):
+        """Loss, parameter gradient (flat) and prediction of one forward and backward pass on a
+        batch of cell maps a1, a2 (nb, h8, w8) with target (nb, h8, w8, 2), in cells."""
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel()
+        a1, a2 = _f32(a1), _f32(a2)
+        t = _f32(target)
+        nb, h8, w8 = a1.shape
+        if a2.shape != a1.shape or t.shape != (nb, h8, w8, 2):
+            raise ValueError("a1, a2 must be (nb, h8, w8) and target (nb, h8, w8, 2)")
+        loss = ctypes.c_double(0)
+        grad = np.empty_like(p)
+        pred = np.empty((nb, h8, w8, 2), np.float32)
+        _check(lib().ofx_coarse_net_gradient(self._h, _p(p), ctypes.c_size_t(p.size), _p(a1), _p(a2), nb, w8, h8,
+                                             _p(t), ctypes.byref(loss), _p(grad), _p(pred)))
+        return loss.value, grad, pred
+
+    def coarse_net_train(self, params, patterns, steps, batch=32, crop=512, n_pairs=20000, lr=2e-3, lr_final=2e-5,
+                         max_disp=48.0, noise=0.01, seed=0):
+        """Trains the network on the GPU on pairs from the renderer (ofx_coarse_net_train); returns
+        (trained parameters, loss of every step)."""
+        p = np.ascontiguousarray(params, dtype=np.float32).ravel().copy()
+        pats = _f32(patterns)
+        if pats.ndim == 2:
+            pats = pats[None]
+        n_pat, ph, pw = pats.shape
+        t = OfxNetTrain(int(steps), int(batch), int(crop), int(n_pairs), float(lr), float(lr_final), float(max_disp),
+                        float(noise), int(seed))
+        loss = np.zeros(int(steps), np.float64)
+        _check(lib().ofx_coarse_net_train(self._h, ctypes.byref(t), _p(np.ascontiguousarray(pats)), n_pat, pw, ph,
+                                          _p(p), _p(loss)))
+        return p, loss
 
     def sequence(self, frame0, cfg, mask=None):
         """Opens a streaming sequence on frame 0 (the Sequence iterator of track.rs:1112-1225): a

@@ -6,7 +6,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libofx.so")
-SOURCES = ["copy.hip", "fft.hip", "fine.hip", "guard.hip", "poc.hip", "seq.hip", "hybrid.hip", "tavg.hip", "engine.hip", "output.cpp"]
+SOURCES = ["copy.hip", "fft.hip", "fine.hip", "guard.hip", "poc.hip", "seq.hip", "hybrid.hip", "tavg.hip", "cnn.hip", "engine.hip", "output.cpp"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
          # packed FP32 instructions stay off: they have given wrong results on this hardware

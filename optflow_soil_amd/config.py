@@ -61,6 +61,13 @@ class Config:
     # once; N > 0 refines frames with a side over N px in N x N cores with the reference's margin,
     # as its tiled driver does with N = 1024 (rust/crates/optflow-core/src/tile.rs:56-98, track.rs:583).
     tile: int = 0
+    # Not keys of the reference's config either: the coarse stage. "poc" is the reference's POC
+    # pyramid; "learned" starts the level-0 POC grid from the proposal of the engine's learned
+    # coarse network instead of the coarser pyramid levels, and falls back to the pyramid when fewer
+    # than coarse_gate of the grid points confirm the proposal (a POC peak of at least poc_peak_min
+    # and a result within poc_win / 8 px of the proposal).
+    coarse_model: str = "poc"
+    coarse_gate: float = 0.9
 
     def to_json(self) -> str:
         return json.dumps(asdict(self), indent=2)
@@ -148,6 +155,10 @@ class Config:
             raise ValueError("prewarp_freeze_at must be >= 0")
         if not isinstance(self.tile, int) or isinstance(self.tile, bool) or (self.tile != 0 and self.tile < 64):
             raise ValueError("tile must be 0 (whole frame) or an integer >= 64")
+        if self.coarse_model not in ("poc", "learned"):
+            raise ValueError(f"coarse_model must be 'poc' or 'learned', got {self.coarse_model!r}")
+        if not _is_finite(self.coarse_gate) or not 0 <= self.coarse_gate <= 1:
+            raise ValueError("coarse_gate must lie in [0, 1]")
         if (self.n_orient * len(self.scales) > 64 or self.sigma_w > 256 or self.sigma_s > 256
                 or self.strain_win > 255 or self.poc_win > 1024):
             raise ValueError("at most 64 filters (n_orient * len(scales)); sigma_w, sigma_s <= 256, "

@@ -17,6 +17,7 @@
 #include <memory>
 #include <mutex>
 
+#include "cnn.hpp"
 #include "fine.hpp"
 
 namespace ofx {
@@ -298,6 +299,9 @@ struct Workspace {
     DevBuf banch, bgood_prev, bupx, bupy, bnfb, bpadj_x, bpadj_y, badj_x, badj_y, badj_c, blink, bflags, bax, bay,
         bcax, bcay, bca_adj, bgate0, bgate, bhcnt, bgscr, bhlow, bhux, bhuy, bfb;
     CoarseWs *coarse = nullptr;
+    // the engine's learned coarse network (read only during a call) and this lane's activations
+    const CoarseNet *net = nullptr;
+    std::unique_ptr<NetActs> nacts;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     // the tiled fine stage: the workspaces and streams of the tile groups that run beside the
     // lane's own stream, and the events that fork them from it and join them back (first use)
@@ -329,6 +333,7 @@ struct Lane {
 struct ofx_engine {
     int device = 0;
     std::vector<std::unique_ptr<ofx::Lane>> lanes;
+    std::unique_ptr<ofx::CoarseNet> net;  // the learned coarse network (ofx_coarse_net_load)
 };
 
 namespace ofx {
@@ -349,6 +354,7 @@ struct LaneGuard {
         if (!lane) throw Error(OFX_EBUSY, "every lane of the GPU engine is busy");
         HIPCHECK(hipSetDevice(e->device));
         if (!lane->ws) lane->ws.reset(new Workspace);
+        lane->ws->net = e->net.get();
     }
     ~LaneGuard() {
         if (lane) lane->busy.store(false);
@@ -392,6 +398,8 @@ void validate(const ofx_config *cfg) {
     if (!(c.conf_min >= 0 && c.conf_min <= 1) || !(c.poc_peak_min >= 0 && c.poc_peak_min <= 1))
         throw Error(OFX_EINVAL, "conf_min and poc_peak_min must lie in [0, 1]");
     if (c.prewarp_freeze_at < 0) throw Error(OFX_EINVAL, "prewarp_freeze_at must be >= 0");
+    if ((c.coarse_learned != 0 && c.coarse_learned != 1) || !(c.coarse_gate >= 0 && c.coarse_gate <= 1))
+        throw Error(OFX_EINVAL, "coarse_learned must be 0 or 1 and coarse_gate must lie in [0, 1]");
     if (c.tile != 0 && c.tile < 64) throw Error(OFX_EINVAL, "tile must be 0 (whole frame) or an integer >= 64");
     if (c.n_orient * c.n_scales > 64 || c.sigma_w > 256 || c.sigma_s > 256 || c.strain_win > 255 || c.poc_win > 1024)
         throw Error(OFX_EINVAL,
@@ -910,6 +918,36 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
+// The coarse stage of a pair: the POC pyramid (poc.rs:694-766), or with coarse_learned (and no dense
+// guess) the level-0 grid started from the learned proposal, the pyramid running instead when
+// fewer than coarse_gate of the grid points confirm the proposal (a POC peak of at least
+// poc_peak_min and a result within poc_win / 8 px of it). The outcome goes to o->coarse_*.
+void coarse_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, const ofx_config &c,
+                  const double *init_x, const double *init_y, double *ux, double *uy, ofx_result *o, hipStream_t s) {
+    o->coarse_confirmed = -1.0;
+    o->coarse_fallback = 0;
+    if (!c.coarse_learned || init_x) {
+        coarse_dense(i1, i2, w, h, c, init_x, init_y, ux, uy, ws.coarse, s);
+        return;
+    }
+    if (!ws.net) throw Error(OFX_EINVAL, "coarse_learned needs a network: load one with ofx_coarse_net_load");
+    if (!ws.nacts) ws.nacts.reset(new NetActs);
+    NetActs &A = *ws.nacts;
+    net_propose(*ws.net, A, i1, i2, w, h, s);
+    int confirmed = 0;
+    const int np = coarse_dense_from(
+        i1, i2, w, h, c,
+        [&](int x0, int y0, int step, int nx, int ny, double *ix, double *iy) {
+            prop_to_grid(A, x0, y0, step, nx, ny, ix, iy, s);
+        },
+        c.poc_win / 8.0, ux, uy, ws.coarse, &confirmed, s);
+    o->coarse_confirmed = (double)confirmed / np;
+    if (o->coarse_confirmed < c.coarse_gate) {
+        o->coarse_fallback = 1;
+        coarse_dense(i1, i2, w, h, c, nullptr, nullptr, ux, uy, ws.coarse, s);
+    }
+}
+
 // The pair pipeline (track.py:115-146) on device inputs; init as double (or NULL); outputs to
 // device result arrays. Timing by events on the lane's stream.
 void pair_device(Workspace &ws, const float *i1, const float *i2, int w, int h, const uint8_t *mask,
@@ -921,7 +959,7 @@ void pair_device(Workspace &ws, const float *i1, const float *i2, int w, int h, 
     ws.prof.mark(ST_COARSE, s);
     double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
     step_mark("pair: coarse stage");
-    coarse_dense(i1, i2, w, h, c, init_x, init_y, ux, uy, ws.coarse, s);
+    coarse_stage(ws, i1, i2, w, h, c, init_x, init_y, ux, uy, o, s);
     HIPCHECK(hipEventRecord(ws.ev[1], s));
     step_mark("pair: fine stage");
     fine_stage(ws, i1, i2, w, h, c, s, false);
@@ -1020,6 +1058,8 @@ void track_pair_on_lane(Lane &lane, const float *i1, const float *i2, int w, int
     out->t_coarse = st.d.t_coarse;
     out->t_fine = st.d.t_fine;
     out->t_strain = st.d.t_strain;
+    out->coarse_confirmed = st.d.coarse_confirmed;
+    out->coarse_fallback = st.d.coarse_fallback;
     out->reference = 0;
     out->mean_conf = 0;
 }
@@ -1457,6 +1497,8 @@ void ofx_config_default(ofx_config *c) {
     c->fb_min_support = 20;
     c->fb_tol = 0.5;
     c->prewarp_freeze_at = 0;
+    c->coarse_learned = 0;
+    c->coarse_gate = 0.9;
 }
 
 const char *ofx_last_error(void) { return g_last_error.c_str(); }
@@ -1738,6 +1780,143 @@ int ofx_frame_to_gray(ofx_engine *e, const void *pixels, int bits, int channels,
         frame_to_gray(dp, bits, channels, w, h, factor, dout, s);
         copy_d2h(ws.pin, out, dout, (size_t)ow * oh * sizeof(float), s);
         wait_stream(s);
+    });
+}
+
+size_t ofx_coarse_net_params(void) { return net_param_count(); }
+
+int ofx_coarse_net_load(ofx_engine *e, const float *params, size_t n) {
+    return guarded([&] {
+        if (!e) throw Error(OFX_EINVAL, "engine is NULL");
+        if (!params) throw Error(OFX_EINVAL, "params is NULL");
+        if (n != net_param_count())
+            throw Error(OFX_EINVAL, "the coarse network has " + std::to_string(net_param_count()) + " parameters, got " +
+                                        std::to_string(n));
+        for (size_t i = 0; i < n; i++)
+            if (!std::isfinite(params[i])) throw Error(OFX_EINVAL, "parameter " + std::to_string(i) + " is not finite");
+        // every lane held while the network changes
+        std::vector<Lane *> held;
+        struct Release {
+            std::vector<Lane *> &h;
+            ~Release() {
+                for (Lane *l : h) l->busy.store(false);
+            }
+        } release{held};
+        for (auto &l : e->lanes) {
+            bool expected = false;
+            if (!l->busy.compare_exchange_strong(expected, true))
+                throw Error(OFX_EBUSY, "a call is running on the engine: the network cannot change now");
+            held.push_back(l.get());
+        }
+        HIPCHECK(hipSetDevice(e->device));
+        std::vector<float> m;
+        net_flat_to_master(params, m);
+        if (!e->net) e->net.reset(new CoarseNet);
+        e->net->upload(m, e->lanes[0]->stream);
+        for (auto &l : e->lanes)
+            if (l->ws) l->ws->net = e->net.get();
+    });
+}
+
+int ofx_coarse_net_propose(ofx_engine *e, const float *i1, const float *i2, int w, int h, float *out) {
+    return guarded([&] {
+        if (!i1 || !i2 || !out) throw Error(OFX_EINVAL, "input or output pointer is NULL");
+        if (w < 1 || h < 1) throw Error(OFX_EINVAL, "empty frame");
+        LaneGuard g(e);
+        Workspace &ws = *g.lane->ws;
+        hipStream_t s = g.lane->stream;
+        if (!ws.net) throw Error(OFX_EINVAL, "no coarse network loaded (ofx_coarse_net_load)");
+        const size_t n = (size_t)w * h;
+        float *d1 = B<float>(ws.bi1, n), *d2 = B<float>(ws.bi2, n);
+        copy_h2d(ws.pin, d1, i1, n * sizeof(float), s);
+        copy_h2d(ws.pin, d2, i2, n * sizeof(float), s);
+        if (!ws.nacts) ws.nacts.reset(new NetActs);
+        net_propose(*ws.net, *ws.nacts, d1, d2, w, h, s);
+        const size_t nc = (size_t)ws.nacts->w8 * ws.nacts->h8 * 2;
+        copy_d2h(ws.pin, out, ws.nacts->pred, nc * sizeof(float), s);
+        wait_stream(s);
+        for (size_t i = 0; i < nc; i++) out[i] *= 8.f;
+    });
+}
+
+int ofx_coarse_net_gradient(ofx_engine *e, const float *params, size_t n, const float *a1, const float *a2, int nb,
+                            int w8, int h8, const float *target, double *loss, float *grad, float *pred) {
+    return guarded([&] {
+        if (!params || !a1 || !a2 || !target || !loss || !grad) throw Error(OFX_EINVAL, "input or output pointer is NULL");
+        if (n != net_param_count()) throw Error(OFX_EINVAL, "wrong number of parameters");
+        if (nb < 1 || w8 < 1 || h8 < 1) throw Error(OFX_EINVAL, "empty batch");
+        LaneGuard g(e);
+        Workspace &ws = *g.lane->ws;
+        hipStream_t s = g.lane->stream;
+        std::vector<float> m;
+        net_flat_to_master(params, m);
+        CoarseNet net;
+        net.upload(m, s);
+        NetTrainer tr;
+        tr.ensure(nb, h8, w8);
+        const size_t nc = (size_t)nb * w8 * h8;
+        copy_h2d(ws.pin, tr.A.a1, a1, nc * sizeof(float), s);
+        copy_h2d(ws.pin, tr.A.a2, a2, nc * sizeof(float), s);
+        copy_h2d(ws.pin, tr.target, target, 2 * nc * sizeof(float), s);
+        GrowBuf lb;
+        double *dl = lb.get<double>(1);
+        tr.forward_backward(net, dl, s);
+        copy_d2h(ws.pin, loss, dl, sizeof(double), s);
+        std::vector<float> gm(net_master_floats());
+        copy_d2h(ws.pin, gm.data(), tr.grad, gm.size() * sizeof(float), s);
+        if (pred) copy_d2h(ws.pin, pred, tr.A.pred, 2 * nc * sizeof(float), s);
+        wait_stream(s);
+        net_master_to_flat(gm, grad);
+    });
+}
+
+int ofx_coarse_net_train(ofx_engine *e, const ofx_net_train *t, const float *patterns, int n_pat, int pw, int ph,
+                         float *params, double *loss) {
+    return guarded([&] {
+        if (!t || !patterns || !params) throw Error(OFX_EINVAL, "input or output pointer is NULL");
+        if (t->steps < 1 || t->batch < 1 || t->n_pairs < t->batch || t->n_pairs % t->batch)
+            throw Error(OFX_EINVAL, "steps >= 1, batch >= 1 and n_pairs a positive multiple of batch required");
+        if (t->crop < 64 || t->crop % 8) throw Error(OFX_EINVAL, "crop must be a multiple of 8 and >= 64");
+        if (!(t->lr > 0) || !(t->lr_final >= 0) || !(t->max_disp >= 0) || !(t->noise >= 0))
+            throw Error(OFX_EINVAL, "lr > 0, lr_final >= 0, max_disp >= 0 and noise >= 0 required");
+        if (n_pat < 1 || pw < t->crop + 129 || ph < t->crop + 129)
+            throw Error(OFX_EINVAL, "patterns must exceed the crop by more than 128 px a side");
+        LaneGuard g(e);
+        Workspace &ws = *g.lane->ws;
+        hipStream_t s = g.lane->stream;
+        step_mark("coarse net: setting up the training");
+        std::vector<float> m;
+        net_flat_to_master(params, m);
+        CoarseNet net;
+        net.upload(m, s);
+        GrowBuf pb, sb, lb;
+        float *pats = pb.get<float>((size_t)n_pat * pw * ph);
+        copy_h2d(ws.pin, pats, patterns, (size_t)n_pat * pw * ph * sizeof(float), s);
+        std::vector<TrainSample> smp;
+        make_train_samples(t->seed, 0, t->n_pairs, n_pat, pw, ph, t->crop, t->max_disp, smp);
+        TrainSample *dsmp = sb.get<TrainSample>(smp.size());
+        copy_h2d(ws.pin, dsmp, smp.data(), smp.size() * sizeof(TrainSample), s);
+        double *dloss = lb.get<double>(t->steps);
+        const int c8 = t->crop / 8;
+        NetTrainer tr;
+        tr.ensure(t->batch, c8, c8);
+        tr.zero_moments(s);
+        step_mark("coarse net: training steps");
+        for (int k = 0; k < t->steps; k++) {
+            const int first = (int)(((long)k * t->batch) % t->n_pairs);
+            render_train_pairs(pats, pw, ph, dsmp + first, t->batch, c8, c8, (float)t->noise, tr.A.a1, tr.A.a2,
+                               tr.target, s);
+            tr.forward_backward(net, dloss + k, s);
+            const double f = t->steps > 1 ? (double)k / (t->steps - 1) : 1.0;
+            const double lr = t->lr_final + 0.5 * (t->lr - t->lr_final) * (1.0 + std::cos(M_PI * f));
+            tr.adam_step(net, k + 1, lr, s);
+        }
+        step_mark("coarse net: reading the trained parameters");
+        std::vector<float> out(net_master_floats());
+        copy_d2h(ws.pin, out.data(), net.master, out.size() * sizeof(float), s);
+        if (loss) copy_d2h(ws.pin, loss, dloss, (size_t)t->steps * sizeof(double), s);
+        wait_stream(s);
+        net_master_to_flat(out, params);
     });
 }
 

@@ -70,6 +70,10 @@ struct CoarseWs;
 void coarse_dense(const float *i1, const float *i2, int w, int h, const ofx_config &c, const double *init_ux,
                   const double *init_uy, double *ux, double *uy, CoarseWs *&ws, hipStream_t s);
 void coarse_release(CoarseWs *&ws);
+// the start of a grid (x0 + j step, y0 + i step; nx x ny) into ix, iy (device), in pixels
+using GuessFn = std::function<void(int x0, int y0, int step, int nx, int ny, double *ix, double *iy)>;
+int coarse_dense_from(const float *i1, const float *i2, int w, int h, const ofx_config &c, const GuessFn &guess,
+                      double tol, double *ux, double *uy, CoarseWs *&ws, int *confirmed, hipStream_t s);
 void coarse_grid(const CoarseWs *ws, int w, int h, const ofx_config &c, std::vector<int> &xs, std::vector<int> &ys,
                  std::vector<double> &ux, std::vector<double> &uy, std::vector<double> &peak, PinnedBuf &pin,
                  hipStream_t s);

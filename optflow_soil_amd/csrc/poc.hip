@@ -470,6 +470,17 @@ __device__ double median_small(double *v, int n) {
     return (n & 1) ? v[n / 2] : 0.5 * (v[n / 2 - 1] + v[n / 2]);
 }
 
+// points of a raw grid that confirm their start: peak >= peak_min and |result - start| <= tol on
+// both axes (one count per wavefront)
+__global__ void k_raw_check(const double *__restrict__ gux, const double *__restrict__ guy,
+                            const double *__restrict__ gpk, const double *__restrict__ ix, const double *__restrict__ iy,
+                            int n, double peak_min, double tol, int *count) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool ok = k < n && gpk[k] >= peak_min && fabs(gux[k] - ix[k]) <= tol && fabs(guy[k] - iy[k]) <= tol;
+    const unsigned long long m = __ballot(ok);
+    if ((threadIdx.x & 63) == 0 && m) atomicAdd(count, (int)__popcll(m));
+}
+
 __global__ void k_zero_int(int *p) {
     if (threadIdx.x == 0) p[0] = 0;
 }
@@ -673,6 +684,29 @@ void ensure(T *&p, size_t n, size_t &cap) {
     cap = n;
 }
 
+// the grid buffers of the workspace, sized for npmax points
+void ensure_grid(CoarseWs *ws, size_t npmax) {
+    if (ws->gux && ws->gcap >= npmax) return;
+    release_device(ws->gux); release_device(ws->guy); release_device(ws->gpk);
+    release_device(ws->ix); release_device(ws->iy); release_device(ws->gflag); release_device(ws->gwork);
+    HIPCHECK(hipMalloc(&ws->gux, npmax * sizeof(double) * 2));
+    HIPCHECK(hipMalloc(&ws->guy, npmax * sizeof(double) * 2));
+    HIPCHECK(hipMalloc(&ws->gpk, npmax * sizeof(double)));
+    HIPCHECK(hipMalloc(&ws->ix, npmax * sizeof(double)));
+    HIPCHECK(hipMalloc(&ws->iy, npmax * sizeof(double)));
+    HIPCHECK(hipMalloc(&ws->gflag, npmax * sizeof(int)));
+    HIPCHECK(hipMalloc(&ws->gwork, (2 * npmax + 64) * sizeof(int)));
+    ws->gcap = npmax;
+}
+
+// One pyramid level (poc.rs:738-762): the POC grid of a, b (lw x lh) with windows wy x wx at the
+// points of `cur`, started from ws->ix / ws->iy when have_init; the raw grid is in ws->gux, guy,
+// gpk, and with raw_check (not NULL) the points whose peak is at least poc_peak_min and whose
+// result lies within tol px of the start on both axes are counted into *raw_check; then `clean`
+// with that start as the fallback.
+void poc_level(const float *a, const float *b, int lw, int lh, int wy, int wx, const GridGeom &cur, int have_init,
+               const ofx_config &c, CoarseWs *ws, int *raw_check, double tol, hipStream_t s);
+
 }  // namespace
 
 // The level-0 grid of the last coarse_dense call with this workspace: positions and (ux, uy,
@@ -712,6 +746,109 @@ void coarse_release(CoarseWs *&ws) {
     (void)hipFree(ws->hann);
     delete ws;
     ws = nullptr;
+}
+
+namespace {
+void poc_level(const float *a, const float *b, int lw, int lh, int wy, int wx, const GridGeom &cur, int have_init,
+               const ofx_config &c, CoarseWs *ws, int *raw_check, double tol, hipStream_t s) {
+    const int x0 = cur.x0, y0 = cur.y0, nx = cur.nx, ny = cur.ny;
+    const size_t np = (size_t)nx * ny;
+    const FftPlan &PX = fft_plan(wx, s), &PY = fft_plan(wy, s);
+    PocArgs P;
+    std::memset(&P, 0, sizeof(P));
+    P.W = lw;
+    P.H = lh;
+    P.wy = wy;
+    P.wx = wx;
+    P.x0 = x0;
+    P.y0 = y0;
+    P.step = c.grid_step;
+    P.nx = nx;
+    P.ny = ny;
+    P.nfx = PX.nfac;
+    P.nfy = PY.nfac;
+    for (int i = 0; i < MAX_FACTORS; i++) {
+        P.facx[i] = PX.fac[i];
+        P.facy[i] = PY.fac[i];
+    }
+    P.have_init = have_init;
+    hipLaunchKernelGGL(k_hann, dim3((wy + wx + 255) / 256), dim3(256), 0, s, ws->hann, wy, wx);
+    LAUNCHCHECK();
+    if (wy == 64 && wx == 64) {
+        // 67.6 KiB: above the 64 KiB default, within the 160 KiB of a CDNA4 workgroup
+        const size_t lds = (2 * (size_t)64 * T64 + 64) * sizeof(float2) + 128 * sizeof(float);
+        static std::once_flag once;
+        std::call_once(once, [] {
+            (void)hipFuncSetAttribute((const void *)k_poc_grid<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                      160 * 1024);
+        });
+        hipLaunchKernelGGL(k_poc_grid<64>, dim3((unsigned)np), dim3(POC_NT), lds, s, a, b, P, PX.tw, PY.tw,
+                           ws->hann, ws->ix, ws->iy, ws->gux, ws->guy, ws->gpk);
+    } else {
+        const size_t lds = (2 * (size_t)wy * wx + wx + wy) * sizeof(float2) + (size_t)(wy + wx) * sizeof(float);
+        if (lds > 65536) {
+            static std::once_flag once;
+            std::call_once(once, [] {
+                (void)hipFuncSetAttribute((const void *)k_poc_grid<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                          160 * 1024);
+            });
+        }
+        hipLaunchKernelGGL(k_poc_grid<0>, dim3((unsigned)np), dim3(POC_NT), lds, s, a, b, P, PX.tw, PY.tw,
+                           ws->hann, ws->ix, ws->iy, ws->gux, ws->guy, ws->gpk);
+    }
+    LAUNCHCHECK();
+    if (raw_check) {
+        hipLaunchKernelGGL(k_raw_check, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, (const double *)ws->gux,
+                           (const double *)ws->guy, (const double *)ws->gpk, (const double *)ws->ix,
+                           (const double *)ws->iy, (int)np, c.poc_peak_min, tol, raw_check);
+        LAUNCHCHECK();
+    }
+    int *cnt = ws->gwork + 2 * ws->gcap;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, cnt);
+    hipLaunchKernelGGL(k_clean_test, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, (const double *)ws->gux,
+                       (const double *)ws->guy, (const double *)ws->gpk, ws->gflag, cnt, ny, nx, c.poc_peak_min,
+                       c.median_thresh);
+    hipLaunchKernelGGL(k_clean_fill, dim3(1), dim3(1024), 0, s, ws->gux, ws->guy, ws->gpk, ws->ix, ws->iy, have_init,
+                       ws->gflag, ws->gwork, (const int *)cnt, ny, nx);
+    LAUNCHCHECK();
+}
+}  // namespace
+
+// The coarse stage from a proposal in place of the pyramid: the level-0 POC grid started from the
+// proposal (guess fills the start at the grid points), cleaned with that start as the fallback,
+// then the dense start. Returns the number of grid points and, in *confirmed (host), how many of
+// them confirmed the proposal: a peak of at least poc_peak_min and a result within tol px of it.
+int coarse_dense_from(const float *i1, const float *i2, int w, int h, const ofx_config &c, const GuessFn &guess,
+                      double tol, double *ux, double *uy, CoarseWs *&ws, int *confirmed, hipStream_t s) {
+    if (!ws) {
+        ws = new CoarseWs;
+        HIPCHECK(hipMalloc(&ws->hann, 2 * (size_t)POC_MAX * sizeof(float)));
+    }
+    ws->w = w;
+    ws->h = h;
+    ws->L = 0;
+    const int wy = c.poc_win, wx = c.poc_win;
+    if ((long)wy * wx > POC_MAX)
+        throw Error(OFX_EUNSUPPORTED, "POC window " + std::to_string(wy) + "x" + std::to_string(wx) +
+                                          " exceeds the 4096 samples a workgroup holds");
+    int x0, nx, y0, ny;
+    grid_axis(w, wx, c.grid_step, x0, nx);
+    grid_axis(h, wy, c.grid_step, y0, ny);
+    const size_t np = (size_t)nx * ny;
+    if (np == 0) throw Error(OFX_EINVAL, "no POC grid point fits the image");
+    ensure_grid(ws, np);
+    const GridGeom cur{x0, y0, c.grid_step, nx, ny};
+    guess(x0, y0, c.grid_step, nx, ny, ws->ix, ws->iy);
+    int *dcount = ws->gwork + 2 * ws->gcap + 32;
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, dcount);
+    LAUNCHCHECK();
+    poc_level(i1, i2, w, h, wy, wx, cur, 1, c, ws, dcount, tol, s);
+    hipLaunchKernelGGL(k_dense, dim3((w + 255) / 256, h), dim3(256), 0, s, cur, (const double *)ws->gux,
+                       (const double *)ws->guy, (const double *)nullptr, (const double *)nullptr, w, h, ux, uy);
+    LAUNCHCHECK();
+    PinnedBuf pin;
+    copy_d2h(pin, confirmed, dcount, sizeof(int), s);
+    return (int)np;
 }
 
 void coarse_dense(const float *i1, const float *i2, int w, int h, const ofx_config &c, const double *init_ux,
@@ -770,18 +907,7 @@ void coarse_dense(const float *i1, const float *i2, int w, int h, const ofx_conf
         grid_axis(lh, wy, c.grid_step, y0, ny);
         npmax = std::max(npmax, (size_t)nx * ny);
     }
-    if (!ws->gux || ws->gcap < npmax) {
-        release_device(ws->gux); release_device(ws->guy); release_device(ws->gpk);
-        release_device(ws->ix); release_device(ws->iy); release_device(ws->gflag); release_device(ws->gwork);
-        HIPCHECK(hipMalloc(&ws->gux, npmax * sizeof(double) * 2));
-        HIPCHECK(hipMalloc(&ws->guy, npmax * sizeof(double) * 2));
-        HIPCHECK(hipMalloc(&ws->gpk, npmax * sizeof(double)));
-        HIPCHECK(hipMalloc(&ws->ix, npmax * sizeof(double)));
-        HIPCHECK(hipMalloc(&ws->iy, npmax * sizeof(double)));
-        HIPCHECK(hipMalloc(&ws->gflag, npmax * sizeof(int)));
-        HIPCHECK(hipMalloc(&ws->gwork, (2 * npmax + 64) * sizeof(int)));
-        ws->gcap = npmax;
-    }
+    ensure_grid(ws, npmax);
     GridGeom prev{};
     bool have_prev = false;
     for (int lev = L; lev >= 0; lev--) {
@@ -812,58 +938,7 @@ void coarse_dense(const float *i1, const float *i2, int w, int h, const ofx_conf
             have_init = 1;
         }
         LAUNCHCHECK();
-        const FftPlan &PX = fft_plan(wx, s), &PY = fft_plan(wy, s);
-        PocArgs P;
-        std::memset(&P, 0, sizeof(P));
-        P.W = lw;
-        P.H = lh;
-        P.wy = wy;
-        P.wx = wx;
-        P.x0 = x0;
-        P.y0 = y0;
-        P.step = c.grid_step;
-        P.nx = nx;
-        P.ny = ny;
-        P.nfx = PX.nfac;
-        P.nfy = PY.nfac;
-        for (int i = 0; i < MAX_FACTORS; i++) {
-            P.facx[i] = PX.fac[i];
-            P.facy[i] = PY.fac[i];
-        }
-        P.have_init = have_init;
-        hipLaunchKernelGGL(k_hann, dim3((wy + wx + 255) / 256), dim3(256), 0, s, ws->hann, wy, wx);
-        LAUNCHCHECK();
-        if (wy == 64 && wx == 64) {
-            // 67.6 KiB: above the 64 KiB default, within the 160 KiB of a CDNA4 workgroup
-            const size_t lds = (2 * (size_t)64 * T64 + 64) * sizeof(float2) + 128 * sizeof(float);
-            static std::once_flag once;
-            std::call_once(once, [] {
-                (void)hipFuncSetAttribute((const void *)k_poc_grid<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                          160 * 1024);
-            });
-            hipLaunchKernelGGL(k_poc_grid<64>, dim3((unsigned)np), dim3(POC_NT), lds, s, a, b, P, PX.tw, PY.tw,
-                               ws->hann, ws->ix, ws->iy, ws->gux, ws->guy, ws->gpk);
-        } else {
-            const size_t lds = (2 * (size_t)wy * wx + wx + wy) * sizeof(float2) + (size_t)(wy + wx) * sizeof(float);
-            if (lds > 65536) {
-                static std::once_flag once;
-                std::call_once(once, [] {
-                    (void)hipFuncSetAttribute((const void *)k_poc_grid<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                              160 * 1024);
-                });
-            }
-            hipLaunchKernelGGL(k_poc_grid<0>, dim3((unsigned)np), dim3(POC_NT), lds, s, a, b, P, PX.tw, PY.tw,
-                               ws->hann, ws->ix, ws->iy, ws->gux, ws->guy, ws->gpk);
-        }
-        LAUNCHCHECK();
-        int *cnt = ws->gwork + 2 * ws->gcap;
-        hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, s, cnt);
-        hipLaunchKernelGGL(k_clean_test, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, s, (const double *)ws->gux,
-                           (const double *)ws->guy, (const double *)ws->gpk, ws->gflag, cnt, ny, nx, c.poc_peak_min,
-                           c.median_thresh);
-        hipLaunchKernelGGL(k_clean_fill, dim3(1), dim3(1024), 0, s, ws->gux, ws->guy, ws->gpk, ws->ix, ws->iy, have_init,
-                           ws->gflag, ws->gwork, (const int *)cnt, ny, nx);
-        LAUNCHCHECK();
+        poc_level(a, b, lw, lh, wy, wx, cur, have_init, c, ws, nullptr, 0.0, s);
         prev = cur;
         have_prev = true;
     }
