@@ -239,6 +239,13 @@ int ofx_seq_step(ofx_sequence *seq, const float *frame, ofx_result *out);
 int ofx_seq_step_device(ofx_sequence *seq, const float *d_frame, ofx_result *d_out);
 void ofx_seq_end(ofx_sequence *seq);
 
+/* No counterpart in the reference. The whole-frame fine stage runs from a HIP graph once it has
+ * run eagerly with the same sizes, settings, inputs and buffers (captured on the second such call,
+ * replayed from the third; results bit-identical; OFX_GRAPHS=0 in the environment turns this off,
+ * and it is off while ofx_profile records). The number of captures and replays so far, summed over
+ * the engine's lanes. */
+int ofx_graph_stats(ofx_engine *e, int64_t *captures, int64_t *launches);
+
 /* ---------------------------------------------------------------- learned coarse proposal
  * No counterpart in the reference: a 6-layer 3 x 3 convolutional network in bfloat16 (32 hidden
  * channels, float32 accumulation on the matrix cores) that maps the correlation volume of the two

@@ -26,7 +26,7 @@ SYMBOLS = ("ofx_config_default", "ofx_create", "ofx_destroy", "ofx_last_error", 
            "ofx_track_sequence", "ofx_profile", "ofx_profile_read", "ofx_profile_stage", "ofx_crop_plan",
            "ofx_write_crop_json", "ofx_write_result", "ofx_pyramid_levels", "ofx_poc_level_window",
            "ofx_format_json_number", "ofx_coarse_net_params", "ofx_coarse_net_load", "ofx_coarse_net_propose",
-           "ofx_coarse_net_gradient", "ofx_coarse_net_train")
+           "ofx_coarse_net_gradient", "ofx_coarse_net_train", "ofx_graph_stats")
 
 
 class OfxConfig(ctypes.Structure):
@@ -207,6 +207,12 @@ class Engine:
         calls = (ctypes.c_int64 * len(names))()
         _check(lib().ofx_profile_read(self._h, ms, calls, len(names)))
         return {n: (ms[i], calls[i]) for i, n in enumerate(names)}
+
+    def graph_stats(self):
+        """(captures, replays) of the fine stage's HIP graphs on this engine so far."""
+        c, l = ctypes.c_int64(0), ctypes.c_int64(0)
+        _check(lib().ofx_graph_stats(self._h, ctypes.byref(c), ctypes.byref(l)))
+        return c.value, l.value
 
     def check_window(self, w, h, cfg):
         _check(lib().ofx_check_window(self._h, int(w), int(h), ctypes.byref(to_c(cfg))))

@@ -299,6 +299,15 @@ struct Workspace {
     DevBuf banch, bgood_prev, bupx, bupy, bnfb, bpadj_x, bpadj_y, badj_x, badj_y, badj_c, blink, bflags, bax, bay,
         bcax, bcay, bca_adj, bgate0, bgate, bhcnt, bgscr, bhlow, bhux, bhuy, bfb;
     CoarseWs *coarse = nullptr;
+    // captured fine stages (refine_graph): the keys seen once eagerly, and two graphs
+    std::vector<std::vector<uint64_t>> warm_keys;
+    struct FineGraph {
+        std::vector<uint64_t> key;
+        hipGraphExec_t exec = nullptr;
+    } graphs[2];
+    int graph_next = 0;
+    bool graphs_off = false;  // a capture failed: this workspace runs eagerly from then on
+    long long graph_launches = 0, graph_captures = 0;
     // the engine's learned coarse network (read only during a call) and this lane's activations
     const CoarseNet *net = nullptr;
     std::unique_ptr<NetActs> nacts;
@@ -310,6 +319,8 @@ struct Workspace {
     std::vector<hipEvent_t> sub_done;
     hipEvent_t fork = nullptr;
     ~Workspace() {
+        for (auto &g : graphs)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
         coarse_release(coarse);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -619,6 +630,106 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
     confidence(energy, resid, med, ux, uy, c.phase_tol, c.conf_min, Bw, w, h, nullptr, conf, conf64, s);
 }
 
+// ---------------------------------------------------------------- the fine stage as a graph
+// refine_device enqueues only kernels once its FFT plans, filter taps and buffers exist (no host
+// wait, no copy by the runtime, no memset), so after one eager call with given sizes, settings,
+// inputs and buffers, the second such call captures it into a HIP graph and every later one
+// replays that graph: one launch instead of about a thousand. The key holds everything the
+// captured launches depend on: the input pointers, every workspace buffer, the sizes and shift,
+// the settings' bytes, the cached-pack state and the exact-reset switch. A call whose key differs
+// runs eagerly (and warms its own key). Off with OFX_GRAPHS=0, with the stage profiler on, and when
+// the inner iterations are odd in number (the result would end in the other field buffer).
+bool graphs_enabled() {
+    const char *v = std::getenv("OFX_GRAPHS");
+    return !(v && v[0] == '0');
+}
+
+bool pack_cached(const Workspace &ws, int w, int h, const Bank &bank, bool cache_i1) {
+    return cache_i1 && ws.pack_valid && ws.pack_w == w && ws.pack_h == h && same_bank(ws.pack_bank, bank);
+}
+
+std::vector<uint64_t> fine_key(const Workspace &ws, const float *i1, const float *i2, int w, int h, int w2, int h2,
+                               long shx, long shy, const ofx_config &c, bool cache_i1) {
+    std::vector<uint64_t> k = {(uint64_t)(uintptr_t)i1, (uint64_t)(uintptr_t)i2, (uint64_t)w, (uint64_t)h,
+                               (uint64_t)w2, (uint64_t)h2, (uint64_t)shx, (uint64_t)shy, (uint64_t)cache_i1,
+                               (uint64_t)pack_cached(ws, w, h, make_bank(c), cache_i1),
+                               (uint64_t)(std::getenv("OFX_EXACT_ENERGY_RESET") != nullptr)};
+    for (const DevBuf *b : {&ws.bspec, &ws.btmp, &ws.btmp2, &ws.bresp, &ws.brefp, &ws.bpack, &ws.bc2, &ws.bpftmp,
+                            &ws.bux, &ws.buy, &ws.bnux, &ws.bnuy, &ws.bsx, &ws.bsy, &ws.bcsx, &ws.bcsy, &ws.bwarped,
+                            &ws.bfx, &ws.bfy, &ws.bbt, &ws.bsums, &ws.bym, &ws.benergy, &ws.bresid, &ws.bslope,
+                            &ws.bmed, &ws.bmedscr, &ws.blow, &ws.bkept, &ws.bwithin, &ws.bok, &ws.bcold2, &ws.bcolrow,
+                            &ws.bcnt, &ws.bconf, &ws.bconf64, &ws.bcolsum, &ws.btaps_s, &ws.btaps_w, &ws.bmom2}) {
+        k.push_back((uint64_t)(uintptr_t)b->p);
+        k.push_back((uint64_t)b->bytes);
+    }
+    uint64_t cw[(sizeof(ofx_config) + 7) / 8] = {0};
+    std::memcpy(cw, &c, sizeof(ofx_config));
+    k.insert(k.end(), cw, cw + (sizeof(ofx_config) + 7) / 8);
+    return k;
+}
+
+// refine_device, replayed from a graph where it can be (see above); results bit-identical.
+void refine_graph(Workspace &ws, const float *i1, const float *i2, int w, int h, int w2, int h2, long shx, long shy,
+                  const ofx_config &c, hipStream_t s, bool cache_i1) {
+    const bool can = graphs_enabled() && !ws.graphs_off && !ws.prof.on && (c.outer_iters * c.inner_iters) % 2 == 0;
+    if (!can) {
+        refine_device(ws, i1, i2, w, h, w2, h2, shx, shy, c, s, cache_i1);
+        return;
+    }
+    const std::vector<uint64_t> key = fine_key(ws, i1, i2, w, h, w2, h2, shx, shy, c, cache_i1);
+    for (auto &g : ws.graphs)
+        if (g.exec && g.key == key) {
+            step_mark("fine: replaying the captured fine stage");
+            HIPCHECK(hipGraphLaunch(g.exec, s));
+            ws.graph_launches++;
+            // the cached reference pack a capture with cache_i1 computed is valid after a replay too
+            if (cache_i1) {
+                ws.pack_valid = true;
+                ws.pack_w = w;
+                ws.pack_h = h;
+                ws.pack_bank = make_bank(c);
+            }
+            return;
+        }
+    bool warm = false;
+    for (const auto &k : ws.warm_keys) warm = warm || k == key;
+    if (!warm) {
+        // first call with this key: eager; it creates the plans, taps and buffers a capture needs
+        refine_device(ws, i1, i2, w, h, w2, h2, shx, shy, c, s, cache_i1);
+        const std::vector<uint64_t> after = fine_key(ws, i1, i2, w, h, w2, h2, shx, shy, c, cache_i1);
+        if (ws.warm_keys.size() >= 4) ws.warm_keys.erase(ws.warm_keys.begin());
+        ws.warm_keys.push_back(after);
+        return;
+    }
+    step_mark("fine: capturing the fine stage");
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    HIPCHECK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    try {
+        refine_device(ws, i1, i2, w, h, w2, h2, shx, shy, c, s, cache_i1);
+    } catch (...) {
+        (void)hipStreamEndCapture(s, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        ws.graphs_off = true;
+        throw;
+    }
+    HIPCHECK(hipStreamEndCapture(s, &graph));
+    const hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ie != hipSuccess) {
+        ws.graphs_off = true;
+        throw Error(OFX_EDEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(ie));
+    }
+    auto &slot = ws.graphs[ws.graph_next];
+    ws.graph_next ^= 1;
+    if (slot.exec) (void)hipGraphExecDestroy(slot.exec);
+    slot.exec = exec;
+    slot.key = key;
+    ws.graph_captures++;
+    HIPCHECK(hipGraphLaunch(exec, s));
+    ws.graph_launches++;
+}
+
 // ---------------------------------------------------------------- tiled fine stage
 // rust/crates/optflow-core/src/tile.rs:17-146 (geometry) and :225-500 (the tiled driver and
 // join): cores of c.tile px partition the frame; each is refined on its extension (core plus
@@ -692,7 +803,7 @@ constexpr int TILE_GROUPS = 3;
 void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, const ofx_config &c, hipStream_t s,
                 bool cache_i1) {
     if (c.tile <= 0 || std::max(w, h) <= c.tile) {
-        refine_device(ws, i1, i2, w, h, w, h, 0, 0, c, s, cache_i1);
+        refine_graph(ws, i1, i2, w, h, w, h, 0, 0, c, s, cache_i1);
         return;
     }
     const size_t n = (size_t)w * h;
@@ -1626,7 +1737,7 @@ int ofx_refine_raw(ofx_engine *e, const float *i1, int w, int h, const float *i2
         copy_h2d(ws.pin, dy, uy, n * sizeof(float), s);
         to_f64(dx, B<double>(ws.bux, n), n, s);
         to_f64(dy, B<double>(ws.buy, n), n, s);
-        refine_device(ws, d1, d2, w, h, w2, h2, (long)shift_x, (long)shift_y, *cfg, s, false);
+        refine_graph(ws, d1, d2, w, h, w2, h2, (long)shift_x, (long)shift_y, *cfg, s, false);
         if (out_ux) copy_d2h(ws.pin, out_ux, ws.bux.p, n * sizeof(double), s);
         if (out_uy) copy_d2h(ws.pin, out_uy, ws.buy.p, n * sizeof(double), s);
         if (out_energy) copy_d2h(ws.pin, out_energy, ws.benergy.p, n * sizeof(double), s);
@@ -1780,6 +1891,18 @@ int ofx_frame_to_gray(ofx_engine *e, const void *pixels, int bits, int channels,
         frame_to_gray(dp, bits, channels, w, h, factor, dout, s);
         copy_d2h(ws.pin, out, dout, (size_t)ow * oh * sizeof(float), s);
         wait_stream(s);
+    });
+}
+
+int ofx_graph_stats(ofx_engine *e, int64_t *captures, int64_t *launches) {
+    return guarded([&] {
+        if (!e || !captures || !launches) throw Error(OFX_EINVAL, "engine or output pointer is NULL");
+        *captures = *launches = 0;
+        for (auto &l : e->lanes)
+            if (l->ws) {
+                *captures += l->ws->graph_captures;
+                *launches += l->ws->graph_launches;
+            }
     });
 }
 
