@@ -219,51 +219,6 @@ def main():
     print(json.dumps(line))
 
 
-def render_c3_gpu(pat, n, nf):
-    """The frames of the long sequence rendered with torch on the GPU in f64, the same steps as
-    synth.render: the inverse of x = X + u(X) by 30 fixed-point steps, a bilinear sample of the
-    pattern (edges clamped as synth._bilinear), then the brightness drift, Gaussian noise of sigma
-    0.01 (torch's generator, seeded per frame) and uint8 quantisation. Returns host f32 frames."""
-    import torch
-    dev = torch.device("cuda:0")
-    P = torch.from_numpy(pat).to(dev)
-    H, W = P.shape
-    m = (H - n) // 2
-    yy, xx = torch.meshgrid(torch.arange(n, dtype=torch.float64, device=dev),
-                            torch.arange(n, dtype=torch.float64, device=dev), indexing="ij")
-    cx = cy = (n - 1) / 2.0
-    a = np.deg2rad(35.0)
-    tx, ty, nx, ny = np.cos(a), np.sin(a), -np.sin(a), np.cos(a)
-
-    def field(X, Y, s):
-        r2 = (X - cx) ** 2 + (Y - cy) ** 2
-        uy = -6.0 * torch.exp(-r2 / (2 * 300.0 ** 2))
-        g = 0.5 * 3.0 * torch.tanh((nx * (X - cx) + ny * (Y - cy)) / 20.0)
-        return s * g * tx, s * (uy + g * ty)
-    out = []
-    for k in range(nf):
-        s = k / (nf - 1)
-        X, Y = xx, yy
-        if k:
-            for _ in range(30):
-                ux, uy = field(X, Y, s)
-                X, Y = xx - ux, yy - uy
-        xf = torch.clamp(X + m, 0, W - 1.000001)
-        yf = torch.clamp(Y + m, 0, H - 1.000001)
-        x0, y0 = torch.floor(xf).long(), torch.floor(yf).long()
-        fx, fy = xf - x0, yf - y0
-        v = ((1 - fy) * ((1 - fx) * P[y0, x0] + fx * P[y0, x0 + 1])
-             + fy * ((1 - fx) * P[y0 + 1, x0] + fx * P[y0 + 1, x0 + 1]))
-        gen = torch.Generator(device=dev)
-        gen.manual_seed(100 + k)
-        v = v * (0.97 + 0.06 * s) + 0.01 * torch.randn(v.shape, generator=gen, device=dev, dtype=torch.float64)
-        q = torch.clamp(torch.round(v * 255.0), 0, 255)
-        out.append((q.float() / 255.0).cpu().numpy())
-        if (k + 1) % 20 == 0:
-            print(f"# rendered frame {k + 1}/{nf}", file=sys.stderr, flush=True)
-    return out
-
-
 def bench_c3(args):
     """The long-sequence workload at 2048^2 (--frames frames, default 12; 200 is the full workload,
     rendered on the GPU above 24 frames): the field grows linearly to the bowl-and-band field at
@@ -293,7 +248,10 @@ def bench_c3(args):
             frames.append(synth.as_float(np.clip(np.rint(v * 255.0), 0, 255).astype(np.uint8)))
             print(f"# rendered frame {k + 1}/{nf}", file=sys.stderr, flush=True)
     else:
-        frames = render_c3_gpu(pat, n, nf)
+        for k, f in enumerate(synth.growing_bowl_and_band_frames_gpu(n, nf)):
+            frames.append(f)
+            if (k + 1) % 20 == 0:
+                print(f"# rendered frame {k + 1}/{nf}", file=sys.stderr, flush=True)
     print(f"# rendered {nf} frames in {time.perf_counter() - t:.1f} s", file=sys.stderr, flush=True)
     cfg = Config(seq_mode="hybrid", prewarp_freeze_at=3, ref_update_conf=0.3)
     eng = ofx.Engine(0)

@@ -180,30 +180,91 @@ def affine_field(exx, eyy, exy, rot_deg, t, center):
     def f(X, Y):
         dx, dy = X - center[0], Y - center[1]
         return a11 * dx + a12 * dy + t[0], a21 * dx + a22 * dy + t[1]
+    f.coeffs = (a11, a12, a21, a22, (float(center[0]), float(center[1])), (float(t[0]), float(t[1])))
     return f
 
 
-def workload_c4(n_pairs=64, distinct_per_size=2, seed=0):
-    """The batched uneven workload: n_pairs pair indices with sizes drawn uniformly from C4_SIZES;
-    each drawn pair is one of `distinct_per_size` rendered pairs of its size (a random affine field:
-    strains within +-2 %, rotation +-1 deg, translation +-8 px; noise sigma 0.005) with a convex
-    8-gon ROI of its own covering 40-90 %. Returns the list of (I1 u8, I2 u8, roi)."""
+def render_affine(pattern, ox, oy, h, w, f, noise=0.0, seed=0):
+    """An h x w frame of an affine field f (affine_field) over the pattern's window at (ox, oy):
+    the same steps as render, with the inverse map X = c + M^-1 (x - c - t) in closed form
+    (M = I + A) instead of the fixed point."""
+    a11, a12, a21, a22, (cx, cy), (tx, ty) = f.coeffs
+    det = (1 + a11) * (1 + a22) - a12 * a21
+    yy, xx = np.mgrid[0:h, 0:w].astype(np.float64)
+    rx, ry = xx - cx - tx, yy - cy - ty
+    X = cx + ((1 + a22) * rx - a12 * ry) / det
+    Y = cy + (-a21 * rx + (1 + a11) * ry) / det
+    v = _bilinear(pattern, X + ox, Y + oy, 0)
+    if noise > 0:
+        v = v + np.random.default_rng(seed).normal(0.0, noise, v.shape)
+    return np.clip(np.rint(v * 255.0), 0, 255).astype(np.uint8)
+
+
+def workload_c4(n_pairs=64, seed=0):
+    """The batched uneven workload: n_pairs distinct pairs, sizes drawn uniformly from C4_SIZES.
+    Each pair is its own window of one 4096 x 4096 speckle pattern (a seeded place), under its own
+    random affine field (strains within +-2 %, rotation +-1 deg, translation +-8 px), with its own
+    noise (sigma 0.005) and a convex 8-gon ROI of its own covering 40-90 %. Returns the list of
+    (I1 u8, I2 u8, roi)."""
     rng = np.random.default_rng(seed)
-    rendered = {}
-    for si, (h, w) in enumerate(C4_SIZES):
-        for d in range(distinct_per_size):
-            s = 1000 * si + d
-            pat = make_pattern(h, w, seed=s)
-            f = affine_field(*rng.uniform(-0.02, 0.02, 3), rng.uniform(-1.0, 1.0), rng.uniform(-8.0, 8.0, 2),
-                             ((w - 1) / 2.0, (h - 1) / 2.0))
-            rendered[(si, d)] = (render(pat, None, noise=0.005, seed=s + 1), render(pat, f, noise=0.005, seed=s + 2))
+    pat = make_pattern(4096, 4096, seed=seed + 4000, margin=0)
     out = []
     for k in range(n_pairs):
-        si = int(rng.integers(len(C4_SIZES)))
-        a, b = rendered[(si, k % distinct_per_size)]
-        h, w = C4_SIZES[si]
-        out.append((a, b, convex_polygon_mask(h, w, seed=k, coverage=float(rng.uniform(0.4, 0.9)), search_step=8)))
+        h, w = C4_SIZES[int(rng.integers(len(C4_SIZES)))]
+        ox = int(rng.integers(64, 4096 - w - 64))
+        oy = int(rng.integers(64, 4096 - h - 64))
+        f = affine_field(*rng.uniform(-0.02, 0.02, 3), rng.uniform(-1.0, 1.0), rng.uniform(-8.0, 8.0, 2),
+                         ((w - 1) / 2.0, (h - 1) / 2.0))
+        a = np.clip(np.rint((pat[oy:oy + h, ox:ox + w]
+                             + np.random.default_rng(10 * k + 1).normal(0.0, 0.005, (h, w))) * 255.0), 0, 255)
+        b = render_affine(pat, ox, oy, h, w, f, noise=0.005, seed=10 * k + 2)
+        roi = convex_polygon_mask(h, w, seed=k, coverage=float(rng.uniform(0.4, 0.9)), search_step=8)
+        out.append((a.astype(np.uint8), b, roi))
     return out
+
+
+def growing_bowl_and_band_frames_gpu(n, nf, seed=0):
+    """The long-sequence workload's frames, rendered with torch on the GPU in f64 (the steps of
+    render): frame k carries s = k / (nf - 1) times the bowl-and-band field of an n x n frame (the
+    inverse of x = X + u(X) by 30 fixed-point steps, a bilinear sample of the pattern, edges clamped
+    as _bilinear), then a brightness drift of -3 % .. +3 %, Gaussian noise of sigma 0.01 (torch's
+    generator, seeded per frame) and 8-bit quantisation. Yields host f32 frames one at a time, and
+    the truth of the last frame is bowl_and_band scaled by 1."""
+    import torch
+    dev = torch.device("cuda:0")
+    pat = make_pattern(n, n, seed=seed)
+    P = torch.from_numpy(pat).to(dev)
+    H, W = P.shape
+    m = (H - n) // 2
+    yy, xx = torch.meshgrid(torch.arange(n, dtype=torch.float64, device=dev),
+                            torch.arange(n, dtype=torch.float64, device=dev), indexing="ij")
+    c = (n - 1) / 2.0
+    a = np.deg2rad(35.0)
+    tx, ty, nx, ny = np.cos(a), np.sin(a), -np.sin(a), np.cos(a)
+
+    def field(X, Y, s):
+        r2 = (X - c) ** 2 + (Y - c) ** 2
+        uy = -6.0 * torch.exp(-r2 / (2 * 300.0 ** 2))
+        g = 0.5 * 3.0 * torch.tanh((nx * (X - c) + ny * (Y - c)) / 20.0)
+        return s * g * tx, s * (uy + g * ty)
+    for k in range(nf):
+        s = k / (nf - 1)
+        X, Y = xx, yy
+        if k:
+            for _ in range(30):
+                ux, uy = field(X, Y, s)
+                X, Y = xx - ux, yy - uy
+        xf = torch.clamp(X + m, 0, W - 1.000001)
+        yf = torch.clamp(Y + m, 0, H - 1.000001)
+        x0, y0 = torch.floor(xf).long(), torch.floor(yf).long()
+        fx, fy = xf - x0, yf - y0
+        v = ((1 - fy) * ((1 - fx) * P[y0, x0] + fx * P[y0, x0 + 1])
+             + fy * ((1 - fx) * P[y0 + 1, x0] + fx * P[y0 + 1, x0 + 1]))
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(100 + k)
+        v = v * (0.97 + 0.06 * s) + 0.01 * torch.randn(v.shape, generator=gen, device=dev, dtype=torch.float64)
+        q = torch.clamp(torch.round(v * 255.0), 0, 255)
+        yield (q.float() / 255.0).cpu().numpy()
 
 
 def gaussian_bumps(h, w, n=6, amp=40.0, scale=(150.0, 600.0), seed=0):

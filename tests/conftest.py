@@ -11,6 +11,17 @@ if ROOT not in sys.path:
 
 GOLDEN = os.path.join(ROOT, "tests", "golden", "reference")
 
+# On a GPU machine, torch's HIP runtime starts before libofx.so is loaded, so that the tests that
+# render or hold device arrays with torch and the library share one runtime (loaded the other way
+# round, torch finds no device). Without a GPU this does nothing.
+try:
+    import torch
+
+    if torch.cuda.is_available():
+        torch.cuda.init()
+except ImportError:
+    pass
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device) to run")
