@@ -67,7 +67,9 @@ def main():
                     help="c2: one 2048^2 pair (default); c4: 64 uneven pairs through the batched entry point; "
                          "c3: a 2048^2 hybrid sequence with temporal averaging (--frames frames); "
                          "c5: one 4000 x 3000 pair with displacements up to 40 px (plain POC-initialised method)")
-    ap.add_argument("--lanes", type=int, default=4, help="engine lanes for --workload c4")
+    ap.add_argument("--lanes", default="auto",
+                    help="engine lanes for --workload c4: a number, or auto (the autotuner picks lanes and the "
+                         "graph switch on a sample of the batch)")
     ap.add_argument("--tile", type=int, default=0,
                     help="fine-stage tile size for c2/c5 (0: whole frame; 1024 is the reference CLI's tiling)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -305,7 +307,7 @@ def bench_c3(args):
 def bench_c4(args):
     """64 independent pairs of uneven sizes (512^2 to 1024 x 2048, mean 1.31 Mpix) with their own
     affine fields and ROIs, through ofx_track_pairs on --lanes lanes (one host thread and stream
-    each). One step = the whole batch, host arrays in and out (the copies overlap other lanes'
+    each; by default the autotuner's choice of lanes and graph switch). One step = the whole batch, host arrays in and out (the copies overlap other lanes'
     work). Also times the same batch one pair at a time on one lane."""
     import optflow_soil_amd as ofx
     from optflow_soil_amd import synth
@@ -318,7 +320,18 @@ def bench_c4(args):
     print(f"# rendered {len(pairs)} pairs, {mpix:.1f} Mpix, in {time.perf_counter() - t:.1f} s", file=sys.stderr,
           flush=True)
     cfg = Config()
-    eng = ofx.Engine(0, lanes=args.lanes)
+    tuned = None
+    if args.lanes == "auto":
+        from optflow_soil_amd import autotune
+        t = time.perf_counter()
+        best, table = autotune.tune(pairs, cfg, log=lambda m: print("# " + m, file=sys.stderr, flush=True))
+        tuned = {"chosen": best, "candidates": table, "seconds": round(time.perf_counter() - t, 1),
+                 "sample": "the two largest pairs of each size class"}
+        lanes, graphs = best["lanes"], best["graphs"]
+    else:
+        lanes, graphs = int(args.lanes), True
+    eng = ofx.Engine(0, lanes=lanes)
+    eng.set_schedule(graphs=graphs)
     for i in range(args.warmup):
         eng.track_pairs(pairs, cfg)
         print(f"# warmup {i + 1}/{args.warmup} done", file=sys.stderr, flush=True)
@@ -341,9 +354,9 @@ def bench_c4(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": "64 uneven pairs (512^2, 768x1024, 1024^2, 1536^2, 1024x2048), affine fields, "
                                "own ROIs 40-90 %, default settings, host arrays in and out",
-                   "lanes": args.lanes, "mpix_per_step": round(mpix, 2)},
+                   "lanes": lanes, "graphs": graphs, "mpix_per_step": round(mpix, 2)},
         "sequential_value": round(mpix / dt_seq, 3), "batched_over_sequential": round(dt_seq / dt, 3),
-        "valid_frac_roi": round(valid, 5),
+        "valid_frac_roi": round(valid, 5), "autotune": tuned,
     }
     print(json.dumps(line))
 

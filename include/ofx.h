@@ -239,6 +239,18 @@ int ofx_seq_step(ofx_sequence *seq, const float *frame, ofx_result *out);
 int ofx_seq_step_device(ofx_sequence *seq, const float *d_frame, ofx_result *d_out);
 void ofx_seq_end(ofx_sequence *seq);
 
+/* No counterpart in the reference: the engine's schedule, the knobs its autotuner
+ * (optflow_soil_amd/autotune.py) chooses among. tile_groups: tiles refined side by side in a tiled
+ * fine stage (1 .. 8, default 3; each group beyond the first has its own stream and workspace);
+ * graphs: 1 (default) replays the whole-frame fine stage from HIP graphs, 0 runs it eagerly. Takes
+ * effect for calls that start after it; results do not depend on it. */
+typedef struct ofx_schedule {
+    int tile_groups;
+    int graphs;
+} ofx_schedule;
+int ofx_set_schedule(ofx_engine *e, const ofx_schedule *sc);
+int ofx_get_schedule(const ofx_engine *e, ofx_schedule *sc);
+
 /* No counterpart in the reference. The whole-frame fine stage runs from a HIP graph once it has
  * run eagerly with the same sizes, settings, inputs and buffers (captured on the second such call,
  * replayed from the third; results bit-identical; OFX_GRAPHS=0 in the environment turns this off,

@@ -26,7 +26,8 @@ SYMBOLS = ("ofx_config_default", "ofx_create", "ofx_destroy", "ofx_last_error", 
            "ofx_track_sequence", "ofx_profile", "ofx_profile_read", "ofx_profile_stage", "ofx_crop_plan",
            "ofx_write_crop_json", "ofx_write_result", "ofx_pyramid_levels", "ofx_poc_level_window",
            "ofx_format_json_number", "ofx_coarse_net_params", "ofx_coarse_net_load", "ofx_coarse_net_propose",
-           "ofx_coarse_net_gradient", "ofx_coarse_net_train", "ofx_graph_stats")
+           "ofx_coarse_net_gradient", "ofx_coarse_net_train", "ofx_graph_stats", "ofx_set_schedule",
+           "ofx_get_schedule")
 
 
 class OfxConfig(ctypes.Structure):
@@ -122,6 +123,10 @@ class Result:
     fb: np.ndarray = None  # hybrid sequences: consecutive frames carried by the adjacent leg
 
 
+class OfxSchedule(ctypes.Structure):
+    _fields_ = [("tile_groups", ctypes.c_int), ("graphs", ctypes.c_int)]
+
+
 class OfxNetTrain(ctypes.Structure):
     _fields_ = [("steps", ctypes.c_int), ("batch", ctypes.c_int), ("crop", ctypes.c_int), ("n_pairs", ctypes.c_int),
                 ("lr", ctypes.c_double), ("lr_final", ctypes.c_double), ("max_disp", ctypes.c_double),
@@ -207,6 +212,19 @@ class Engine:
         calls = (ctypes.c_int64 * len(names))()
         _check(lib().ofx_profile_read(self._h, ms, calls, len(names)))
         return {n: (ms[i], calls[i]) for i, n in enumerate(names)}
+
+    def schedule(self):
+        """The engine's schedule: {"tile_groups": n, "graphs": bool}."""
+        sc = OfxSchedule()
+        _check(lib().ofx_get_schedule(self._h, ctypes.byref(sc)))
+        return {"tile_groups": sc.tile_groups, "graphs": bool(sc.graphs)}
+
+    def set_schedule(self, tile_groups=None, graphs=None):
+        """Changes the schedule knobs given (ofx_set_schedule); results do not depend on them."""
+        cur = self.schedule()
+        sc = OfxSchedule(int(cur["tile_groups"] if tile_groups is None else tile_groups),
+                         int(cur["graphs"] if graphs is None else graphs))
+        _check(lib().ofx_set_schedule(self._h, ctypes.byref(sc)))
 
     def graph_stats(self):
         """(captures, replays) of the fine stage's HIP graphs on this engine so far."""

@@ -307,6 +307,9 @@ struct Workspace {
     } graphs[2];
     int graph_next = 0;
     bool graphs_off = false;  // a capture failed: this workspace runs eagerly from then on
+    // the engine's schedule (ofx_set_schedule), copied in when a call takes the lane
+    int tile_groups = 3;
+    bool graphs_on = true;
     long long graph_launches = 0, graph_captures = 0;
     // the engine's learned coarse network (read only during a call) and this lane's activations
     const CoarseNet *net = nullptr;
@@ -345,6 +348,8 @@ struct ofx_engine {
     int device = 0;
     std::vector<std::unique_ptr<ofx::Lane>> lanes;
     std::unique_ptr<ofx::CoarseNet> net;  // the learned coarse network (ofx_coarse_net_load)
+    std::atomic<int> tile_groups{3};      // ofx_set_schedule
+    std::atomic<bool> graphs{true};
 };
 
 namespace ofx {
@@ -366,6 +371,8 @@ struct LaneGuard {
         HIPCHECK(hipSetDevice(e->device));
         if (!lane->ws) lane->ws.reset(new Workspace);
         lane->ws->net = e->net.get();
+        lane->ws->tile_groups = e->tile_groups.load();
+        lane->ws->graphs_on = e->graphs.load();
     }
     ~LaneGuard() {
         if (lane) lane->busy.store(false);
@@ -671,7 +678,8 @@ std::vector<uint64_t> fine_key(const Workspace &ws, const float *i1, const float
 // refine_device, replayed from a graph where it can be (see above); results bit-identical.
 void refine_graph(Workspace &ws, const float *i1, const float *i2, int w, int h, int w2, int h2, long shx, long shy,
                   const ofx_config &c, hipStream_t s, bool cache_i1) {
-    const bool can = graphs_enabled() && !ws.graphs_off && !ws.prof.on && (c.outer_iters * c.inner_iters) % 2 == 0;
+    const bool can = graphs_enabled() && ws.graphs_on && !ws.graphs_off && !ws.prof.on &&
+                     (c.outer_iters * c.inner_iters) % 2 == 0;
     if (!can) {
         refine_device(ws, i1, i2, w, h, w2, h2, shx, shy, c, s, cache_i1);
         return;
@@ -793,9 +801,10 @@ __global__ void k_write_core(const double *__restrict__ tux, const double *__res
     jr[i] = tres[l];
 }
 
-// Tile groups refined side by side in the tiled fine stage: the lane's stream and two more, which
-// with the stream of the caller keeps to the four hardware queues a process gets by default.
-constexpr int TILE_GROUPS = 3;
+// Tile groups refined side by side in the tiled fine stage: by default the lane's stream and two
+// more, which with the stream of the caller keeps to the four hardware queues a process gets by
+// default (ofx_set_schedule changes it, 1 .. MAX_TILE_GROUPS).
+constexpr int MAX_TILE_GROUPS = 8;
 
 // The fine stage of a pair whose dense start is in ws.bux / ws.buy (w x h): the whole frame, or
 // tiled when the frame has a side over c.tile (track.rs:580-598). Leaves the field in ws.bux /
@@ -876,12 +885,12 @@ void fine_stage(Workspace &ws, const float *i1, const float *i2, int w, int h, c
                                            ", exceeds the " + std::to_string(MAX_I2_SIDE) +
                                            " px side the spline rows hold (the start field spans too far)");
     }
-    // The tiles are independent refines (tile.rs:225-330): they run in TILE_GROUPS groups, the
+    // The tiles are independent refines (tile.rs:225-330): they run in ws.tile_groups groups, the
     // group g taking the tiles order[g], order[g + G], ... (largest first), group 0 on the lane's
     // stream and workspace, the others on streams and workspaces of their own that fork from the
     // lane's stream here and join it before the joined planes are read. Each group's workspace
     // is used by one stream only; the cores the tiles write into jx .. jr are disjoint.
-    const int G = std::min(TILE_GROUPS, nt);
+    const int G = std::max(1, std::min(ws.tile_groups, nt));
     while ((int)ws.sub.size() < G - 1) {
         ws.sub.emplace_back(new Workspace);
         hipStream_t t;
@@ -1891,6 +1900,24 @@ int ofx_frame_to_gray(ofx_engine *e, const void *pixels, int bits, int channels,
         frame_to_gray(dp, bits, channels, w, h, factor, dout, s);
         copy_d2h(ws.pin, out, dout, (size_t)ow * oh * sizeof(float), s);
         wait_stream(s);
+    });
+}
+
+int ofx_set_schedule(ofx_engine *e, const ofx_schedule *sc) {
+    return guarded([&] {
+        if (!e || !sc) throw Error(OFX_EINVAL, "engine or schedule is NULL");
+        if (sc->tile_groups < 1 || sc->tile_groups > MAX_TILE_GROUPS)
+            throw Error(OFX_EINVAL, "tile_groups must lie in [1, " + std::to_string(MAX_TILE_GROUPS) + "]");
+        e->tile_groups.store(sc->tile_groups);
+        e->graphs.store(sc->graphs != 0);
+    });
+}
+
+int ofx_get_schedule(const ofx_engine *e, ofx_schedule *sc) {
+    return guarded([&] {
+        if (!e || !sc) throw Error(OFX_EINVAL, "engine or schedule is NULL");
+        sc->tile_groups = e->tile_groups.load();
+        sc->graphs = e->graphs.load() ? 1 : 0;
     });
 }
 

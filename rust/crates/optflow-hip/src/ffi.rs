@@ -91,6 +91,15 @@ pub struct OfxCrop {
     pub margin: i64,
 }
 
+/// `ofx_schedule`: tiles refined side by side and the graph switch (no counterpart in the
+/// reference; results do not depend on it).
+#[repr(C)]
+#[derive(Clone, Copy, Default)]
+pub struct OfxSchedule {
+    pub tile_groups: c_int,
+    pub graphs: c_int,
+}
+
 #[repr(C)]
 pub struct OfxEngine {
     _private: [u8; 0],
@@ -172,4 +181,7 @@ extern "C" {
         raw: c_int,
         seq: c_int,
     ) -> c_int;
+    pub fn ofx_set_schedule(e: *mut OfxEngine, sc: *const OfxSchedule) -> c_int;
+    pub fn ofx_get_schedule(e: *const OfxEngine, sc: *mut OfxSchedule) -> c_int;
+    pub fn ofx_graph_stats(e: *mut OfxEngine, captures: *mut i64, launches: *mut i64) -> c_int;
 }

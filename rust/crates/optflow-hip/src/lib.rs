@@ -100,6 +100,12 @@ impl HipEngine {
     pub fn raw(&self) -> *mut ffi::OfxEngine {
         self.0
     }
+
+    /// Tiles refined side by side (1 .. 8) and the graph switch; results do not depend on them.
+    pub fn set_schedule(&self, tile_groups: i32, graphs: bool) -> Result<(), EngineError> {
+        let sc = ffi::OfxSchedule { tile_groups, graphs: graphs as i32 };
+        check(unsafe { ffi::ofx_set_schedule(self.0, &sc) })
+    }
 }
 
 impl Drop for HipEngine {

@@ -85,3 +85,35 @@ def test_hybrid_sequence_with_graphs_is_bit_identical(eng, monkeypatch):
     cap, run = eng.graph_stats()
     print(f"sequence: captures {cap - cap0}, replays {run - run0}", flush=True)
     assert run - run0 >= 4
+
+
+def test_schedules_do_not_change_results_and_the_autotuner_picks_one(eng):
+    """Tile groups (1 .. 4) and the graph switch change how a tiled pair runs, not its result; the
+    autotuner times the candidates on a sample and returns the fastest."""
+    import optflow_soil_amd as ofx
+    from optflow_soil_amd import autotune
+    a, b = pair(288, 416, 51)
+    cfg = Config(tile=128)
+    ref = None
+    for tg in (1, 2, 4):
+        for gr in (True, False):
+            eng.set_schedule(tile_groups=tg, graphs=gr)
+            r = eng.track_pair(a, b, cfg)
+            if ref is None:
+                ref = r
+            else:
+                same(r, ref, f"tile groups {tg}, graphs {gr}")
+    eng.set_schedule(tile_groups=3, graphs=True)
+    assert eng.schedule() == {"tile_groups": 3, "graphs": True}
+    pairs = [pair(192, 256, 53) + (None,), pair(160, 224, 55) + (None,), pair(192, 256, 57) + (None,)]
+    best, table = autotune.tune(pairs, Config(), lanes=(1, 2), repeats=1)
+    print(f"autotune table {table}", flush=True)
+    assert len(table) == 4 and best in table and best["mpix_per_s"] == max(r["mpix_per_s"] for r in table)
+    one = ofx.Engine(0, lanes=best["lanes"])
+    try:
+        one.set_schedule(graphs=best["graphs"])
+        got = one.track_pairs(pairs, Config())
+    finally:
+        one.close()
+    for (x, y, _), r in zip(pairs, got):
+        same(r, eng.track_pair(x, y, Config()), "tuned batch")

@@ -45,7 +45,8 @@ def rust_struct(text, name):
 
 def test_ffi_structs_match_the_header():
     h, r = open(HEADER).read(), open(FFI).read()
-    for cname, rname in (("ofx_config", "OfxConfig"), ("ofx_result", "OfxResult"), ("ofx_crop", "OfxCrop")):
+    for cname, rname in (("ofx_config", "OfxConfig"), ("ofx_result", "OfxResult"), ("ofx_crop", "OfxCrop"),
+                         ("ofx_schedule", "OfxSchedule")):
         cf, rf = c_struct(h, cname), rust_struct(r, rname)
         assert [n for n, _ in cf] == [n for n, _ in rf], cname
         for (n, ct), (_, rt) in zip(cf, rf):
