@@ -524,7 +524,10 @@ __device__ __forceinline__ void solve_pixel(const double (&mp)[3][3][3], const d
         M[a][a] += RIDGE * tr / 6.0;
         b0[a] = bv[a];
     }
-    // Cholesky (lower triangle in place), then L y = b, L^T x = y
+    // Cholesky (lower triangle in place), then L y = b, L^T x = y; one division per column (the
+    // reciprocal of its diagonal) instead of one per element. A diagonal that is not positive gives a
+    // NaN reciprocal, so the result is not finite and the pixel falls back, as with the divisions.
+    double il[6];
 #pragma unroll
     for (int j = 0; j < 6; j++) {
         double d = M[j][j];
@@ -532,13 +535,14 @@ __device__ __forceinline__ void solve_pixel(const double (&mp)[3][3][3], const d
         for (int k = 0; k < j; k++) d -= M[j][k] * M[j][k];
         ok = ok && isfinite(d) && d > 0;
         const double l = sqrt(d);
+        il[j] = 1.0 / l;
         M[j][j] = l;
 #pragma unroll
         for (int a = j + 1; a < 6; a++) {
             double s = M[a][j];
 #pragma unroll
             for (int k = 0; k < j; k++) s -= M[a][k] * M[j][k];
-            M[a][j] = s / l;
+            M[a][j] = s * il[j];
         }
     }
 #pragma unroll
@@ -546,14 +550,14 @@ __device__ __forceinline__ void solve_pixel(const double (&mp)[3][3][3], const d
         double s = bv[a];
 #pragma unroll
         for (int k = 0; k < a; k++) s -= M[a][k] * bv[k];
-        bv[a] = s / M[a][a];
+        bv[a] = s * il[a];
     }
 #pragma unroll
     for (int a = 5; a >= 0; a--) {
         double s = bv[a];
 #pragma unroll
         for (int k = a + 1; k < 6; k++) s -= M[k][a] * bv[k];
-        bv[a] = s / M[a][a];
+        bv[a] = s * il[a];
     }
 #pragma unroll
     for (int a = 0; a < 6; a++) ok = ok && isfinite(bv[a]);
@@ -773,6 +777,169 @@ __global__ void __launch_bounds__(256) k_moments_y_rb(const double *__restrict__
         case 5: y_block<R, 1>(src, out + 13 * n, n, w, h, x, y0, kt); break;
         default: y_block<R, 1>(src, out + 14 * n, n, w, h, x, y0, kt); break;
     }
+}
+
+// x pass for a compile-time radius R, register-blocked along x, then the solve. A workgroup takes a
+// 256-pixel row segment:
+//  1. the 15 y-pass planes of the segment and its 2R-column halo are staged in LDS;
+//  2. each wavefront takes a fixed set of planes, balanced by the x powers they feed (7, 7, 6 and 6
+//     moments), and each lane 4 consecutive pixels: the lane reads its 4 + 2R inputs of a plane once
+//     and adds each into the 4 outputs it reaches, in the row-factor form of y_block (the x powers
+//     recombined per output from T_p = sum_t k_t a^p v);
+//  3. the 26 moments of every pixel go to LDS, over the staging area;
+//  4. each thread solves one pixel.
+// A lane's inputs sit at columns 4 l + j. Column c of a plane is stored at (c mod 4) XB_YQ + c / 4,
+// so the 64 lanes read 64 consecutive words; moment m of pixel p is stored at
+// m XB_P + (p mod 4) 64 + p / 4, so the lanes write consecutive words in step 3 and, solving pixel
+// 4 l + wave in step 4, read consecutive words.
+constexpr int XB_P = 256, XB_B = 4, XB_C = 2;
+
+template <int R>
+struct XbLayout {
+    static constexpr int YQ = (XB_P + 2 * R + 3) / 4;
+    static constexpr int YS = 4 * YQ;
+    static constexpr int WORDS = (15 * YS > 26 * XB_P) ? 15 * YS : 26 * XB_P;
+};
+
+template <int R, int NP, int ROW>
+__device__ __forceinline__ void x_plane(const double *__restrict__ ys, int lane, const KT &kt,
+                                        double (&res)[7][XB_B]) {
+    constexpr int YQ = XbLayout<R>::YQ;
+    double acc[XB_B][NP];
+#pragma unroll
+    for (int q = 0; q < XB_B; q++)
+#pragma unroll
+        for (int p = 0; p < NP; p++) acc[q][p] = 0;
+#pragma unroll
+    for (int j = 0; j < XB_B + 2 * R; j++) {
+        const double v = ys[(j & 3) * YQ + lane + (j >> 2)];
+        const double a = (double)(j - R - XB_C);
+        double vp[NP];
+        vp[0] = v;
+        if constexpr (NP > 1) vp[1] = a * v;
+        if constexpr (NP > 2) vp[2] = (a * a) * v;
+#pragma unroll
+        for (int q = 0; q < XB_B; q++) {
+            const int t = j - q;  // tap 0 .. 2R of output q
+            if (t >= 0 && t <= 2 * R) {
+                const double k = kt.k[0][t <= R ? t : 2 * R - t];
+#pragma unroll
+                for (int p = 0; p < NP; p++) acc[q][p] = fma(k, vp[p], acc[q][p]);
+            }
+        }
+        // Without this, the compiler orders the multiply-adds chain by chain and keeps every
+        // input and its row factors live at once (over 200 VGPRs); touching the accumulators
+        // every two inputs keeps the input-major order.
+        if (j & 1) {
+#pragma unroll
+            for (int q = 0; q < XB_B; q++)
+#pragma unroll
+                for (int p = 0; p < NP; p++) asm volatile("" : "+v"(acc[q][p]));
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < XB_B; q++) {
+        const double c = (double)(q - XB_C);
+        res[ROW][q] = acc[q][0];
+        if constexpr (NP > 1) res[ROW + 1][q] = fma(-c, acc[q][0], acc[q][1]);
+        if constexpr (NP > 2) res[ROW + 2][q] = fma(c * c, acc[q][0], fma(-2 * c, acc[q][1], acc[q][2]));
+    }
+}
+
+template <int NP, int ROW>
+__device__ __forceinline__ void x_put(double *__restrict__ mom, int base, int lane, const double (&res)[7][XB_B]) {
+#pragma unroll
+    for (int p = 0; p < NP; p++)
+#pragma unroll
+        for (int q = 0; q < XB_B; q++) mom[(base + p) * XB_P + q * 64 + lane] = res[ROW + p][q];
+}
+
+template <int R>
+__global__ void __launch_bounds__(256) k_moments_x_rb(const double *__restrict__ ym, const double *__restrict__ ux_in,
+                                                      const double *__restrict__ uy_in, double *__restrict__ ux_out,
+                                                      double *__restrict__ uy_out, double *__restrict__ energy,
+                                                      double *__restrict__ resid, double *__restrict__ slope, int w,
+                                                      int h, const KT kt) {
+    using L = XbLayout<R>;
+    __shared__ double lds[L::WORDS];
+    const int x0 = blockIdx.x * XB_P, y = blockIdx.y;
+    const size_t n = (size_t)w * h;
+    // stage: thread t takes column t of every plane and, for t < 2R, column 256 + t; the plane
+    // offset is uniform, so the addresses cost no per-plane vector arithmetic. Every load is issued
+    // before the first is waited for.
+    {
+        static_assert(2 * R <= XB_P, "the halo columns are taken by the first 2R threads");
+        const int c0 = threadIdx.x, c1 = threadIdx.x + XB_P;
+        const bool two = c1 < XB_P + 2 * R;
+        const double *row = ym + (size_t)y * w;
+        const double *p0 = row + min(max(x0 - R + c0, 0), w - 1);
+        const double *p1 = row + min(max(x0 - R + (two ? c1 : c0), 0), w - 1);
+        double v0[15], v1[15];
+#pragma unroll
+        for (int q = 0; q < 15; q++) v0[q] = p0[(size_t)q * n];
+        if (two) {
+#pragma unroll
+            for (int q = 0; q < 15; q++) v1[q] = p1[(size_t)q * n];
+        }
+        double *d0 = lds + (c0 & 3) * L::YQ + (c0 >> 2), *d1 = lds + (c1 & 3) * L::YQ + (c1 >> 2);
+#pragma unroll
+        for (int q = 0; q < 15; q++) d0[q * L::YS] = v0[q];
+        if (two) {
+#pragma unroll
+            for (int q = 0; q < 15; q++) d1[q * L::YS] = v1[q];
+        }
+    }
+    __syncthreads();
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    // y-pass planes: gxx t^0,1,2 | gxy t^0,1,2 | gyy t^0,1,2 | gxc t^0,1 | gyc t^0,1 | gcc | w.
+    // moments: mp[p][a][b] at 6 p + (0, 1, 2 | 3, 4 | 5) for (a, b) = (0..2, 0 | 0..1, 1 | 0, 2);
+    // mr[p][a][b] at 18 + 3 p + (0, 1 | 2) for (a, b) = (0..1, 0 | 0, 1); gcc 24, w 25.
+    // Every wavefront runs the same four jobs, of 3, 2, 1 and 1 x powers, on planes from these
+    // tables (byte v for wavefront v): the third power of plane 1 (gxx t^1) is computed and
+    // dropped, and wavefront 3 has no fourth job (it repeats plane 14 and stores nothing).
+    constexpr unsigned P3 = 0x01060300u, B3 = 0x030c0600u;  // planes 0, 3, 6, 1 -> moments 0, 6, 12, 3
+    constexpr unsigned P2 = 0x0b090704u, B2 = 0x15120f09u;  // planes 4, 7, 9, 11 -> 9, 15, 18, 21
+    constexpr unsigned P1 = 0x0a080502u, B1 = 0x14110b05u;  // planes 2, 5, 8, 10 -> 5, 11, 17, 20
+    constexpr unsigned Q1 = 0x0e0e0d0cu, C1 = 0xff191817u;  // planes 12, 13, 14 -> 23, 24, 25
+    const int sh = 8 * wave;
+    double res[7][XB_B];
+    x_plane<R, 3, 0>(lds + ((P3 >> sh) & 255) * L::YS, lane, kt, res);
+    x_plane<R, 2, 3>(lds + ((P2 >> sh) & 255) * L::YS, lane, kt, res);
+    x_plane<R, 1, 5>(lds + ((P1 >> sh) & 255) * L::YS, lane, kt, res);
+    x_plane<R, 1, 6>(lds + ((Q1 >> sh) & 255) * L::YS, lane, kt, res);
+    __syncthreads();
+    double *mom = lds;
+    if (wave < 3)
+        x_put<3, 0>(mom, (B3 >> sh) & 255, lane, res);
+    else
+        x_put<2, 0>(mom, (B3 >> sh) & 255, lane, res);
+    x_put<2, 3>(mom, (B2 >> sh) & 255, lane, res);
+    x_put<1, 5>(mom, (B1 >> sh) & 255, lane, res);
+    if (wave < 3) x_put<1, 6>(mom, (C1 >> sh) & 255, lane, res);
+    __syncthreads();
+    const int x = x0 + 4 * lane + wave;
+    if (x >= w) return;
+    const double *mm = mom + wave * 64 + lane;
+    double mp[3][3][3], mr[2][2][2];
+#pragma unroll
+    for (int p = 0; p < 3; p++) {
+        mp[p][0][0] = mm[(6 * p + 0) * XB_P];
+        mp[p][1][0] = mm[(6 * p + 1) * XB_P];
+        mp[p][2][0] = mm[(6 * p + 2) * XB_P];
+        mp[p][0][1] = mm[(6 * p + 3) * XB_P];
+        mp[p][1][1] = mm[(6 * p + 4) * XB_P];
+        mp[p][0][2] = mm[(6 * p + 5) * XB_P];
+        mp[p][1][2] = mp[p][2][1] = mp[p][2][2] = 0;
+    }
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+        mr[p][0][0] = mm[(18 + 3 * p + 0) * XB_P];
+        mr[p][1][0] = mm[(18 + 3 * p + 1) * XB_P];
+        mr[p][0][1] = mm[(18 + 3 * p + 2) * XB_P];
+        mr[p][1][1] = 0;
+    }
+    solve_pixel(mp, mr, mm[24 * XB_P], mm[25 * XB_P], (size_t)y * w + x, ux_in, uy_in, ux_out, uy_out, energy, resid,
+                slope);
 }
 
 // Energy threshold (phaseflow.py:149): pixels with energy <= 1e-9 median + 1e-30 (or NaN) keep the
@@ -1224,9 +1391,8 @@ void moments_solve(const double *sums, double *ym, const double *ux_in, const do
         if (r == 16) {
             hipLaunchKernelGGL(k_moments_y_rb<16>, dim3((w + 255) / 256, (h + RB_ROWS - 1) / RB_ROWS, 7), dim3(256), 0,
                                s, sums, ym, w, h, kt_arg);
-            hipLaunchKernelGGL(k_moments_x_solve_tiled<16>, dim3((w + 255) / 256, h), dim3(256),
-                               (size_t)15 * (256 + 2 * r) * sizeof(double), s, ym, ux_in, uy_in, ux_out, uy_out, energy,
-                               resid, slope, w, h, kt_arg, r);
+            hipLaunchKernelGGL(k_moments_x_rb<16>, dim3((w + XB_P - 1) / XB_P, h), dim3(256), 0, s, ym, ux_in, uy_in,
+                               ux_out, uy_out, energy, resid, slope, w, h, kt_arg);
             LAUNCHCHECK();
             return;
         }
