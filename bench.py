@@ -36,6 +36,12 @@ def accumulate_bytes(w, h, F):
     return n * (16 + 8 + 8 + 16 * F + 56) + 8 * F * np_
 
 
+FP64_PEAK_TFLOPS = 78.6
+# f64 multiply-adds per pixel of one window-moment pass pair (sigma_w 4, 33 taps): the y pass forms
+# 15 planes from the 7 sums (15 x 33), the x pass the 26 moments from those (26 x 33)
+MOMENT_FMA_PER_PIXEL = 15 * 33 + 26 * 33
+
+
 def pipeline_bytes(w, h, tile=0):
     """Algorithmic HBM bytes of one whole pair with the default bank (F = 12 filters, 8 outer and
     2 inner iterations), each intermediate read once and written once: 6388 B per frame pixel
@@ -195,6 +201,17 @@ def main():
         with open(tfile) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
     stages = {k: round(v[0] / args.steps, 3) for k, v in prof.items() if v[1]}
+    # the second kernel pair by time: the window moments (y pass, then x pass with the 6 x 6 solve),
+    # bound by f64 arithmetic; flops of the moments only (the solve's are not counted)
+    mom_ms, mom_calls = prof["moments"]
+    mom_flop = 2.0 * MOMENT_FMA_PER_PIXEL * w * h * (mom_calls / n_tiles)
+    moments_roof = None
+    if mom_ms > 0 and cfg.shape_order == 1 and cfg.sigma_w == 4.0:
+        got = mom_flop / (mom_ms * 1e-3) / 1e12
+        moments_roof = {"kernels": "k_moments_y_rb<16> + k_moments_x_rb<16>", "bound": "fp64", "achieved": round(got, 2),
+                        "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": round(got / FP64_PEAK_TFLOPS, 4),
+                        "flop_per_launch_pair": 2 * MOMENT_FMA_PER_PIXEL * w * h,
+                        "avg_launch_pair_ms": round(mom_ms / max(mom_calls, 1), 4)}
 
     cpu = err = None
     if not args.no_cpu:
@@ -209,6 +226,7 @@ def main():
         "roofline": {"kernel": "k_accumulate", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_launch": acc_bytes, "avg_launch_ms": round(acc_avg_s * 1e3, 4)},
+        "roofline_moments": moments_roof,
         "stage_ms_per_step": stages,
         "epe_px": round(epe, 5), "valid_frac_roi": round(valid_frac, 5), "err_vs_oracle": err,
         "learned_coarse": learned,
