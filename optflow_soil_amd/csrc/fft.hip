@@ -203,6 +203,40 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                         a, b, DirectCore<SH>{P, tw_g});
 }
 
+// The compile-time form of the lengths 16 R1 R2 listed in ct_form (lines.hpp fft_line_ct).
+template <int N, int R1, int R2, int SH>
+struct CtCore {
+    const float2 *T, *tw;
+    template <class L>
+    __device__ float2 *operator()(L load, float2 *a, float2 *b, int tid, int) const {
+        return fft_line_ct<N, R1, R2, SH>(load, a, b, T, tw, tid);
+    }
+};
+
+template <int MODE, int SH, int N, int R1, int R2>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_lines_ct(const float2 *__restrict__ in, float2 *__restrict__ out, const float2 *__restrict__ T,
+                                                  const float2 *__restrict__ tw_g, Bank bank, int k, LineArgs A) {
+    extern __shared__ float2 lds[];
+    constexpr int ln = lp_len<SH>(N);
+    float2 *a = lds, *b = lds + ln;
+    line_body<MODE, SH>(in + blockIdx.y * A.in_plane, out + blockIdx.y * A.out_plane, N, bank, k + (int)blockIdx.y, A,
+                        a, b, CtCore<N, R1, R2, SH>{T, tw_g});
+}
+
+// T[k (R - 1) + q - 1] = tw[q k step] for k < p, 0 < q < R: a pass table of the compile-time form
+__global__ void k_ct_twiddles(float2 *__restrict__ T, const float2 *__restrict__ tw, int p, int R, int step) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= p * (R - 1)) return;
+    const int k = e / (R - 1), q = e - k * (R - 1) + 1;
+    T[e] = tw[q * k * step];
+}
+
+// The lengths with a compile-time form: n = 16 R1 R2, the plan's factors exactly (16, R1, R2)
+struct CtForm {
+    int n, r1, r2;
+};
+constexpr CtForm CT_FORMS[] = {{2112, 12, 11}, {576, 12, 3}, {832, 4, 13}};
+
 // The chirp-z form: the two line buffers of length m (m <= 8192: at most 136 KiB) leave room for
 // one workgroup per CU, so a workgroup of 1024 threads keeps 16 wavefronts on the CU.
 template <int MODE, int SH>
@@ -283,6 +317,10 @@ const FftPlan &plan_locked(int n, int max_n, hipStream_t s) {
         m /= r;
         big = std::max(big, r);
     };
+    // a radix-16 first pass (it needs no twiddles) and a radix-12 pass: fewer passes (2112 = 16 12 11
+    // in three instead of 8 8 3 11 in four), so fewer LDS round trips and barriers per line
+    if (m % 16 == 0) push(16);
+    if (m % 12 == 0) push(12);
     while (m % 8 == 0) push(8);
     while (m % 4 == 0) push(4);
     while (m % 2 == 0) push(2);
@@ -296,6 +334,15 @@ const FftPlan &plan_locked(int n, int max_n, hipStream_t s) {
     // can find the plan
     HIPCHECK(hipMalloc(&p.tw, n * sizeof(float2)));
     make_twiddles(p.tw, n, s);
+    for (const CtForm &f : CT_FORMS)
+        if (f.n == n && p.nfac == 3 && p.fac[0] == 16 && p.fac[1] == f.r1 && p.fac[2] == f.r2) {
+            const int t1 = 16 * (f.r1 - 1), t2 = 16 * f.r1 * (f.r2 - 1);
+            HIPCHECK(hipMalloc(&p.ct_tw, (t1 + t2) * sizeof(float2)));
+            hipLaunchKernelGGL(k_ct_twiddles, dim3((t1 + 255) / 256), dim3(256), 0, s, p.ct_tw, p.tw, 16, f.r1, f.r2);
+            hipLaunchKernelGGL(k_ct_twiddles, dim3((t2 + 255) / 256), dim3(256), 0, s, p.ct_tw + t1, p.tw, 16 * f.r1,
+                               f.r2, 1);
+            LAUNCHCHECK();
+        }
     if (big >= BLUE_MIN_PRIME) {
         // the smallest 2^a 3^b >= 2n - 1 (at most 8192 for n <= 4096)
         int best = 0;
@@ -374,6 +421,20 @@ void launch_lines(const float2 *in, float2 *out, int n, int lines, int planes, c
         return;
     }
     const size_t lds = line_lds_bytes(n);
+    if (P.ct_tw) {
+        const Bank bk = bank ? *bank : dummy;
+#define OFX_CT_LAUNCH(N_, R1_, R2_)                                                                                  \
+    if (n == N_) {                                                                                                  \
+        hipLaunchKernelGGL((k_lines_ct<MODE, 4, N_, R1_, R2_>), dim3(lines, planes), dim3(256), lds, s, in, out,    \
+                           P.ct_tw, P.tw, bk, k, A);                                                                \
+        LAUNCHCHECK();                                                                                              \
+        return;                                                                                                     \
+    }
+        OFX_CT_LAUNCH(2112, 12, 11)
+        OFX_CT_LAUNCH(576, 12, 3)
+        OFX_CT_LAUNCH(832, 4, 13)
+#undef OFX_CT_LAUNCH
+    }
     if (lds > 65536) {
         static std::once_flag once;
         std::call_once(once, [] {
@@ -425,6 +486,7 @@ void fft_release() {
         (void)hipFree(kv.second.tw);
         (void)hipFree(kv.second.chirp);
         (void)hipFree(kv.second.chirp_spec);
+        (void)hipFree(kv.second.ct_tw);
     }
     g_plans.clear();
 }

@@ -117,6 +117,8 @@ struct FftPlan {
     int blue_m = 0;
     float2 *chirp = nullptr;  // device: exp(i pi j^2 / n), j in [0, n)
     float2 *chirp_spec = nullptr;  // device: length-blue_m DFT of the wrapped chirp, divided by blue_m
+    // Lengths with a compile-time form (16 R1 R2, fft.hip): the twiddle tables of its passes 1 and 2
+    float2 *ct_tw = nullptr;
 };
 constexpr int BLUE_MIN_PRIME = 67;
 

@@ -118,6 +118,76 @@ __device__ __forceinline__ void dft_prime(float2 *x, const float2 *w) {
     x[0] = y0;
 }
 
+// exp(-2 pi i m / N) for N = 12 and 16, as (cos, -sin)
+template <int N>
+struct TwConst;
+template <>
+struct TwConst<16> {
+    static constexpr float c[16] = {1.f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+                                    0.f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
+                                    -1.f, -0.92387953251128675613f, -0.70710678118654752440f, -0.38268343236508977173f,
+                                    0.f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f};
+    static constexpr int Q = 4;  // c[(m + Q) mod N] is -sin(2 pi m / N)
+};
+template <>
+struct TwConst<12> {
+    static constexpr float c[12] = {1.f, 0.86602540378443864676f, 0.5f, 0.f, -0.5f, -0.86602540378443864676f,
+                                    -1.f, -0.86602540378443864676f, -0.5f, 0.f, 0.5f, 0.86602540378443864676f};
+    static constexpr int Q = 3;
+};
+
+// x * exp(-2 pi i m / N) for a compile-time m: exact for the multiples of N / 4
+template <int N, int M>
+__device__ __forceinline__ float2 tw_mul(float2 x) {
+    constexpr int m = M % N;
+    if constexpr (m == 0) return x;
+    else if constexpr (4 * m == N) return mul_minus_i(x);
+    else if constexpr (2 * m == N) return make_float2(-x.x, -x.y);
+    else if constexpr (4 * m == 3 * N) return mul_plus_i(x);
+    else {
+        constexpr float c = TwConst<N>::c[m], sn = TwConst<N>::c[(m + TwConst<N>::Q) % N];  // cos, -sin
+        return make_float2(x.x * c - x.y * sn, x.x * sn + x.y * c);
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void dft_small(float2 *x);
+
+// DFT of length N = N1 N2 from DFTs of N1 and N2 (Cooley-Tukey, twiddles in between):
+// X[k1 + N1 k2] = sum_n2 W_N2^(n2 k2) W_N^(n2 k1) sum_n1 W_N1^(n1 k1) x[N2 n1 + n2].
+template <int N, int N2I, int N1, int K = 0>
+__device__ __forceinline__ void twiddle_row(const float2 (&t)[N1], float2 (&row)[N1]) {
+    if constexpr (K < N1) {
+        row[K] = tw_mul<N, N2I * K>(t[K]);
+        twiddle_row<N, N2I, N1, K + 1>(t, row);
+    }
+}
+template <int N1, int N2, int N2I = 0>
+__device__ __forceinline__ void inner_columns(const float2 *x, float2 (&y)[N2][N1]) {
+    if constexpr (N2I < N2) {
+        float2 t[N1];
+#pragma unroll
+        for (int n1 = 0; n1 < N1; n1++) t[n1] = x[N2 * n1 + N2I];
+        dft_small<N1>(t);
+        twiddle_row<N1 * N2, N2I, N1>(t, y[N2I]);
+        inner_columns<N1, N2, N2I + 1>(x, y);
+    }
+}
+template <int N1, int N2>
+__device__ __forceinline__ void dft_composite(float2 *x) {
+    float2 y[N2][N1];
+    inner_columns<N1, N2>(x, y);
+#pragma unroll
+    for (int k1 = 0; k1 < N1; k1++) {
+        float2 t[N2];
+#pragma unroll
+        for (int n2 = 0; n2 < N2; n2++) t[n2] = y[n2][k1];
+        dft_small<N2>(t);
+#pragma unroll
+        for (int k2 = 0; k2 < N2; k2++) x[k1 + N1 * k2] = t[k2];
+    }
+}
+
 template <int R>
 __device__ __forceinline__ void dft_small(float2 *x) {
     if constexpr (R == 2) dft2(x);
@@ -125,6 +195,8 @@ __device__ __forceinline__ void dft_small(float2 *x) {
     else if constexpr (R == 4) dft4(x);
     else if constexpr (R == 5) dft5(x);
     else if constexpr (R == 8) dft8(x);
+    else if constexpr (R == 12) dft_composite<4, 3>(x);
+    else if constexpr (R == 16) dft_composite<4, 4>(x);
 }
 
 // One self-sorting Stockham pass (radix R, p = product of the radices already applied) of `lines`
@@ -278,6 +350,8 @@ __device__ __forceinline__ float2 *fft_line_from(L load, float2 *a, float2 *b, i
         case 5: first_pass_small<5, SH>(load, b, n, tw, tid, nt); break;
         case 7: first_pass_small<7, SH>(load, b, n, tw, tid, nt); break;
         case 8: first_pass_small<8, SH>(load, b, n, tw, tid, nt); break;
+        case 12: first_pass_small<12, SH>(load, b, n, tw, tid, nt); break;
+        case 16: first_pass_small<16, SH>(load, b, n, tw, tid, nt); break;
         case 11: first_pass_small<11, SH>(load, b, n, tw, tid, nt); break;
         case 13: first_pass_small<13, SH>(load, b, n, tw, tid, nt); break;
         default:
@@ -301,6 +375,7 @@ __device__ __forceinline__ float2 *fft_line_from(L load, float2 *a, float2 *b, i
             case 4: stockham_line<4, SH>(a, b, n, p, mg, tw, tid, nt); break;
             case 5: stockham_line<5, SH>(a, b, n, p, mg, tw, tid, nt); break;
             case 8: stockham_line<8, SH>(a, b, n, p, mg, tw, tid, nt); break;
+            case 12: stockham_line<12, SH>(a, b, n, p, mg, tw, tid, nt); break;
             case 7: stockham_line<7, SH>(a, b, n, p, mg, tw, tid, nt); break;
             case 11: stockham_line<11, SH>(a, b, n, p, mg, tw, tid, nt); break;
             case 13: stockham_line<13, SH>(a, b, n, p, mg, tw, tid, nt); break;
@@ -312,6 +387,71 @@ __device__ __forceinline__ float2 *fft_line_from(L load, float2 *a, float2 *b, i
         b = t;
         p *= R;
     }
+    return a;
+}
+
+// ---------------------------------------------------------------- compile-time line transforms
+// For a line length N = R0 R1 R2 with R0 = 16 whose later strides are multiples of 16, the padded
+// LDS index lp(i + q M) of a butterfly's q-th element is lp(i) + q M 17 / 16, so every read and write
+// of a pass is a constant offset from one base address, and the pass's twiddles W_{P R}^{q k} come
+// from a table laid out [k][q - 1] (values gathered from the line's table, so the arithmetic is that
+// of the runtime passes). 256 threads per line.
+template <int R, int N, int P, int SH>
+__device__ __forceinline__ void ct_pass(const float2 *src, float2 *dst, const float2 *__restrict__ T,
+                                        const float2 *__restrict__ tw_line, int tid) {
+    constexpr int M = N / R;
+    static_assert(SH == 4 && M % 16 == 0 && P % 16 == 0, "strides of whole 16-element groups");
+    constexpr int SM = M + M / 16, SP = P + P / 16;
+    float2 wr[R];
+    if constexpr (R == 7 || R == 11 || R == 13) {
+#pragma unroll
+        for (int q = 0; q < R; q++) wr[q] = tw_line[q * M];
+    }
+#pragma unroll
+    for (int i0 = 0; i0 < M; i0 += 256) {
+        const int i = i0 + tid;
+        if (M % 256 != 0 && i >= M) break;
+        const int k = i % P;
+        float2 x[R];
+        const float2 *sp = src + lp<SH>(i);
+#pragma unroll
+        for (int q = 0; q < R; q++) x[q] = sp[q * SM];
+        const float2 *t = T + k * (R - 1);
+#pragma unroll
+        for (int q = 1; q < R; q++) x[q] = cmul(x[q], t[q - 1]);
+        if constexpr (R == 7 || R == 11 || R == 13) dft_prime<R>(x, wr);
+        else dft_small<R>(x);
+        float2 *dp = dst + lp<SH>((i / P) * P * R + k);
+#pragma unroll
+        for (int q = 0; q < R; q++) dp[q * SP] = x[q];
+    }
+}
+
+// The three passes R0 = 16, R1, R2 of a length-N line whose element e is load(e); T holds the
+// tables of passes 1 and 2 one after the other. Returns the buffer holding the result (a); ends
+// with a barrier.
+template <int N, int R1, int R2, int SH, class L>
+__device__ __forceinline__ float2 *fft_line_ct(L load, float2 *a, float2 *b, const float2 *__restrict__ T,
+                                               const float2 *__restrict__ tw_line, int tid) {
+    constexpr int M0 = N / 16;
+    static_assert(N == 16 * R1 * R2, "N = 16 R1 R2");
+#pragma unroll
+    for (int i0 = 0; i0 < M0; i0 += 256) {
+        const int i = i0 + tid;
+        if (M0 % 256 != 0 && i >= M0) break;
+        float2 x[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) x[q] = load(i + q * M0);
+        dft_small<16>(x);
+        float2 *dp = a + 17 * i;  // lp(16 i + q) = 17 i + q
+#pragma unroll
+        for (int q = 0; q < 16; q++) dp[q] = x[q];
+    }
+    __syncthreads();
+    ct_pass<R1, N, 16, SH>(a, b, T, tw_line, tid);
+    __syncthreads();
+    ct_pass<R2, N, 16 * R1, SH>(b, a, T + 16 * (R1 - 1), tw_line, tid);
+    __syncthreads();
     return a;
 }
 
