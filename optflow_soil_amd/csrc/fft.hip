@@ -203,24 +203,24 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
                         a, b, DirectCore<SH>{P, tw_g});
 }
 
-// The compile-time form of the lengths 16 R1 R2 listed in ct_form (lines.hpp fft_line_ct).
-template <int N, int R1, int R2, int SH>
+// The compile-time form of a line length 16 Rs... (lines.hpp fft_line_ct), 256 threads per line.
+template <int N, int SH, int... Rs>
 struct CtCore {
     const float2 *T, *tw;
     template <class L>
     __device__ float2 *operator()(L load, float2 *a, float2 *b, int tid, int) const {
-        return fft_line_ct<N, R1, R2, SH>(load, a, b, T, tw, tid);
+        return fft_line_ct<N, SH, 256, Rs...>(load, a, b, T, tw, tid);
     }
 };
 
-template <int MODE, int SH, int N, int R1, int R2>
+template <int MODE, int SH, int N, int... Rs>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k_lines_ct(const float2 *__restrict__ in, float2 *__restrict__ out, const float2 *__restrict__ T,
                                                   const float2 *__restrict__ tw_g, Bank bank, int k, LineArgs A) {
     extern __shared__ float2 lds[];
     constexpr int ln = lp_len<SH>(N);
     float2 *a = lds, *b = lds + ln;
     line_body<MODE, SH>(in + blockIdx.y * A.in_plane, out + blockIdx.y * A.out_plane, N, bank, k + (int)blockIdx.y, A,
-                        a, b, CtCore<N, R1, R2, SH>{T, tw_g});
+                        a, b, CtCore<N, SH, Rs...>{T, tw_g});
 }
 
 // T[k (R - 1) + q - 1] = tw[q k step] for k < p, 0 < q < R: a pass table of the compile-time form
@@ -231,11 +231,15 @@ __global__ void k_ct_twiddles(float2 *__restrict__ T, const float2 *__restrict__
     T[e] = tw[q * k * step];
 }
 
-// The lengths with a compile-time form: n = 16 R1 R2, the plan's factors exactly (16, R1, R2)
+// The lengths with a compile-time form: the plan's factors exactly (16, r[0], r[1], ...). 2112,
+// 1088, 576 and 832 are the padded sides of 2048, 1024, 512 and 768 px frames; 1216 that of a
+// 1024 px tile with its margins; 1600 that of 1536 px; 8192 and 6144 the chirp-z lengths of the
+// 4000 x 3000 frame's sides (4064 and 3064 padded).
 struct CtForm {
-    int n, r1, r2;
+    int n, nr, r[3];
 };
-constexpr CtForm CT_FORMS[] = {{2112, 12, 11}, {576, 12, 3}, {832, 4, 13}};
+constexpr CtForm CT_FORMS[] = {{2112, 2, {12, 11}}, {1088, 2, {4, 17}}, {576, 2, {12, 3}},  {832, 2, {4, 13}},
+                               {1216, 2, {4, 19}},  {1600, 3, {4, 5, 5}}, {8192, 3, {8, 8, 8}}, {6144, 3, {12, 8, 4}}};
 
 // The chirp-z form: the two line buffers of length m (m <= 8192: at most 136 KiB) leave room for
 // one workgroup per CU, so a workgroup of 1024 threads keeps 16 wavefronts on the CU.
@@ -246,6 +250,38 @@ __global__ void __launch_bounds__(1024) k_lines_blue(const float2 *__restrict__ 
     const int lm = lp_len<SH>(B.M.n);
     line_body<MODE, SH>(in + blockIdx.y * A.in_plane, out + blockIdx.y * A.out_plane, n, bank, k + (int)blockIdx.y, A,
                         lds, lds + lm, BlueCore<SH>{B, n});
+}
+
+// The chirp-z form with its length-m transforms in the compile-time form (m = 16 Rs...).
+template <int M, int SH, int... Rs>
+struct BlueCtCore {
+    const float2 *T, *tw, *ch, *C;
+    int n;
+    template <class L>
+    __device__ float2 *operator()(L load, float2 *a, float2 *b, int tid, int) const {
+        const int n_ = n;
+        const float2 *ch_ = ch, *C_ = C;
+        float2 *r = fft_line_ct<M, SH, 1024, Rs...>(
+            [&](int e) { return e < n_ ? cmul(load(e), cconj(ch_[e])) : make_float2(0.f, 0.f); }, a, b, T, tw, tid);
+        float2 *o = (r == a) ? b : a;
+        // the first pass reads r through the loader and writes o, so r is free for the later passes
+        float2 *r2 = fft_line_ct<M, SH, 1024, Rs...>([&](int e) { return cconj(cmul(r[lp<SH>(e)], C_[e])); }, o, r, T,
+                                                     tw, tid);
+        for (int k = tid; k < n_; k += 1024) r2[lp<SH>(k)] = cconj(cmul(ch_[k], r2[lp<SH>(k)]));
+        __syncthreads();
+        return r2;
+    }
+};
+
+template <int MODE, int SH, int M, int... Rs>
+__global__ void __launch_bounds__(1024) k_lines_blue_ct(const float2 *__restrict__ in, float2 *__restrict__ out, int n,
+                                                        const float2 *__restrict__ T, const float2 *__restrict__ tw,
+                                                        const float2 *__restrict__ ch, const float2 *__restrict__ C,
+                                                        Bank bank, int k, LineArgs A) {
+    extern __shared__ float2 lds[];
+    constexpr int lm = lp_len<SH>(M);
+    line_body<MODE, SH>(in + blockIdx.y * A.in_plane, out + blockIdx.y * A.out_plane, n, bank, k + (int)blockIdx.y, A,
+                        lds, lds + lm, BlueCtCore<M, SH, Rs...>{T, tw, ch, C, n});
 }
 
 // b_j = exp(i pi (j^2 mod 2n) / n) from the double angle (the product j^2 is exact in 64 bits)
@@ -334,15 +370,22 @@ const FftPlan &plan_locked(int n, int max_n, hipStream_t s) {
     // can find the plan
     HIPCHECK(hipMalloc(&p.tw, n * sizeof(float2)));
     make_twiddles(p.tw, n, s);
-    for (const CtForm &f : CT_FORMS)
-        if (f.n == n && p.nfac == 3 && p.fac[0] == 16 && p.fac[1] == f.r1 && p.fac[2] == f.r2) {
-            const int t1 = 16 * (f.r1 - 1), t2 = 16 * f.r1 * (f.r2 - 1);
-            HIPCHECK(hipMalloc(&p.ct_tw, (t1 + t2) * sizeof(float2)));
-            hipLaunchKernelGGL(k_ct_twiddles, dim3((t1 + 255) / 256), dim3(256), 0, s, p.ct_tw, p.tw, 16, f.r1, f.r2);
-            hipLaunchKernelGGL(k_ct_twiddles, dim3((t2 + 255) / 256), dim3(256), 0, s, p.ct_tw + t1, p.tw, 16 * f.r1,
-                               f.r2, 1);
-            LAUNCHCHECK();
+    for (const CtForm &f : CT_FORMS) {
+        bool match = f.n == n && p.nfac == f.nr + 1 && p.fac[0] == 16;
+        for (int j = 0; match && j < f.nr; j++) match = p.fac[j + 1] == f.r[j];
+        if (!match) continue;
+        size_t total = 0;
+        for (int j = 0, pp = 16; j < f.nr; pp *= f.r[j], j++) total += (size_t)pp * (f.r[j] - 1);
+        HIPCHECK(hipMalloc(&p.ct_tw, total * sizeof(float2)));
+        size_t off = 0;
+        for (int j = 0, pp = 16; j < f.nr; pp *= f.r[j], j++) {
+            const int cnt = pp * (f.r[j] - 1);
+            hipLaunchKernelGGL(k_ct_twiddles, dim3((cnt + 255) / 256), dim3(256), 0, s, p.ct_tw + off, p.tw, pp,
+                               f.r[j], n / (pp * f.r[j]));
+            off += cnt;
         }
+        LAUNCHCHECK();
+    }
     if (big >= BLUE_MIN_PRIME) {
         // the smallest 2^a 3^b >= 2n - 1 (at most 8192 for n <= 4096)
         int best = 0;
@@ -409,6 +452,24 @@ void launch_lines(const float2 *in, float2 *out, int n, int lines, int planes, c
     if (P.blue_m) {
         const FftPlan &M = fft_plan_m(P.blue_m, s);
         const size_t lds = 2 * (size_t)lp_len<4>(P.blue_m) * sizeof(float2);
+        if (M.ct_tw) {
+            const Bank bk = bank ? *bank : dummy;
+#define OFX_BLUE_CT_LAUNCH(M_, ...)                                                                                 \
+    if (P.blue_m == M_) {                                                                                           \
+        static std::once_flag once_ct;                                                                              \
+        std::call_once(once_ct, [] {                                                                                \
+            (void)hipFuncSetAttribute((const void *)k_lines_blue_ct<MODE, 4, M_, __VA_ARGS__>,                      \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);                      \
+        });                                                                                                         \
+        hipLaunchKernelGGL((k_lines_blue_ct<MODE, 4, M_, __VA_ARGS__>), dim3(lines, planes), dim3(1024), lds, s, in, \
+                           out, n, M.ct_tw, M.tw, P.chirp, P.chirp_spec, bk, k, A);                                  \
+        LAUNCHCHECK();                                                                                              \
+        return;                                                                                                     \
+    }
+            OFX_BLUE_CT_LAUNCH(8192, 8, 8, 8)
+            OFX_BLUE_CT_LAUNCH(6144, 12, 8, 4)
+#undef OFX_BLUE_CT_LAUNCH
+        }
         static std::once_flag once;
         std::call_once(once, [] {
             (void)hipFuncSetAttribute((const void *)k_lines_blue<MODE, 4>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -423,16 +484,19 @@ void launch_lines(const float2 *in, float2 *out, int n, int lines, int planes, c
     const size_t lds = line_lds_bytes(n);
     if (P.ct_tw) {
         const Bank bk = bank ? *bank : dummy;
-#define OFX_CT_LAUNCH(N_, R1_, R2_)                                                                                  \
+#define OFX_CT_LAUNCH(N_, ...)                                                                                      \
     if (n == N_) {                                                                                                  \
-        hipLaunchKernelGGL((k_lines_ct<MODE, 4, N_, R1_, R2_>), dim3(lines, planes), dim3(256), lds, s, in, out,    \
+        hipLaunchKernelGGL((k_lines_ct<MODE, 4, N_, __VA_ARGS__>), dim3(lines, planes), dim3(256), lds, s, in, out, \
                            P.ct_tw, P.tw, bk, k, A);                                                                \
         LAUNCHCHECK();                                                                                              \
         return;                                                                                                     \
     }
         OFX_CT_LAUNCH(2112, 12, 11)
+        OFX_CT_LAUNCH(1088, 4, 17)
         OFX_CT_LAUNCH(576, 12, 3)
         OFX_CT_LAUNCH(832, 4, 13)
+        OFX_CT_LAUNCH(1216, 4, 19)
+        OFX_CT_LAUNCH(1600, 4, 5, 5)
 #undef OFX_CT_LAUNCH
     }
     if (lds > 65536) {

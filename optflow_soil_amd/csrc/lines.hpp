@@ -391,26 +391,31 @@ __device__ __forceinline__ float2 *fft_line_from(L load, float2 *a, float2 *b, i
 }
 
 // ---------------------------------------------------------------- compile-time line transforms
-// For a line length N = R0 R1 R2 with R0 = 16 whose later strides are multiples of 16, the padded
-// LDS index lp(i + q M) of a butterfly's q-th element is lp(i) + q M 17 / 16, so every read and write
-// of a pass is a constant offset from one base address, and the pass's twiddles W_{P R}^{q k} come
-// from a table laid out [k][q - 1] (values gathered from the line's table, so the arithmetic is that
-// of the runtime passes). 256 threads per line.
-template <int R, int N, int P, int SH>
+// For a line length N = 16 R1 R2 ... whose later strides are multiples of 16, the padded LDS index
+// lp(i + q M) of a butterfly's q-th element is lp(i) + q M 17 / 16, so every read and write of a
+// pass is a constant offset from one base address, and the pass's twiddles W_{P R}^{q k} come from
+// a table laid out [k][q - 1] (values gathered from the line's table, so the arithmetic is that of
+// the runtime passes). NT threads per line.
+template <int R>
+constexpr bool prime_radix() {
+    return R == 7 || R == 11 || R == 13 || R == 17 || R == 19;
+}
+
+template <int R, int N, int P, int SH, int NT>
 __device__ __forceinline__ void ct_pass(const float2 *src, float2 *dst, const float2 *__restrict__ T,
                                         const float2 *__restrict__ tw_line, int tid) {
     constexpr int M = N / R;
     static_assert(SH == 4 && M % 16 == 0 && P % 16 == 0, "strides of whole 16-element groups");
     constexpr int SM = M + M / 16, SP = P + P / 16;
     float2 wr[R];
-    if constexpr (R == 7 || R == 11 || R == 13) {
+    if constexpr (prime_radix<R>()) {
 #pragma unroll
         for (int q = 0; q < R; q++) wr[q] = tw_line[q * M];
     }
 #pragma unroll
-    for (int i0 = 0; i0 < M; i0 += 256) {
+    for (int i0 = 0; i0 < M; i0 += NT) {
         const int i = i0 + tid;
-        if (M % 256 != 0 && i >= M) break;
+        if (M % NT != 0 && i >= M) break;
         const int k = i % P;
         float2 x[R];
         const float2 *sp = src + lp<SH>(i);
@@ -419,7 +424,7 @@ __device__ __forceinline__ void ct_pass(const float2 *src, float2 *dst, const fl
         const float2 *t = T + k * (R - 1);
 #pragma unroll
         for (int q = 1; q < R; q++) x[q] = cmul(x[q], t[q - 1]);
-        if constexpr (R == 7 || R == 11 || R == 13) dft_prime<R>(x, wr);
+        if constexpr (prime_radix<R>()) dft_prime<R>(x, wr);
         else dft_small<R>(x);
         float2 *dp = dst + lp<SH>((i / P) * P * R + k);
 #pragma unroll
@@ -427,18 +432,34 @@ __device__ __forceinline__ void ct_pass(const float2 *src, float2 *dst, const fl
     }
 }
 
-// The three passes R0 = 16, R1, R2 of a length-N line whose element e is load(e); T holds the
-// tables of passes 1 and 2 one after the other. Returns the buffer holding the result (a); ends
-// with a barrier.
-template <int N, int R1, int R2, int SH, class L>
+// The passes after the first: radix R at stage P, then the rest; returns the buffer holding the
+// result. Each pass ends with a barrier.
+template <int N, int P, int SH, int NT, int R, int... Rest>
+__device__ __forceinline__ float2 *ct_passes(float2 *src, float2 *dst, const float2 *__restrict__ T,
+                                             const float2 *__restrict__ tw_line, int tid) {
+    ct_pass<R, N, P, SH, NT>(src, dst, T, tw_line, tid);
+    __syncthreads();
+    if constexpr (sizeof...(Rest) == 0) return dst;
+    else return ct_passes<N, P * R, SH, NT, Rest...>(dst, src, T + P * (R - 1), tw_line, tid);
+}
+
+template <int... Rs>
+constexpr int radix_product() {
+    return (1 * ... * Rs);
+}
+
+// The transform of a length-N line (N = 16 Rs...) whose element e is load(e): a radix-16 first pass
+// from the loader, then the passes Rs; T holds the tables of the later passes one after the other.
+// Returns the buffer holding the result; ends with a barrier.
+template <int N, int SH, int NT, int... Rs, class L>
 __device__ __forceinline__ float2 *fft_line_ct(L load, float2 *a, float2 *b, const float2 *__restrict__ T,
                                                const float2 *__restrict__ tw_line, int tid) {
     constexpr int M0 = N / 16;
-    static_assert(N == 16 * R1 * R2, "N = 16 R1 R2");
+    static_assert(N == 16 * radix_product<Rs...>(), "N = 16 Rs...");
 #pragma unroll
-    for (int i0 = 0; i0 < M0; i0 += 256) {
+    for (int i0 = 0; i0 < M0; i0 += NT) {
         const int i = i0 + tid;
-        if (M0 % 256 != 0 && i >= M0) break;
+        if (M0 % NT != 0 && i >= M0) break;
         float2 x[16];
 #pragma unroll
         for (int q = 0; q < 16; q++) x[q] = load(i + q * M0);
@@ -448,11 +469,7 @@ __device__ __forceinline__ float2 *fft_line_ct(L load, float2 *a, float2 *b, con
         for (int q = 0; q < 16; q++) dp[q] = x[q];
     }
     __syncthreads();
-    ct_pass<R1, N, 16, SH>(a, b, T, tw_line, tid);
-    __syncthreads();
-    ct_pass<R2, N, 16 * R1, SH>(b, a, T + 16 * (R1 - 1), tw_line, tid);
-    __syncthreads();
-    return a;
+    return ct_passes<N, 16, SH, NT, Rs...>(a, b, T, tw_line, tid);
 }
 
 // Stockham pass with compile-time radix R, line length N and p = P over `lines` lines (line stride N):
