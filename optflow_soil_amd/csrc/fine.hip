@@ -292,9 +292,19 @@ __global__ void __launch_bounds__(256) k_accumulate_tiled(const double *__restri
         for (int j = 0; j < 4; j++)
 #pragma unroll
             for (int k = 0; k < 4; k++) o[4 * j + k] = (oy + j) * bw + ox + k;
-        for (int e = threadIdx.x; e < bn; e += blockDim.x) {
+        // offsets in a response plane of this thread's box elements e = tid + 256 q, the same for
+        // every filter (a plane holds fewer than 2^31 nodes: the sides are at most 7744)
+        int boff[ACC_BOX / 256 + 1];
+#pragma unroll
+        for (int q = 0; q < ACC_BOX / 256 + 1; q++) {
+            const int e = threadIdx.x + 256 * q;
             const int r = e / bw, c = e - r * bw;
-            box[0][e] = resp[(size_t)(by0 + r) * wp + bx0 + c];
+            boff[q] = e < bn ? (by0 + r) * wp + bx0 + c : 0;
+        }
+#pragma unroll
+        for (int q = 0; q < ACC_BOX / 256 + 1; q++) {
+            const int e = threadIdx.x + 256 * q;
+            if (e < bn) box[0][e] = resp[boff[q]];
         }
         float4 pc = inside ? refp[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         for (int f = 0; f < nf; f++) {
@@ -306,10 +316,7 @@ __global__ void __launch_bounds__(256) k_accumulate_tiled(const double *__restri
 #pragma unroll
                 for (int q = 0; q < ACC_BOX / 256 + 1; q++) {
                     const int e = threadIdx.x + 256 * q;
-                    if (e < bn) {
-                        const int r = e / bw, c = e - r * bw;
-                        nx[q] = plane[(size_t)(by0 + r) * wp + bx0 + c];
-                    }
+                    if (e < bn) nx[q] = plane[boff[q]];
                 }
                 if (inside) pn = refp[(size_t)(f + 1) * n + i];
             }
