@@ -213,6 +213,7 @@ def main():
                         "flop_per_launch_pair": 2 * MOMENT_FMA_PER_PIXEL * w * h,
                         "avg_launch_pair_ms": round(mom_ms / max(mom_calls, 1), 4)}
 
+    writer = output_writer_rate(outs, w, h)
     cpu = err = None
     if not args.no_cpu:
         cpu, crop_info = cpu_baseline(a, b, roi, cfg)
@@ -234,9 +235,40 @@ def main():
                      "achieved": round(pipeline_bytes(w, h, cfg.tile) / dt / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s",
                      "frac": round(pipeline_bytes(w, h, cfg.tile) / dt / 1e9 / HBM_PEAK_GBS, 4)},
+        "output_writer": writer,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
+
+
+def output_writer_rate(outs, w, h, runs=3):
+    """The host rate of writing one pair's result folder (the reference CLI's files, the --raw maps
+    included) through ofx_write_result, the writer the command-line twin uses: the last pair's maps
+    copied to the host, then the folder written into a temporary directory `runs` times, the
+    fastest kept. Not part of the timed value (the metric excludes I/O)."""
+    import shutil
+    import tempfile
+    from optflow_soil_amd import Result, io
+    host = {k: v.cpu().numpy() for k, v in outs.items()}
+    strain = {"Exx": host["exx"], "Eyy": host["eyy"], "Exy": host["exy"], "gmax": host["gmax"], "vol": host["vol"]}
+    res = Result(host["ux"], host["uy"], host["conf"], strain, host["raw_ux"], host["raw_uy"], host["valid"] != 0,
+                 {"coarse": 0.0, "fine": 0.0, "strain": 0.0})
+    d = tempfile.mkdtemp(prefix="ofx-writer-")
+    try:
+        best = float("inf")
+        for k in range(runs):
+            t0 = time.perf_counter()
+            io.write_result(os.path.join(d, f"run{k}"), res, raw=True)
+            best = min(best, time.perf_counter() - t0)
+        folder = os.path.join(d, "run0")
+        nbytes = sum(os.path.getsize(os.path.join(folder, f)) for f in os.listdir(folder))
+        nfiles = len(os.listdir(folder))
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return {"mpix_per_s": round(w * h / 1e6 / best, 1), "gb_per_s": round(nbytes / best / 1e9, 2),
+            "ms_per_folder": round(best * 1e3, 2), "files": nfiles, "bytes": nbytes,
+            "sample": f"the last pair's result folder with the raw maps, min of {runs} writes to a temporary "
+                      "directory on the host"}
 
 
 def bench_c3(args):

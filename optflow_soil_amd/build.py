@@ -38,7 +38,7 @@ CLI = os.path.join(HERE, "ofx_cli")
 def build_cli():
     """The command-line twin (cli/ofx_cli.cpp): host C++ over libofx.so, found beside it at run time."""
     tmp = CLI + ".tmp"
-    subprocess.run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", os.path.join(HERE, "cli", "ofx_cli.cpp"),
+    subprocess.run([os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall", "-pthread", os.path.join(HERE, "cli", "ofx_cli.cpp"),
                     "-L" + HERE, "-lofx", "-Wl,-rpath,$ORIGIN", "-Wl,--allow-shlib-undefined", "-o", tmp], check=True)
     os.replace(tmp, CLI)
     return CLI

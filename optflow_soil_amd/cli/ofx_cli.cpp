@@ -20,15 +20,22 @@
 #include <algorithm>
 #include <cctype>
 #include <cerrno>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <deque>
 #include <dirent.h>
+#include <exception>
 #include <limits.h>
 #include <map>
+#include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <unistd.h>
 #include <vector>
 
@@ -43,6 +50,60 @@ struct Fail : std::runtime_error {
 void check(int rc) {
     if (rc != OFX_OK) throw Fail(ofx_last_error());
 }
+
+double seconds_since(std::chrono::steady_clock::time_point t0) {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// A bounded first-in first-out hand-off from one producing thread to one consuming thread. The
+// producer ends the stream with finish() (passing its error, if any, to the consumer); the consumer
+// gives up with abandon(), after which put() returns false at once.
+template <class T>
+class Channel {
+  public:
+    explicit Channel(size_t capacity) : cap_(capacity) {}
+    bool put(T v) {
+        std::unique_lock<std::mutex> l(m_);
+        cv_.wait(l, [&] { return q_.size() < cap_ || abandoned_; });
+        if (abandoned_) return false;
+        q_.push_back(std::move(v));
+        cv_.notify_all();
+        return true;
+    }
+    // false once the stream has ended and every item has been taken; rethrows the producer's error
+    bool take(T &v) {
+        std::unique_lock<std::mutex> l(m_);
+        cv_.wait(l, [&] { return !q_.empty() || finished_; });
+        if (q_.empty()) {
+            if (err_) std::rethrow_exception(err_);
+            return false;
+        }
+        v = std::move(q_.front());
+        q_.pop_front();
+        cv_.notify_all();
+        return true;
+    }
+    void finish(std::exception_ptr err = nullptr) {
+        std::lock_guard<std::mutex> l(m_);
+        finished_ = true;
+        err_ = err;
+        cv_.notify_all();
+    }
+    void abandon() {
+        std::lock_guard<std::mutex> l(m_);
+        abandoned_ = true;
+        q_.clear();
+        cv_.notify_all();
+    }
+
+  private:
+    size_t cap_;
+    std::mutex m_;
+    std::condition_variable cv_;
+    std::deque<T> q_;
+    bool finished_ = false, abandoned_ = false;
+    std::exception_ptr err_;
+};
 
 // ------------------------------------------------------------------ frames
 struct Pixels {
@@ -694,17 +755,77 @@ int run(const Args &a) {
         ofx_sequence *q;
         ~End() { ofx_seq_end(q); }
     } end{q};
-    for (size_t k = 1; k < paths.size(); k++) {
+    // Three stages overlap: a reader thread decodes frame k + 1 from its file while this thread
+    // converts and tracks frame k on the GPU, and a writer thread writes frame k - 1's folder. Each
+    // hand-off holds at most two items, which bounds the host memory at a few frames.
+    const auto t_all = std::chrono::steady_clock::now();
+    double t_read = 0, t_track = 0, t_write = 0;
+    Channel<Pixels> decoded(2);
+    struct Done {
+        size_t k;
+        std::unique_ptr<Result> res;
+    };
+    Channel<Done> tracked(2);
+    std::thread reader([&] {
+        try {
+            for (size_t k = 1; k < paths.size(); k++) {
+                const auto t0 = std::chrono::steady_clock::now();
+                Pixels p = read_pixels(paths[k]);
+                t_read += seconds_since(t0);
+                if (!decoded.put(std::move(p))) return;
+            }
+            decoded.finish();
+        } catch (...) {
+            decoded.finish(std::current_exception());
+        }
+    });
+    std::exception_ptr write_err;
+    std::thread writer([&] {
+        try {
+            Done d;
+            while (tracked.take(d)) {
+                const auto t0 = std::chrono::steady_clock::now();
+                char dir[32];
+                std::snprintf(dir, sizeof dir, "/frame_%05zu", d.k);
+                check(ofx_write_result((a.out + dir).c_str(), &d.res->r, cw, ch, place, w, h, a.raw, 1));
+                t_write += seconds_since(t0);
+                std::printf("frame %zu: reference %d, mean confidence %.4f\n", d.k, d.res->r.reference,
+                            d.res->r.mean_conf);
+            }
+        } catch (...) {
+            write_err = std::current_exception();
+            tracked.abandon();
+        }
+    });
+    // stops both threads on every way out of this scope, an error here included
+    struct Join {
+        Channel<Pixels> &decoded;
+        Channel<Done> &tracked;
+        std::thread &reader, &writer;
+        ~Join() {
+            decoded.abandon();
+            tracked.finish();
+            if (reader.joinable()) reader.join();
+            if (writer.joinable()) writer.join();
+        }
+    } join{decoded, tracked, reader, writer};
+    Pixels px;
+    for (size_t k = 1; decoded.take(px); k++) {
+        const auto t0 = std::chrono::steady_clock::now();
         int wk, hk;
-        const std::vector<float> fk = to_gray(e, read_pixels(paths[k]), (int)a.scale, wk, hk);
+        const std::vector<float> fk = to_gray(e, px, (int)a.scale, wk, hk);
         if (wk != w || hk != h) throw Fail("frame " + paths[k] + " differs in size from the first");
-        Result res(cw, ch, st.c.seq_hybrid != 0);
-        check(ofx_seq_step(q, view(fk).data(), &res.r));
-        char dir[32];
-        std::snprintf(dir, sizeof dir, "/frame_%05zu", k);
-        check(ofx_write_result((a.out + dir).c_str(), &res.r, cw, ch, place, w, h, a.raw, 1));
-        std::printf("frame %zu: reference %d, mean confidence %.4f\n", k, res.r.reference, res.r.mean_conf);
+        auto res = std::make_unique<Result>(cw, ch, st.c.seq_hybrid != 0);
+        check(ofx_seq_step(q, view(fk).data(), &res->r));
+        t_track += seconds_since(t0);
+        if (!tracked.put(Done{k, std::move(res)})) break;
     }
+    tracked.finish();
+    writer.join();
+    if (write_err) std::rethrow_exception(write_err);
+    reader.join();
+    std::fprintf(stderr, "seq: %zu frames in %.3f s; read %.3f s, track %.3f s, write %.3f s, overlapped\n",
+                 paths.size() - 1, seconds_since(t_all), t_read, t_track, t_write);
     return 0;
 }
 

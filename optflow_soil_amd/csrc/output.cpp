@@ -2,13 +2,19 @@
 // writer of a result folder in the reference's format (.npy maps, timing.json, crop.json). No
 // device work; the Python package and the command-line twin both write through these functions,
 // so a folder is the same bytes whichever of them wrote it.
+#include <algorithm>
+#include <atomic>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
+#include <functional>
 #include <limits>
+#include <mutex>
 #include <string>
 #include <sys/stat.h>
+#include <thread>
 #include <vector>
 
 #include "internal.hpp"
@@ -123,8 +129,12 @@ std::string npy_header(const char *descr, int h, int w) {
     return out;
 }
 
+// Rows of a map written with a crop are formed in a buffer of about this many bytes and written in
+// one call each; a map without a crop is written in one call.
+constexpr size_t ROW_CHUNK = 4u << 20;
+
 // One map of the result (w x h) as the map written (fw x fh): the crop's rectangle holds it, the
-// rest is `fill`; the whole map when there is no crop. Rows are written as they are formed.
+// rest is `fill`; the whole map when there is no crop.
 template <class T>
 void write_map(const std::string &path, const char *descr, const T *data, int w, int h, const ofx_crop *crop,
                int fw, int fh, T fill) {
@@ -132,16 +142,25 @@ void write_map(const std::string &path, const char *descr, const T *data, int w,
     const int ow = crop ? fw : w, oh = crop ? fh : h;
     const std::string hdr = npy_header(descr, oh, ow);
     f.put(hdr.data(), hdr.size());
-    std::vector<T> row((size_t)ow);
-    for (int y = 0; y < oh; y++) {
-        if (!crop) {
-            f.put(data + (size_t)y * w, (size_t)w * sizeof(T));
-            continue;
+    if (!crop) {
+        f.put(data, (size_t)w * h * sizeof(T));
+        return;
+    }
+    const int rows = (int)std::max<size_t>(1, ROW_CHUNK / ((size_t)ow * sizeof(T)));
+    std::vector<T> buf((size_t)std::min(rows, oh) * ow);
+    for (int y0 = 0; y0 < oh; y0 += rows) {
+        const int y1 = std::min(oh, y0 + rows);
+        for (int y = y0; y < y1; y++) {
+            T *row = buf.data() + (size_t)(y - y0) * ow;
+            if (y >= crop->y0 && y < crop->y1) {
+                std::fill(row, row + crop->x0, fill);
+                std::memcpy(row + crop->x0, data + (size_t)(y - crop->y0) * w, (size_t)w * sizeof(T));
+                std::fill(row + crop->x0 + w, row + ow, fill);
+            } else {
+                std::fill(row, row + ow, fill);
+            }
         }
-        std::fill(row.begin(), row.end(), fill);
-        if (y >= crop->y0 && y < crop->y1)
-            std::memcpy(row.data() + crop->x0, data + (size_t)(y - crop->y0) * w, (size_t)w * sizeof(T));
-        f.put(row.data(), row.size() * sizeof(T));
+        f.put(buf.data(), (size_t)(y1 - y0) * ow * sizeof(T));
     }
 }
 
@@ -151,17 +170,50 @@ void write_valid(const std::string &path, const float *ux, int w, int h, const o
     const int ow = crop ? fw : w, oh = crop ? fh : h;
     const std::string hdr = npy_header("|u1", oh, ow);
     f.put(hdr.data(), hdr.size());
-    std::vector<uint8_t> row((size_t)ow);
-    for (int y = 0; y < oh; y++) {
-        std::fill(row.begin(), row.end(), 0);
-        const bool inside = !crop || (y >= crop->y0 && y < crop->y1);
-        if (inside) {
-            const int x0 = crop ? crop->x0 : 0;
-            const float *src = ux + (size_t)(y - (crop ? crop->y0 : 0)) * w;
-            for (int x = 0; x < w; x++) row[x0 + x] = std::isfinite(src[x]) ? 1 : 0;
+    const int rows = (int)std::max<size_t>(1, ROW_CHUNK / (size_t)ow);
+    std::vector<uint8_t> buf((size_t)std::min(rows, oh) * ow);
+    const int x0 = crop ? crop->x0 : 0, ys = crop ? crop->y0 : 0;
+    for (int y0 = 0; y0 < oh; y0 += rows) {
+        const int y1 = std::min(oh, y0 + rows);
+        std::fill(buf.begin(), buf.end(), 0);
+        for (int y = std::max(y0, ys); y < std::min(y1, ys + h); y++) {
+            uint8_t *row = buf.data() + (size_t)(y - y0) * ow + x0;
+            const float *src = ux + (size_t)(y - ys) * w;
+            for (int x = 0; x < w; x++) row[x] = std::isfinite(src[x]) ? 1 : 0;
         }
-        f.put(row.data(), row.size());
+        f.put(buf.data(), (size_t)(y1 - y0) * ow);
     }
+}
+
+// Runs the jobs on up to `threads` host threads (the calling one included): the files of a result
+// folder are independent, and the page cache takes them faster side by side. The first error is
+// rethrown once every job has ended.
+void run_jobs(const std::vector<std::function<void()>> &jobs, unsigned threads) {
+    std::atomic<size_t> next{0};
+    std::mutex m;
+    std::exception_ptr err;
+    auto work = [&] {
+        for (size_t j; (j = next.fetch_add(1)) < jobs.size();) {
+            try {
+                jobs[j]();
+            } catch (...) {
+                std::lock_guard<std::mutex> l(m);
+                if (!err) err = std::current_exception();
+            }
+        }
+    };
+    std::vector<std::thread> pool;
+    const unsigned n = std::min<unsigned>(threads, (unsigned)jobs.size());
+    for (unsigned t = 1; t < n; t++) pool.emplace_back(work);
+    work();
+    for (auto &t : pool) t.join();
+    if (err) std::rethrow_exception(err);
+}
+
+// Host threads for writing one folder: the files (up to 12), capped by the cores present.
+unsigned writer_threads() {
+    const unsigned hc = std::thread::hardware_concurrency();
+    return std::max(1u, std::min(6u, hc ? hc : 1u));
 }
 
 void write_text(const std::string &path, const std::string &text) {
@@ -286,15 +338,21 @@ int ofx_write_result(const char *dir, const ofx_result *res, int w, int h, const
         if (raw && (!res->raw_ux || !res->raw_uy)) throw Error(OFX_EINVAL, "raw maps requested but NULL");
         make_dirs(dir);
         const std::string d(dir);
-        for (const auto &m : maps) {
-            write_map<float>(d + "/" + m.name + ".npy", "<f4", m.data, w, h, crop, frame_w, frame_h, m.fill);
-            if (m.data == res->ux) write_valid(d + "/valid.npy", res->ux, w, h, crop, frame_w, frame_h);
-        }
-        if (res->fb) write_map<int32_t>(d + "/fb.npy", "<i4", res->fb, w, h, crop, frame_w, frame_h, 0);
+        std::vector<std::function<void()>> jobs;
+        for (const auto &m : maps)
+            jobs.push_back([&, m] {
+                write_map<float>(d + "/" + m.name + ".npy", "<f4", m.data, w, h, crop, frame_w, frame_h, m.fill);
+            });
+        jobs.push_back([&] { write_valid(d + "/valid.npy", res->ux, w, h, crop, frame_w, frame_h); });
+        if (res->fb)
+            jobs.push_back([&] { write_map<int32_t>(d + "/fb.npy", "<i4", res->fb, w, h, crop, frame_w, frame_h, 0); });
         if (raw) {
-            write_map<float>(d + "/raw_ux.npy", "<f4", res->raw_ux, w, h, crop, frame_w, frame_h, NAN);
-            write_map<float>(d + "/raw_uy.npy", "<f4", res->raw_uy, w, h, crop, frame_w, frame_h, NAN);
+            jobs.push_back(
+                [&] { write_map<float>(d + "/raw_ux.npy", "<f4", res->raw_ux, w, h, crop, frame_w, frame_h, NAN); });
+            jobs.push_back(
+                [&] { write_map<float>(d + "/raw_uy.npy", "<f4", res->raw_uy, w, h, crop, frame_w, frame_h, NAN); });
         }
+        run_jobs(jobs, writer_threads());
         // timing.json in the reference CLI's layout: its Timing fields in their order, "device"
         // after the first, the sequence fields only for sequences, the hybrid shares only with fb
         std::string t = "{\n  \"coarse\": " + json_number(res->t_coarse) + ",\n  \"device\": \"gpu\",\n  \"fine\": " +

@@ -8,11 +8,14 @@
   that is one-sided because the input ends has its neighbours aligned by the dense coarse model
   first. A bounded queue holds the frames later windows still need. The mean itself is
   ofx_tavg_window (or any function with its signature, for tests).
-- iter_frames: io.py:100-121 on a directory or glob of images.
+- iter_frames: io.py:100-121 on a directory or glob of images. Files are decoded ahead on a worker
+  thread (decoded_ahead) while the caller tracks the current frame.
 """
 import collections
+import concurrent.futures
 import dataclasses
 import glob
+import itertools
 import os
 
 import numpy as np
@@ -30,21 +33,46 @@ def to_gray(pixels, factor=1, engine: Engine = None):
     return (engine or default_engine()).frame_to_gray(pixels, factor)
 
 
-def read_image(path, factor=1, engine: Engine = None):
-    """io.py:33-41: one image file as grey levels (Pillow decodes; 16-bit modes stay 16-bit)."""
+def decode_image(path):
+    """io.py:33-41: the pixels of one image file as an array (Pillow decodes; 16-bit modes stay
+    16-bit). Host work only."""
     if path.lower().endswith((".tif", ".tiff")):
         try:
             import tifffile
         except ImportError:
             raise ValueError(f"reading {path} needs tifffile, which is not installed") from None
-        return to_gray(tifffile.imread(path), factor, engine)
+        return tifffile.imread(path)
     from PIL import Image
     with Image.open(path) as im:
         if im.mode in ("I;16", "I;16B", "I;16L", "I"):
-            return to_gray(np.asarray(im).astype(np.uint16), factor, engine)
+            return np.asarray(im).astype(np.uint16)
         if im.mode not in ("L", "RGB", "RGBA"):
             im = im.convert("RGB")
-        return to_gray(np.asarray(im), factor, engine)
+        return np.asarray(im)
+
+
+def read_image(path, factor=1, engine: Engine = None):
+    """io.py:33-41: one image file as grey levels (decoded on the host, converted on the GPU)."""
+    return to_gray(decode_image(path), factor, engine)
+
+
+def decoded_ahead(paths, decode=decode_image, depth=2):
+    """decode(p) for each path in order, with up to `depth` files decoded ahead on one worker
+    thread, so that reading and decoding the next frames overlaps whatever the caller does with the
+    current one (tracking it on the GPU). A decode error is raised when its frame is reached; the
+    frames before it are yielded first."""
+    it = iter(paths)
+    with concurrent.futures.ThreadPoolExecutor(max_workers=1, thread_name_prefix="ofx-decode") as pool:
+        pending = collections.deque(pool.submit(decode, p) for p in itertools.islice(it, depth))
+        try:
+            while pending:
+                current = pending.popleft()
+                for p in itertools.islice(it, 1):
+                    pending.append(pool.submit(decode, p))
+                yield current.result()
+        finally:
+            for f in pending:
+                f.cancel()
 
 
 def list_images(source):
@@ -131,10 +159,10 @@ def iter_frames(source, start=0, stop=None, step=1, scale=1, tavg=0, align=False
         raise ValueError(f"start must be >= 0, got {start}")
     paths = list_images(source)
     if not tavg:
-        for p in paths[start:stop:step]:
-            yield read_image(p, scale, engine)
+        for px in decoded_ahead(paths[start:stop:step]):
+            yield to_gray(px, scale, engine)
         return
     lo = max(0, start - tavg)
     hi = None if stop is None else stop + tavg
-    raw = (read_image(p, scale, engine) for p in paths[lo:hi])
+    raw = (to_gray(px, scale, engine) for px in decoded_ahead(paths[lo:hi]))
     yield from averaged(raw, start, stop, step, tavg, align, cfg, mask_fn, True, gpu_average(engine))

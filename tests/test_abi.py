@@ -134,6 +134,34 @@ def test_output_folder_of_a_crop_is_placed_in_the_frame(tmp_path):
         io.write_result(str(tmp_path), res, crop=io.CropWindow(40, 8, 61, 21, 5), frame_shape=(H, W))
 
 
+def test_a_large_placed_folder_is_written_in_row_chunks_with_the_same_bytes(tmp_path):
+    """Maps wider and taller than one 4 MiB row chunk (2100 x 1300 frame: 499 rows of f4 per
+    chunk, three chunks, the last partial; valid has one chunk): every file, written by the
+    writer's threads, holds numpy.save's bytes of the placed map."""
+    from optflow_soil_amd import io
+    H, W = 1300, 2100
+    crop = io.CropWindow(37, 511, 37 + 2001, 511 + 700, 9)
+    res = _result(700, 2001, seed=4, fb=True)
+    res.ux[::97, ::13] = np.nan
+    io.write_result(str(tmp_path), res, raw=True, crop=crop, frame_shape=(H, W))
+
+    def placed(a, fill, dtype):
+        full = np.full((H, W), fill, dtype)
+        full[crop.y0:crop.y1, crop.x0:crop.x1] = a
+        return full
+    for name, a, fill in [("ux", res.ux, np.nan), ("uy", res.uy, np.nan), ("conf", res.conf, 0.0),
+                          ("raw_ux", res.raw_ux, np.nan), ("raw_uy", res.raw_uy, np.nan),
+                          ("Exx", res.strain["Exx"], np.nan), ("gmax", res.strain["gmax"], np.nan)]:
+        assert (tmp_path / f"{name}.npy").read_bytes() == _npy_bytes(placed(a, fill, "<f4")), name
+    assert (tmp_path / "fb.npy").read_bytes() == _npy_bytes(placed(res.fb, 0, "<i4"))
+    valid = np.isfinite(placed(res.ux, np.nan, "<f4")).astype(np.uint8)
+    assert (tmp_path / "valid.npy").read_bytes() == _npy_bytes(valid)
+    # and without a crop: each map in one write
+    io.write_result(str(tmp_path / "whole"), res)
+    assert (tmp_path / "whole" / "vol.npy").read_bytes() == _npy_bytes(res.strain["vol"])
+    assert (tmp_path / "whole" / "valid.npy").read_bytes() == _npy_bytes(np.isfinite(res.ux).astype(np.uint8))
+
+
 def test_json_numbers_are_spelled_as_the_reference_cli_spells_them():
     """serde_json's spelling (the shortest digits that read back, ryu's layout): what the
     reference CLI's timing.json holds."""

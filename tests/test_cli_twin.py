@@ -126,3 +126,32 @@ def test_sequence_folders_match_the_binding_and_the_oracle(tmp_path):
         both = np.isfinite(ux) & np.isfinite(o["ux"])
         assert (np.isfinite(ux) == np.isfinite(o["ux"])).mean() >= 0.999
         assert np.sqrt(np.mean((ux[both] - o["ux"][both]) ** 2)) <= 1e-3
+
+
+@pytest.mark.gpu
+def test_sequence_pipeline_ends_on_a_bad_frame_and_on_a_failed_write(tmp_path):
+    """The sequence reads frames on one thread, tracks on another and writes on a third. A frame
+    file that cannot be read ends the run with status 2 after the frames before it are written; a
+    frame folder that cannot be made does the same. Neither leaves a thread waiting."""
+    from optflow_soil_amd import synth
+    frames, _ = synth.workload_c1()
+    d = tmp_path / "frames"
+    d.mkdir()
+    for k, f in enumerate(frames):
+        np.save(d / f"f{k:03d}.npy", f)
+    (d / "f003.npy").write_bytes(b"not an array")
+    out = tmp_path / "out"
+    print("ofx_cli seq with a bad frame 3", flush=True)
+    rc, so, err = run("seq", "--frames", str(d), "--out", str(out))
+    assert rc == 2 and "f003.npy: not a .npy file" in err, err
+    assert (out / "frame_00001" / "ux.npy").exists() and (out / "frame_00002" / "ux.npy").exists()
+    assert not (out / "frame_00003").exists()
+    np.save(d / "f003.npy", frames[3])
+    out2 = tmp_path / "out2"
+    out2.mkdir()
+    (out2 / "frame_00002").write_text("a file where a folder goes")
+    print("ofx_cli seq with an unwritable frame 2", flush=True)
+    rc, so, err = run("seq", "--frames", str(d), "--out", str(out2))
+    assert rc == 2 and "frame_00002" in err, err
+    assert (out2 / "frame_00001" / "ux.npy").exists()
+    assert not (out2 / "frame_00004").exists()
