@@ -97,6 +97,7 @@ struct LineArgs {
 // a / b with the twiddle table tw; returns the buffer holding the result.
 template <int SH>
 struct DirectCore {
+    static constexpr bool IN_PLACE = false;
     const PlanArgs &P;
     const float2 *tw;
     template <class L>
@@ -120,6 +121,7 @@ struct BlueArgs {
 
 template <int SH>
 struct BlueCore {
+    static constexpr bool IN_PLACE = false;
     const BlueArgs &B;
     int n;
     template <class L>
@@ -179,8 +181,13 @@ __device__ __forceinline__ void line_body(const float2 *__restrict__ in, float2 
     }
     float2 *o = out + (size_t)line * n;
     if ((MODE == MODE_INV_FIRST || MODE == MODE_INV_SECOND) && A.prefilter) {
-        float2 *sc = (r == a) ? b : a;
-        prefilter_lines<float2, SH, Sgl2>(r, sc, r, n, 1, MODE == MODE_INV_SECOND ? A.scale : 1.0, tid, nt);
+        const double scale = MODE == MODE_INV_SECOND ? A.scale : 1.0;
+        if constexpr (Core::IN_PLACE) {
+            prefilter_line_inplace<Core::LEN, SH>(r, scale, tid, nt);
+        } else {
+            float2 *sc = (r == a) ? b : a;
+            prefilter_lines<float2, SH, Sgl2>(r, sc, r, n, 1, scale, tid, nt);
+        }
         for (int x = tid; x < n; x += nt) o[x] = MODE == MODE_INV_SECOND ? cconj(r[lp<SH>(x)]) : r[lp<SH>(x)];
     } else if (MODE == MODE_INV_SECOND) {
         const float s = (float)A.scale;
@@ -206,6 +213,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 // The compile-time form of a line length 16 Rs... (lines.hpp fft_line_ct), 256 threads per line.
 template <int N, int SH, int... Rs>
 struct CtCore {
+    static constexpr bool IN_PLACE = false;
     const float2 *T, *tw;
     template <class L>
     __device__ float2 *operator()(L load, float2 *a, float2 *b, int tid, int) const {
@@ -221,6 +229,29 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
     float2 *a = lds, *b = lds + ln;
     line_body<MODE, SH>(in + blockIdx.y * A.in_plane, out + blockIdx.y * A.out_plane, N, bank, k + (int)blockIdx.y, A,
                         a, b, CtCore<N, SH, Rs...>{T, tw_g});
+}
+
+// The compile-time form on one LDS buffer per line (lines.hpp fft_line_ct_inplace): 18 KB of LDS
+// at 2112 instead of 36, so eight lines share a CU instead of four.
+template <int N, int SH, int... Rs>
+struct CtCore1 {
+    static constexpr bool IN_PLACE = true;
+    static constexpr int LEN = N;
+    const float2 *T, *tw;
+    template <class L>
+    __device__ float2 *operator()(L load, float2 *a, float2 *, int tid, int) const {
+        fft_line_ct_inplace<N, SH, 256, Rs...>(load, a, T, tw, tid);
+        return a;
+    }
+};
+
+template <int MODE, int SH, int N, int... Rs>
+__global__ void __launch_bounds__(256) k_lines_ct1(const float2 *__restrict__ in, float2 *__restrict__ out,
+                                                   const float2 *__restrict__ T, const float2 *__restrict__ tw_g,
+                                                   Bank bank, int k, LineArgs A) {
+    extern __shared__ float2 lds[];
+    line_body<MODE, SH>(in + blockIdx.y * A.in_plane, out + blockIdx.y * A.out_plane, N, bank, k + (int)blockIdx.y, A,
+                        lds, nullptr, CtCore1<N, SH, Rs...>{T, tw_g});
 }
 
 // T[k (R - 1) + q - 1] = tw[q k step] for k < p, 0 < q < R: a pass table of the compile-time form
@@ -255,6 +286,7 @@ __global__ void __launch_bounds__(1024) k_lines_blue(const float2 *__restrict__ 
 // The chirp-z form with its length-m transforms in the compile-time form (m = 16 Rs...).
 template <int M, int SH, int... Rs>
 struct BlueCtCore {
+    static constexpr bool IN_PLACE = false;
     const float2 *T, *tw, *ch, *C;
     int n;
     template <class L>
@@ -311,23 +343,31 @@ __global__ void k_chirp_spectrum(float2 *__restrict__ C, int n, int m) {
     C[kk] = make_float2((float)(sr / m), (float)(si / m));
 }
 
-// out[c][r] = in[r][c] for an R x C array, 32 x 32 tiles through LDS; blockIdx.z: the plane of a
-// batch of such arrays stored one after the other.
+// out[c][r] = in[r][c] for an R x C array, 64 x 64 tiles through LDS; blockIdx.z: the plane of a
+// batch of such arrays stored one after the other. A wavefront reads 512 contiguous bytes of a row
+// and writes 512 of a column; each thread issues its 16 loads before the first LDS write, so a CU
+// (four such workgroups, 33 KB of LDS each) keeps about 128 KB of reads in flight.
+constexpr int TR_T = 64;
 __global__ void __launch_bounds__(256) k_transpose(const float2 *__restrict__ in, float2 *__restrict__ out, int R,
                                                    int C) {
-    __shared__ float2 tile[32][33];
-    const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
+    __shared__ float2 tile[TR_T][TR_T + 1];
+    const int c0 = blockIdx.x * TR_T, r0 = blockIdx.y * TR_T;
     in += (size_t)blockIdx.z * R * C;
     out += (size_t)blockIdx.z * R * C;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-    for (int j = ty; j < 32; j += 8) {
-        const int r = r0 + j, c = c0 + tx;
-        if (r < R && c < C) tile[j][tx] = in[(size_t)r * C + c];
+    const int tx = threadIdx.x & (TR_T - 1), ty = threadIdx.x >> 6;  // 64 x 4
+    float2 v[TR_T / 4];
+#pragma unroll
+    for (int k = 0; k < TR_T / 4; k++) {
+        const int r = r0 + ty + 4 * k, c = c0 + tx;
+        v[k] = (r < R && c < C) ? in[(size_t)r * C + c] : make_float2(0.f, 0.f);
     }
+#pragma unroll
+    for (int k = 0; k < TR_T / 4; k++) tile[ty + 4 * k][tx] = v[k];
     __syncthreads();
-    for (int j = ty; j < 32; j += 8) {
-        const int c = c0 + j, r = r0 + tx;
-        if (r < R && c < C) out[(size_t)c * R + r] = tile[tx][j];
+#pragma unroll
+    for (int k = 0; k < TR_T / 4; k++) {
+        const int c = c0 + ty + 4 * k, r = r0 + tx;
+        if (r < R && c < C) out[(size_t)c * R + r] = tile[tx][ty + 4 * k];
     }
 }
 
@@ -486,8 +526,8 @@ void launch_lines(const float2 *in, float2 *out, int n, int lines, int planes, c
         const Bank bk = bank ? *bank : dummy;
 #define OFX_CT_LAUNCH(N_, ...)                                                                                      \
     if (n == N_) {                                                                                                  \
-        hipLaunchKernelGGL((k_lines_ct<MODE, 4, N_, __VA_ARGS__>), dim3(lines, planes), dim3(256), lds, s, in, out, \
-                           P.ct_tw, P.tw, bk, k, A);                                                                \
+        hipLaunchKernelGGL((k_lines_ct1<MODE, 4, N_, __VA_ARGS__>), dim3(lines, planes), dim3(256),                 \
+                           lp_len<4>(N_) * sizeof(float2), s, in, out, P.ct_tw, P.tw, bk, k, A);                    \
         LAUNCHCHECK();                                                                                              \
         return;                                                                                                     \
     }
@@ -512,7 +552,8 @@ void launch_lines(const float2 *in, float2 *out, int n, int lines, int planes, c
 }
 
 void transpose(const float2 *in, float2 *out, int R, int C, int planes, hipStream_t s) {
-    hipLaunchKernelGGL(k_transpose, dim3((C + 31) / 32, (R + 31) / 32, planes), dim3(256), 0, s, in, out, R, C);
+    hipLaunchKernelGGL(k_transpose, dim3((C + TR_T - 1) / TR_T, (R + TR_T - 1) / TR_T, planes), dim3(256), 0, s, in,
+                       out, R, C);
     LAUNCHCHECK();
 }
 }  // namespace

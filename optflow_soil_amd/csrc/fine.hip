@@ -1238,11 +1238,39 @@ __global__ void __launch_bounds__(256) k_select_hist(const double *__restrict__ 
     const int shift = 56 - 8 * p;
     const int hi = shift + 8;  // bits above the current digit must equal the prefix
     unsigned int nan_local = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const double x = v[i];
-        if (p == 0 && isnan(x)) nan_local++;
+    // Counts one key. The first digits of the energies fall into a handful of bins, so a wavefront
+    // whose counted keys share one bin adds them with a single LDS atomic.
+    auto count = [&](double x, bool live) {
+        if (p == 0 && live && isnan(x)) nan_local++;
         const unsigned long long k = order_key(x);
-        if (hi >= 64 || (k >> hi) == (prefix >> hi)) atomicAdd(&lh[(k >> shift) & 255], 1u);
+        const bool in = live && (hi >= 64 || (k >> hi) == (prefix >> hi));
+        const unsigned int bin = (unsigned int)(k >> shift) & 255u;
+        const unsigned long long act = __ballot(in);
+        if (!act) return;
+        const unsigned int lead = (unsigned int)__builtin_amdgcn_readlane((int)bin, __builtin_ctzll(act));
+        const unsigned long long same = __ballot(in && bin == lead);
+        if (same == act) {
+            if ((threadIdx.x & 63) == (unsigned int)__builtin_ctzll(act)) atomicAdd(&lh[lead], (unsigned int)__popcll(act));
+        } else if (in) {
+            atomicAdd(&lh[bin], 1u);
+        }
+    };
+    // four loads in flight per thread, then the remainder one at a time; every lane of a wavefront
+    // runs the same trip counts, so the ballots see the whole wavefront
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    const size_t first = (size_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u), lane = threadIdx.x & 63;
+    size_t i = first;
+    for (; i + 3 * stride + 63 < n; i += 4 * stride) {
+        const double x0 = v[i + lane], x1 = v[i + stride + lane], x2 = v[i + 2 * stride + lane],
+                     x3 = v[i + 3 * stride + lane];
+        count(x0, true);
+        count(x1, true);
+        count(x2, true);
+        count(x3, true);
+    }
+    for (; i < n; i += stride) {
+        const bool live = i + lane < n;
+        count(live ? v[i + lane] : 0.0, live);
     }
     if (nan_local) atomicAdd(&st->nan_count, nan_local);
     __syncthreads();
@@ -1261,13 +1289,24 @@ __global__ void __launch_bounds__(256) k_select_below(const double *__restrict__
     if (gated && !st->undecided) return;
     const unsigned long long key = select_key(st);
     unsigned long long cnt = 0, mx = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const unsigned long long k = order_key(v[i]);
+    auto below = [&](double x) {
+        const unsigned long long k = order_key(x);
         if (k < key) {
             cnt++;
             mx = k > mx ? k : mx;
         }
+    };
+    // four loads in flight per thread, then the remainder one at a time
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; i + 3 * stride < n; i += 4 * stride) {
+        const double x0 = v[i], x1 = v[i + stride], x2 = v[i + 2 * stride], x3 = v[i + 3 * stride];
+        below(x0);
+        below(x1);
+        below(x2);
+        below(x3);
     }
+    for (; i < n; i += stride) below(v[i]);
     // wavefront reduction, then one atomic per workgroup
     __shared__ unsigned long long rc[4], rm[4];
     for (int off = 32; off > 0; off >>= 1) {
@@ -1396,6 +1435,8 @@ void moments_solve(const double *sums, double *ym, const double *ux_in, const do
         for (int p = 0; p < 3; p++)
             for (int j = 0; j < klen; j++) kt_arg.k[p][j] = kt_host[p * klen + j];
         if (r == 16) {
+            // one launch over the 7 sum planes: launches split by power count (164 and 100 VGPRs for
+            // the two- and one-power planes) took 158 against 145 us at 2048^2
             hipLaunchKernelGGL(k_moments_y_rb<16>, dim3((w + 255) / 256, (h + RB_ROWS - 1) / RB_ROWS, 7), dim3(256), 0,
                                s, sums, ym, w, h, kt_arg);
             hipLaunchKernelGGL(k_moments_x_rb<16>, dim3((w + XB_P - 1) / XB_P, h), dim3(256), 0, s, ym, ux_in, uy_in,

@@ -472,6 +472,77 @@ __device__ __forceinline__ float2 *fft_line_ct(L load, float2 *a, float2 *b, con
     return ct_passes<N, 16, SH, NT, Rs...>(a, b, T, tw_line, tid);
 }
 
+// The same passes on one LDS buffer per line: a pass reads every butterfly of the thread into
+// registers, waits for the workgroup, then writes its results over the buffer. Two barriers per
+// pass instead of one, but half the LDS, so twice the lines share a CU.
+template <int R, int N, int P, int SH, int NT>
+__device__ __forceinline__ void ct_pass_inplace(float2 *buf, const float2 *__restrict__ T,
+                                                const float2 *__restrict__ tw_line, int tid) {
+    constexpr int M = N / R, IT = (M + NT - 1) / NT;
+    static_assert(SH == 4 && M % 16 == 0 && P % 16 == 0, "strides of whole 16-element groups");
+    constexpr int SM = M + M / 16, SP = P + P / 16;
+    float2 wr[R];
+    if constexpr (prime_radix<R>()) {
+#pragma unroll
+        for (int q = 0; q < R; q++) wr[q] = tw_line[q * M];
+    }
+    float2 x[IT][R];
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int i = it * NT + tid;
+        if (M % NT != 0 && i >= M) break;
+        const int k = i % P;
+        const float2 *sp = buf + lp<SH>(i);
+#pragma unroll
+        for (int q = 0; q < R; q++) x[it][q] = sp[q * SM];
+        const float2 *t = T + k * (R - 1);
+#pragma unroll
+        for (int q = 1; q < R; q++) x[it][q] = cmul(x[it][q], t[q - 1]);
+        if constexpr (prime_radix<R>()) dft_prime<R>(x[it], wr);
+        else dft_small<R>(x[it]);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < IT; it++) {
+        const int i = it * NT + tid;
+        if (M % NT != 0 && i >= M) break;
+        const int k = i % P;
+        float2 *dp = buf + lp<SH>((i / P) * P * R + k);
+#pragma unroll
+        for (int q = 0; q < R; q++) dp[q * SP] = x[it][q];
+    }
+    __syncthreads();
+}
+
+template <int N, int P, int SH, int NT, int R, int... Rest>
+__device__ __forceinline__ void ct_passes_inplace(float2 *buf, const float2 *__restrict__ T,
+                                                  const float2 *__restrict__ tw_line, int tid) {
+    ct_pass_inplace<R, N, P, SH, NT>(buf, T, tw_line, tid);
+    if constexpr (sizeof...(Rest) > 0) ct_passes_inplace<N, P * R, SH, NT, Rest...>(buf, T + P * (R - 1), tw_line, tid);
+}
+
+// fft_line_ct on the one buffer `a`; the result is left in `a`. Ends with a barrier.
+template <int N, int SH, int NT, int... Rs, class L>
+__device__ __forceinline__ void fft_line_ct_inplace(L load, float2 *a, const float2 *__restrict__ T,
+                                                    const float2 *__restrict__ tw_line, int tid) {
+    constexpr int M0 = N / 16;
+    static_assert(N == 16 * radix_product<Rs...>(), "N = 16 Rs...");
+#pragma unroll
+    for (int i0 = 0; i0 < M0; i0 += NT) {
+        const int i = i0 + tid;
+        if (M0 % NT != 0 && i >= M0) break;
+        float2 x[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) x[q] = load(i + q * M0);
+        dft_small<16>(x);
+        float2 *dp = a + 17 * i;  // lp(16 i + q) = 17 i + q
+#pragma unroll
+        for (int q = 0; q < 16; q++) dp[q] = x[q];
+    }
+    __syncthreads();
+    ct_passes_inplace<N, 16, SH, NT, Rs...>(a, T, tw_line, tid);
+}
+
 // Stockham pass with compile-time radix R, line length N and p = P over `lines` lines (line stride N):
 // every index operation is a shift or a mask when N and P are powers of two.
 template <int R, int N, int P>
@@ -652,6 +723,72 @@ __device__ void prefilter_lines(const T *src, T *mid, T *dst, int n, int lines, 
         }
     }
     __syncthreads();
+}
+
+// prefilter_lines for one complex line of compile-time length N held in `buf`, in place, with the
+// float accumulator (Sgl2). Each pass first runs every segment's warm-up (which reads the samples of
+// neighbouring segments) and keeps only its running value; after a barrier each thread walks its own
+// segment, reading each sample just before it overwrites it, so no second buffer is needed. The
+// arithmetic is that of prefilter_lines<float2, SH, Sgl2>, so the coefficients are the same bits.
+template <int N, int SH>
+__device__ void prefilter_line_inplace(float2 *buf, double scale, int tid, int nt) {
+    const double z = -0.26794919243112270647;  // sqrt(3) - 2
+    constexpr int NSEG = (N + PF_SEG - 1) / PF_SEG;
+    static_assert(N > 1, "a line of one sample has nothing to filter");
+    using A = Sgl2;
+    auto at = [&](int i) { return buf[lp<SH>(i)]; };
+    const int j = tid;
+    const bool mine = j < NSEG;
+    const int a = j * PF_SEG, b = min(a + PF_SEG, N);
+    // causal pass: mid[i] = 6 src[i] + z mid[i-1], mid[0] = sum_{k<h} z^k 6 src[k]
+    A acc;
+    if (mine) {
+        if (a <= PF_WARM) {
+            constexpr int H = N < PF_HORIZON ? N : PF_HORIZON;
+            double zk = 1.0;
+            for (int k = 0; k < H; k++) {
+                acc = acc + A(at(k)) * (6.0 * zk);
+                zk *= z;
+            }
+            for (int i = 1; i < a; i++) acc = A(at(i)) * 6.0 + acc * z;
+        } else {
+#pragma unroll 4
+            for (int i = a - PF_WARM; i < a; i++) acc = A(at(i)) * 6.0 + acc * z;
+        }
+    }
+    __syncthreads();
+    if (mine) {
+        if (a == 0) buf[lp<SH>(0)] = acc.out(1.0);
+        for (int i = max(a, 1); i < b; i++) {
+            acc = A(at(i)) * 6.0 + acc * z;
+            buf[lp<SH>(i)] = acc.out(1.0);
+        }
+    }
+    __syncthreads();
+    // anti-causal pass: dst[n-1] = z / (z^2 - 1) (mid[n-1] + z mid[n-2]), dst[i] = z (dst[i+1] - mid[i])
+    const double zend = z / (z * z - 1.0);
+    A bcc;
+    if (mine) {
+        int i;
+        if (b + PF_WARM >= N) {
+            bcc = (A(at(N - 1)) + A(at(N - 2)) * z) * zend;
+            i = N - 2;
+        } else {
+            i = b + PF_WARM - 1;
+        }
+#pragma unroll 4
+        for (; i >= b; i--) bcc = (bcc - A(at(i))) * z;
+    }
+    __syncthreads();
+    if (mine) {
+        if (b == N) buf[lp<SH>(N - 1)] = bcc.out(scale);
+        for (int i = min(b - 1, N - 2); i >= a; i--) {
+            bcc = (bcc - A(at(i))) * z;
+            buf[lp<SH>(i)] = bcc.out(scale);
+        }
+    }
+    __syncthreads();
+    (void)nt;
 }
 
 }  // namespace ofx
