@@ -183,7 +183,7 @@ __device__ __forceinline__ void line_body(const float2 *__restrict__ in, float2 
     if ((MODE == MODE_INV_FIRST || MODE == MODE_INV_SECOND) && A.prefilter) {
         const double scale = MODE == MODE_INV_SECOND ? A.scale : 1.0;
         if constexpr (Core::IN_PLACE) {
-            prefilter_line_inplace<Core::LEN, SH>(r, scale, tid, nt);
+            prefilter_line_inplace<SH>(r, n, scale, tid, nt);
         } else {
             float2 *sc = (r == a) ? b : a;
             prefilter_lines<float2, SH, Sgl2>(r, sc, r, n, 1, scale, tid, nt);
@@ -236,11 +236,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) k
 template <int N, int SH, int... Rs>
 struct CtCore1 {
     static constexpr bool IN_PLACE = true;
-    static constexpr int LEN = N;
     const float2 *T, *tw;
     template <class L>
     __device__ float2 *operator()(L load, float2 *a, float2 *, int tid, int) const {
-        fft_line_ct_inplace<N, SH, 256, Rs...>(load, a, T, tw, tid);
+        fft_line_ct_inplace<N, SH, 256, false, Rs...>(load, a, T, tw, tid);
         return a;
     }
 };
@@ -314,6 +313,37 @@ __global__ void __launch_bounds__(1024) k_lines_blue_ct(const float2 *__restrict
     constexpr int lm = lp_len<SH>(M);
     line_body<MODE, SH>(in + blockIdx.y * A.in_plane, out + blockIdx.y * A.out_plane, n, bank, k + (int)blockIdx.y, A,
                         lds, lds + lm, BlueCtCore<M, SH, Rs...>{T, tw, ch, C, n});
+}
+
+// The chirp-z form on one LDS buffer of length m (lines.hpp fft_line_ct_inplace): 70 KB at 8192
+// instead of 139, so two 1024-thread workgroups share a CU instead of one.
+template <int M, int SH, int... Rs>
+struct BlueCtCore1 {
+    static constexpr bool IN_PLACE = true;
+    const float2 *T, *tw, *ch, *C;
+    int n;
+    template <class L>
+    __device__ float2 *operator()(L load, float2 *a, float2 *, int tid, int) const {
+        const int n_ = n;
+        const float2 *ch_ = ch, *C_ = C;
+        fft_line_ct_inplace<M, SH, 1024, false, Rs...>(
+            [&](int e) { return e < n_ ? cmul(load(e), cconj(ch_[e])) : make_float2(0.f, 0.f); }, a, T, tw, tid);
+        fft_line_ct_inplace<M, SH, 1024, true, Rs...>([&](int e) { return cconj(cmul(a[lp<SH>(e)], C_[e])); }, a, T,
+                                                      tw, tid);
+        for (int k = tid; k < n_; k += 1024) a[lp<SH>(k)] = cconj(cmul(ch_[k], a[lp<SH>(k)]));
+        __syncthreads();
+        return a;
+    }
+};
+
+template <int MODE, int SH, int M, int... Rs>
+__global__ void __launch_bounds__(1024) k_lines_blue_ct1(const float2 *__restrict__ in, float2 *__restrict__ out,
+                                                         int n, const float2 *__restrict__ T,
+                                                         const float2 *__restrict__ tw, const float2 *__restrict__ ch,
+                                                         const float2 *__restrict__ C, Bank bank, int k, LineArgs A) {
+    extern __shared__ float2 lds[];
+    line_body<MODE, SH>(in + blockIdx.y * A.in_plane, out + blockIdx.y * A.out_plane, n, bank, k + (int)blockIdx.y, A,
+                        lds, nullptr, BlueCtCore1<M, SH, Rs...>{T, tw, ch, C, n});
 }
 
 // b_j = exp(i pi (j^2 mod 2n) / n) from the double angle (the product j^2 is exact in 64 bits)
@@ -494,15 +524,16 @@ void launch_lines(const float2 *in, float2 *out, int n, int lines, int planes, c
         const size_t lds = 2 * (size_t)lp_len<4>(P.blue_m) * sizeof(float2);
         if (M.ct_tw) {
             const Bank bk = bank ? *bank : dummy;
+            const size_t lds1 = (size_t)lp_len<4>(P.blue_m) * sizeof(float2);
 #define OFX_BLUE_CT_LAUNCH(M_, ...)                                                                                 \
     if (P.blue_m == M_) {                                                                                           \
         static std::once_flag once_ct;                                                                              \
         std::call_once(once_ct, [] {                                                                                \
-            (void)hipFuncSetAttribute((const void *)k_lines_blue_ct<MODE, 4, M_, __VA_ARGS__>,                      \
+            (void)hipFuncSetAttribute((const void *)k_lines_blue_ct1<MODE, 4, M_, __VA_ARGS__>,                     \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);                      \
         });                                                                                                         \
-        hipLaunchKernelGGL((k_lines_blue_ct<MODE, 4, M_, __VA_ARGS__>), dim3(lines, planes), dim3(1024), lds, s, in, \
-                           out, n, M.ct_tw, M.tw, P.chirp, P.chirp_spec, bk, k, A);                                  \
+        hipLaunchKernelGGL((k_lines_blue_ct1<MODE, 4, M_, __VA_ARGS__>), dim3(lines, planes), dim3(1024), lds1, s,   \
+                           in, out, n, M.ct_tw, M.tw, P.chirp, P.chirp_spec, bk, k, A);                              \
         LAUNCHCHECK();                                                                                              \
         return;                                                                                                     \
     }

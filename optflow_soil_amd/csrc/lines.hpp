@@ -521,23 +521,30 @@ __device__ __forceinline__ void ct_passes_inplace(float2 *buf, const float2 *__r
     if constexpr (sizeof...(Rest) > 0) ct_passes_inplace<N, P * R, SH, NT, Rest...>(buf, T + P * (R - 1), tw_line, tid);
 }
 
-// fft_line_ct on the one buffer `a`; the result is left in `a`. Ends with a barrier.
-template <int N, int SH, int NT, int... Rs, class L>
+// fft_line_ct on the one buffer `a`; the result is left in `a`. Ends with a barrier. FROM_LDS: the
+// loader reads `a` itself, so every thread's loads finish before the first pass writes.
+template <int N, int SH, int NT, bool FROM_LDS, int... Rs, class L>
 __device__ __forceinline__ void fft_line_ct_inplace(L load, float2 *a, const float2 *__restrict__ T,
                                                     const float2 *__restrict__ tw_line, int tid) {
     constexpr int M0 = N / 16;
     static_assert(N == 16 * radix_product<Rs...>(), "N = 16 Rs...");
+    static_assert(!FROM_LDS || M0 <= NT, "a loader reading the buffer needs one butterfly per thread");
 #pragma unroll
     for (int i0 = 0; i0 < M0; i0 += NT) {
         const int i = i0 + tid;
-        if (M0 % NT != 0 && i >= M0) break;
+        const bool on = M0 % NT == 0 || i < M0;
         float2 x[16];
+        if (on) {
 #pragma unroll
-        for (int q = 0; q < 16; q++) x[q] = load(i + q * M0);
-        dft_small<16>(x);
-        float2 *dp = a + 17 * i;  // lp(16 i + q) = 17 i + q
+            for (int q = 0; q < 16; q++) x[q] = load(i + q * M0);
+            dft_small<16>(x);
+        }
+        if constexpr (FROM_LDS) __syncthreads();
+        if (on) {
+            float2 *dp = a + 17 * i;  // lp(16 i + q) = 17 i + q
 #pragma unroll
-        for (int q = 0; q < 16; q++) dp[q] = x[q];
+            for (int q = 0; q < 16; q++) dp[q] = x[q];
+        }
     }
     __syncthreads();
     ct_passes_inplace<N, 16, SH, NT, Rs...>(a, T, tw_line, tid);
@@ -730,11 +737,11 @@ __device__ void prefilter_lines(const T *src, T *mid, T *dst, int n, int lines, 
 // neighbouring segments) and keeps only its running value; after a barrier each thread walks its own
 // segment, reading each sample just before it overwrites it, so no second buffer is needed. The
 // arithmetic is that of prefilter_lines<float2, SH, Sgl2>, so the coefficients are the same bits.
-template <int N, int SH>
-__device__ void prefilter_line_inplace(float2 *buf, double scale, int tid, int nt) {
+// N > 1 samples; the N / 17 segments need as many threads.
+template <int SH>
+__device__ void prefilter_line_inplace(float2 *buf, int N, double scale, int tid, int nt) {
     const double z = -0.26794919243112270647;  // sqrt(3) - 2
-    constexpr int NSEG = (N + PF_SEG - 1) / PF_SEG;
-    static_assert(N > 1, "a line of one sample has nothing to filter");
+    const int NSEG = (N + PF_SEG - 1) / PF_SEG;
     using A = Sgl2;
     auto at = [&](int i) { return buf[lp<SH>(i)]; };
     const int j = tid;
@@ -744,7 +751,7 @@ __device__ void prefilter_line_inplace(float2 *buf, double scale, int tid, int n
     A acc;
     if (mine) {
         if (a <= PF_WARM) {
-            constexpr int H = N < PF_HORIZON ? N : PF_HORIZON;
+            const int H = N < PF_HORIZON ? N : PF_HORIZON;
             double zk = 1.0;
             for (int k = 0; k < H; k++) {
                 acc = acc + A(at(k)) * (6.0 * zk);
