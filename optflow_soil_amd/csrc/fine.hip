@@ -204,7 +204,9 @@ __device__ __forceinline__ void xcd_tile(int L, int gx, int &tx, int &ty) {
     tx = x * c + (s - ty * c);
 }
 
-__global__ void __launch_bounds__(256) k_accumulate_tiled(const double *__restrict__ ux, const double *__restrict__ uy,
+// Six workgroups per CU: the two staged boxes (25.6 KB of LDS) allow six, and 80 VGPRs (81 without
+// the hint) let six wavefronts share each SIMD.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_accumulate_tiled(const double *__restrict__ ux, const double *__restrict__ uy,
                                                           const float *__restrict__ sx, const float *__restrict__ sy,
                                                           const float *__restrict__ csx, const float *__restrict__ csy,
                                                           const float2 *__restrict__ resp,
@@ -805,7 +807,8 @@ template <int R>
 struct XbLayout {
     static constexpr int YQ = (XB_P + 2 * R + 3) / 4;
     static constexpr int YS = 4 * YQ;
-    static constexpr int WORDS = (15 * YS > 26 * XB_P) ? 15 * YS : 26 * XB_P;
+    // the moments go to LDS in two rounds over the staging area: 19, then 7
+    static constexpr int WORDS = (15 * YS > 19 * XB_P) ? 15 * YS : 19 * XB_P;
 };
 
 template <int R, int NP, int ROW>
@@ -914,39 +917,50 @@ __global__ void __launch_bounds__(256) k_moments_x_rb(const double *__restrict__
     x_plane<R, 2, 3>(lds + ((P2 >> sh) & 255) * L::YS, lane, kt, res);
     x_plane<R, 1, 5>(lds + ((P1 >> sh) & 255) * L::YS, lane, kt, res);
     x_plane<R, 1, 6>(lds + ((Q1 >> sh) & 255) * L::YS, lane, kt, res);
+    // The 26 moments reach the solving threads in two rounds through the staging area: first the
+    // 3- and 2-power jobs of every wavefront (19 moments), then the 1-power jobs (7). That takes
+    // 38.9 KB of LDS instead of 53 for all 26 at once: four workgroups per CU instead of three.
+    // Slots of the first round, by moment: 0-4 -> 0-4, 6-10 -> 5-9, 12-16 -> 10-14, 18-19 -> 15-16,
+    // 21-22 -> 17-18; of the second: 5, 11, 17, 20, 23, 24, 25 -> 0-6.
+    constexpr unsigned S3 = 0x030a0500u, S2 = 0x110f0d08u;  // first round: P3 and P2 jobs
+    constexpr unsigned S1 = 0x03020100u, SQ = 0xff060504u;  // second round: P1 and Q1 jobs
     __syncthreads();
     double *mom = lds;
-    if (wave < 3)
-        x_put<3, 0>(mom, (B3 >> sh) & 255, lane, res);
-    else
-        x_put<2, 0>(mom, (B3 >> sh) & 255, lane, res);
-    x_put<2, 3>(mom, (B2 >> sh) & 255, lane, res);
-    x_put<1, 5>(mom, (B1 >> sh) & 255, lane, res);
-    if (wave < 3) x_put<1, 6>(mom, (C1 >> sh) & 255, lane, res);
-    __syncthreads();
-    const int x = x0 + 4 * lane + wave;
-    if (x >= w) return;
     const double *mm = mom + wave * 64 + lane;
+    if (wave < 3)
+        x_put<3, 0>(mom, (S3 >> sh) & 255, lane, res);
+    else
+        x_put<2, 0>(mom, (S3 >> sh) & 255, lane, res);
+    x_put<2, 3>(mom, (S2 >> sh) & 255, lane, res);
+    __syncthreads();
     double mp[3][3][3], mr[2][2][2];
 #pragma unroll
     for (int p = 0; p < 3; p++) {
-        mp[p][0][0] = mm[(6 * p + 0) * XB_P];
-        mp[p][1][0] = mm[(6 * p + 1) * XB_P];
-        mp[p][2][0] = mm[(6 * p + 2) * XB_P];
-        mp[p][0][1] = mm[(6 * p + 3) * XB_P];
-        mp[p][1][1] = mm[(6 * p + 4) * XB_P];
-        mp[p][0][2] = mm[(6 * p + 5) * XB_P];
+        mp[p][0][0] = mm[(5 * p + 0) * XB_P];
+        mp[p][1][0] = mm[(5 * p + 1) * XB_P];
+        mp[p][2][0] = mm[(5 * p + 2) * XB_P];
+        mp[p][0][1] = mm[(5 * p + 3) * XB_P];
+        mp[p][1][1] = mm[(5 * p + 4) * XB_P];
         mp[p][1][2] = mp[p][2][1] = mp[p][2][2] = 0;
     }
 #pragma unroll
     for (int p = 0; p < 2; p++) {
-        mr[p][0][0] = mm[(18 + 3 * p + 0) * XB_P];
-        mr[p][1][0] = mm[(18 + 3 * p + 1) * XB_P];
-        mr[p][0][1] = mm[(18 + 3 * p + 2) * XB_P];
+        mr[p][0][0] = mm[(15 + 2 * p) * XB_P];
+        mr[p][1][0] = mm[(16 + 2 * p) * XB_P];
         mr[p][1][1] = 0;
     }
-    solve_pixel(mp, mr, mm[24 * XB_P], mm[25 * XB_P], (size_t)y * w + x, ux_in, uy_in, ux_out, uy_out, energy, resid,
-                slope);
+    __syncthreads();
+    x_put<1, 5>(mom, (S1 >> sh) & 255, lane, res);
+    if (wave < 3) x_put<1, 6>(mom, (SQ >> sh) & 255, lane, res);
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < 3; p++) mp[p][0][2] = mm[p * XB_P];
+    mr[0][0][1] = mm[3 * XB_P];
+    mr[1][0][1] = mm[4 * XB_P];
+    const double mcc = mm[5 * XB_P], mw = mm[6 * XB_P];
+    const int x = x0 + 4 * lane + wave;
+    if (x >= w) return;
+    solve_pixel(mp, mr, mcc, mw, (size_t)y * w + x, ux_in, uy_in, ux_out, uy_out, energy, resid, slope);
 }
 
 // Energy threshold (phaseflow.py:149): pixels with energy <= 1e-9 median + 1e-30 (or NaN) keep the
