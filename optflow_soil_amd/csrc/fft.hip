@@ -253,6 +253,52 @@ __global__ void __launch_bounds__(256) k_lines_ct1(const float2 *__restrict__ in
                         lds, nullptr, CtCore1<N, SH, Rs...>{T, tw_g});
 }
 
+// The second pass of an inverse straight from the first pass's layout [kx][y] (rows of `rows`
+// samples), with no transpose between the passes. A workgroup of 4 x 256 threads takes the lines
+// y0 .. y0 + 3: it loads them as 32-byte pieces of the rows (the 4 samples y0 .. y0 + 3 of one kx;
+// a wavefront reads 16 such pieces), transposing through LDS, then each 256 threads transform one
+// line in place. The 4 groups whose pieces share 128-byte rows segments (16 lines) run on one XCD
+// one after another: slot s holds group (s mod 8) (slots / 8) + s / 8 (workgroups are dealt to the
+// 8 XCDs in turn), so their reads meet in that XCD's L2.
+constexpr int TL = 4;
+template <int SH, int N, int... Rs>
+__global__ void __launch_bounds__(1024) k_lines_ct1_t(const float2 *__restrict__ in, float2 *__restrict__ out,
+                                                      const float2 *__restrict__ T, const float2 *__restrict__ tw_g,
+                                                      int rows, int groups, LineArgs A) {
+    extern __shared__ float2 lds[];
+    const int slot = blockIdx.x;
+    const int g = (slot & 7) * ((int)gridDim.x >> 3) + (slot >> 3);
+    if (g >= groups) return;  // a padding slot: the whole workgroup leaves before any barrier
+    constexpr int LN = lp_len<SH>(N), LOADS = (N * TL + 1023) / 1024;
+    const int y0 = g * TL, j = threadIdx.x >> 8, tid = threadIdx.x & 255;
+    in += blockIdx.y * A.in_plane;
+    out += blockIdx.y * A.out_plane;
+    float2 v[LOADS];
+#pragma unroll
+    for (int t = 0; t < LOADS; t++) {
+        const int e = t * 1024 + threadIdx.x, kx = e / TL, y = y0 + e % TL;
+        v[t] = (e < N * TL && y < rows) ? in[(size_t)kx * rows + y] : make_float2(0.f, 0.f);
+    }
+#pragma unroll
+    for (int t = 0; t < LOADS; t++) {
+        const int e = t * 1024 + threadIdx.x;
+        if (e < N * TL) lds[(e % TL) * LN + lp<SH>(e / TL)] = v[t];
+    }
+    __syncthreads();
+    float2 *buf = lds + j * LN;
+    fft_line_ct_inplace<N, SH, 256, true, Rs...>([&](int x) { return buf[lp<SH>(x)]; }, buf, T, tw_g, tid);
+    const int y = y0 + j;
+    if (A.prefilter) prefilter_line_inplace<SH>(buf, N, A.scale, tid, 256);
+    if (y >= rows) return;
+    float2 *o = out + (size_t)y * N;
+    if (A.prefilter) {
+        for (int x = tid; x < N; x += 256) o[x] = cconj(buf[lp<SH>(x)]);
+    } else {
+        const float sc = (float)A.scale;
+        for (int x = tid; x < N; x += 256) o[x] = cconj(cscale(buf[lp<SH>(x)], sc));
+    }
+}
+
 // T[k (R - 1) + q - 1] = tw[q k step] for k < p, 0 < q < R: a pass table of the compile-time form
 __global__ void k_ct_twiddles(float2 *__restrict__ T, const float2 *__restrict__ tw, int p, int R, int step) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -589,6 +635,32 @@ void transpose(const float2 *in, float2 *out, int R, int C, int planes, hipStrea
 }
 }  // namespace
 
+namespace {
+// The inverse's second pass (length n = wp over hp lines) from the [kx][y] planes with
+// k_lines_ct1_t, when n has a compile-time form; false otherwise (the caller transposes).
+bool second_pass_from_columns(const float2 *in, float2 *out, int n, int rows, int planes, const LineArgs &A,
+                              hipStream_t s) {
+    const FftPlan &P = fft_plan(n, s);
+    if (P.blue_m || !P.ct_tw) return false;
+    const int groups = (rows + TL - 1) / TL, slots = 8 * ((groups + 7) / 8);
+#define OFX_CT_T_LAUNCH(N_, ...)                                                                                    \
+    if (n == N_) {                                                                                                  \
+        static std::once_flag once_t;                                                                               \
+        std::call_once(once_t, [] {                                                                                 \
+            (void)hipFuncSetAttribute((const void *)k_lines_ct1_t<4, N_, __VA_ARGS__>,                              \
+                                      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);                      \
+        });                                                                                                         \
+        hipLaunchKernelGGL((k_lines_ct1_t<4, N_, __VA_ARGS__>), dim3(slots, planes), dim3(1024),                    \
+                           TL * lp_len<4>(N_) * sizeof(float2), s, in, out, P.ct_tw, P.tw, rows, groups, A);        \
+        LAUNCHCHECK();                                                                                              \
+        return true;                                                                                                \
+    }
+    OFX_CT_T_LAUNCH(2112, 12, 11)
+#undef OFX_CT_T_LAUNCH
+    return false;
+}
+}  // namespace
+
 void fft_forward_padded(const float *img, int w, int h, float2 *spec, float2 *tmp, float2 *tmp2, hipStream_t s) {
     const int hp = h + 2 * PAD, wp = w + 2 * PAD;
     LineArgs A{};
@@ -611,8 +683,9 @@ void fft_inverse_filtered(const float2 *spec, int hp, int wp, const Bank &bank, 
     A.in_plane = 0;  // every filter reads the one spectrum
     A.out_plane = np;
     launch_lines<MODE_INV_FIRST>(spec, tmp, hp, wp, K, &bank, k0, A, s);  // [kx][y] per filter
-    transpose(tmp, tmp2, wp, hp, K, s);                                  // [y][kx]
     A.in_plane = np;
+    if (second_pass_from_columns(tmp, out, wp, hp, K, A, s)) return;
+    transpose(tmp, tmp2, wp, hp, K, s);  // [y][kx]
     launch_lines<MODE_INV_SECOND>(tmp2, out, wp, hp, K, nullptr, 0, A, s);
 }
 
