@@ -543,7 +543,8 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
     const size_t n2 = (size_t)w2 * h2;
     float *c2 = B<float>(ws.bc2, n2);
     dcopy(c2, i2, n2 * sizeof(float), s);
-    prefilter_real(c2, B<float>(ws.bpftmp, std::max(n, n2)), w2, h2, s);
+    // sized for the two pre-warp planes of the outer iterations too (prefilter_real2)
+    prefilter_real(c2, B<float>(ws.bpftmp, std::max(2 * n, n2)), w2, h2, s);
     double *ux = B<double>(ws.bux, n), *uy = B<double>(ws.buy, n);
     double *nux = B<double>(ws.bnux, n), *nuy = B<double>(ws.bnuy, n);
     float *sx = B<float>(ws.bsx, n), *sy = B<float>(ws.bsy, n), *csx = B<float>(ws.bcsx, n), *csy = B<float>(ws.bcsy, n);
@@ -597,10 +598,7 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
         fft_forward_padded(warped, w, h, spec, tmp, tmp2, s);
         fft_inverse_filtered(spec, hp, wp, bank, 0, F, MUL_G, true, tmp, tmp2, resp, s);
         ws.prof.mark(ST_SMOOTH, s);
-        dcopy(csx, sx, n * sizeof(float), s);
-        dcopy(csy, sy, n * sizeof(float), s);
-        prefilter_real(csx, B<float>(ws.bpftmp, n), w, h, s);
-        prefilter_real(csy, B<float>(ws.bpftmp, n), w, h, s);
+        prefilter_real2(sx, sy, csx, csy, B<float>(ws.bpftmp, 2 * n), w, h, s);
         for (int in = 0; in < c.inner_iters; in++) {
             ws.prof.mark(ST_ACCUMULATE, s);
             accumulate(ux, uy, sx, sy, csx, csy, resp, refp, sums, w, h, F, s);

@@ -260,8 +260,9 @@ __global__ void __launch_bounds__(256) k_lines_ct1(const float2 *__restrict__ in
 // line in place. The 4 groups whose pieces share 128-byte rows segments (16 lines) run on one XCD
 // one after another: slot s holds group (s mod 8) (slots / 8) + s / 8 (workgroups are dealt to the
 // 8 XCDs in turn), so their reads meet in that XCD's L2.
+// MODE_PLAIN runs the forward transform's second pass the same way (lines kx from [y][kx]).
 constexpr int TL = 4;
-template <int SH, int N, int... Rs>
+template <int MODE, int SH, int N, int... Rs>
 __global__ void __launch_bounds__(1024) k_lines_ct1_t(const float2 *__restrict__ in, float2 *__restrict__ out,
                                                       const float2 *__restrict__ T, const float2 *__restrict__ tw_g,
                                                       int rows, int groups, LineArgs A) {
@@ -288,10 +289,12 @@ __global__ void __launch_bounds__(1024) k_lines_ct1_t(const float2 *__restrict__
     float2 *buf = lds + j * LN;
     fft_line_ct_inplace<N, SH, 256, true, Rs...>([&](int x) { return buf[lp<SH>(x)]; }, buf, T, tw_g, tid);
     const int y = y0 + j;
-    if (A.prefilter) prefilter_line_inplace<SH>(buf, N, A.scale, tid, 256);
+    if (MODE == MODE_INV_SECOND && A.prefilter) prefilter_line_inplace<SH>(buf, N, A.scale, tid, 256);
     if (y >= rows) return;
     float2 *o = out + (size_t)y * N;
-    if (A.prefilter) {
+    if (MODE == MODE_PLAIN) {
+        for (int x = tid; x < N; x += 256) o[x] = buf[lp<SH>(x)];
+    } else if (A.prefilter) {
         for (int x = tid; x < N; x += 256) o[x] = cconj(buf[lp<SH>(x)]);
     } else {
         const float sc = (float)A.scale;
@@ -636,8 +639,9 @@ void transpose(const float2 *in, float2 *out, int R, int C, int planes, hipStrea
 }  // namespace
 
 namespace {
-// The inverse's second pass (length n = wp over hp lines) from the [kx][y] planes with
+// A second pass (lines of length n, `rows` of them) straight from the first pass's planes with
 // k_lines_ct1_t, when n has a compile-time form; false otherwise (the caller transposes).
+template <int MODE>
 bool second_pass_from_columns(const float2 *in, float2 *out, int n, int rows, int planes, const LineArgs &A,
                               hipStream_t s) {
     const FftPlan &P = fft_plan(n, s);
@@ -647,10 +651,10 @@ bool second_pass_from_columns(const float2 *in, float2 *out, int n, int rows, in
     if (n == N_) {                                                                                                  \
         static std::once_flag once_t;                                                                               \
         std::call_once(once_t, [] {                                                                                 \
-            (void)hipFuncSetAttribute((const void *)k_lines_ct1_t<4, N_, __VA_ARGS__>,                              \
+            (void)hipFuncSetAttribute((const void *)k_lines_ct1_t<MODE, 4, N_, __VA_ARGS__>,                        \
                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);                      \
         });                                                                                                         \
-        hipLaunchKernelGGL((k_lines_ct1_t<4, N_, __VA_ARGS__>), dim3(slots, planes), dim3(1024),                    \
+        hipLaunchKernelGGL((k_lines_ct1_t<MODE, 4, N_, __VA_ARGS__>), dim3(slots, planes), dim3(1024),              \
                            TL * lp_len<4>(N_) * sizeof(float2), s, in, out, P.ct_tw, P.tw, rows, groups, A);        \
         LAUNCHCHECK();                                                                                              \
         return true;                                                                                                \
@@ -668,8 +672,10 @@ void fft_forward_padded(const float *img, int w, int h, float2 *spec, float2 *tm
     A.w = w;
     A.h = h;
     launch_lines<MODE_FWD_REFLECT>(nullptr, tmp, wp, hp, 1, nullptr, 0, A, s);  // [y][kx]
-    transpose(tmp, tmp2, hp, wp, 1, s);                                        // [kx][y]
-    launch_lines<MODE_PLAIN>(tmp2, spec, hp, wp, 1, nullptr, 0, A, s);         // [kx][ky]
+    A.in_plane = A.out_plane = 0;
+    if (second_pass_from_columns<MODE_PLAIN>(tmp, spec, hp, wp, 1, A, s)) return;  // [kx][ky]
+    transpose(tmp, tmp2, hp, wp, 1, s);                                         // [kx][y]
+    launch_lines<MODE_PLAIN>(tmp2, spec, hp, wp, 1, nullptr, 0, A, s);          // [kx][ky]
 }
 
 void fft_inverse_filtered(const float2 *spec, int hp, int wp, const Bank &bank, int k0, int K, SpecMul mul,
@@ -684,7 +690,7 @@ void fft_inverse_filtered(const float2 *spec, int hp, int wp, const Bank &bank, 
     A.out_plane = np;
     launch_lines<MODE_INV_FIRST>(spec, tmp, hp, wp, K, &bank, k0, A, s);  // [kx][y] per filter
     A.in_plane = np;
-    if (second_pass_from_columns(tmp, out, wp, hp, K, A, s)) return;
+    if (second_pass_from_columns<MODE_INV_SECOND>(tmp, out, wp, hp, K, A, s)) return;
     transpose(tmp, tmp2, wp, hp, K, s);  // [y][kx]
     launch_lines<MODE_INV_SECOND>(tmp2, out, wp, hp, K, nullptr, 0, A, s);
 }

@@ -67,6 +67,75 @@ __global__ void k_prefilter_cols_anti(const float *__restrict__ mid, float *__re
     }
 }
 
+// Two planes per launch (blockIdx.z): the spline of each pre-warp component from the component
+// itself, src -> mid -> dst, so the copy into dst comes for free.
+struct PlanePair {
+    const float *src[2];
+    float *mid[2], *dst[2];
+};
+__global__ void k_prefilter_cols_causal2(PlanePair P, int w, int h) {
+    const int z = blockIdx.z;
+    const float *__restrict__ src = z ? P.src[1] : P.src[0];
+    float *__restrict__ mid = z ? P.mid[1] : P.mid[0];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (x >= w) return;
+    const int a = j * PF_SEG, b = min(a + PF_SEG, h);
+    double acc = 0;
+    int i;
+    if (a <= PF_WARM) {
+        const int hz = min(h, PF_HORIZON);
+        double zk = 1;
+        for (int k = 0; k < hz; k++) {
+            acc += zk * 6.0 * (double)src[(size_t)k * w + x];
+            zk *= ZP;
+        }
+        if (a == 0) mid[x] = (float)acc;
+        i = 1;
+    } else {
+        i = a - PF_WARM;
+    }
+    for (; i < a; i++) acc = 6.0 * (double)src[(size_t)i * w + x] + ZP * acc;
+    for (i = max(a, 1); i < b; i++) {
+        acc = 6.0 * (double)src[(size_t)i * w + x] + ZP * acc;
+        mid[(size_t)i * w + x] = (float)acc;
+    }
+}
+
+__global__ void k_prefilter_cols_anti2(PlanePair P, int w, int h) {
+    const int z = blockIdx.z;
+    const float *__restrict__ mid = z ? P.mid[1] : P.mid[0];
+    float *__restrict__ dst = z ? P.dst[1] : P.dst[0];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    if (x >= w) return;
+    const int a = j * PF_SEG, b = min(a + PF_SEG, h);
+    double acc = 0;
+    int i;
+    if (b + PF_WARM >= h) {
+        acc = (ZP / (ZP * ZP - 1.0)) * ((double)mid[(size_t)(h - 1) * w + x] + ZP * (double)mid[(size_t)(h - 2) * w + x]);
+        if (b == h) dst[(size_t)(h - 1) * w + x] = (float)acc;
+        i = h - 2;
+    } else {
+        i = b + PF_WARM - 1;
+    }
+    for (; i >= b; i--) acc = ZP * (acc - (double)mid[(size_t)i * w + x]);
+    for (i = min(b - 1, h - 2); i >= a; i--) {
+        acc = ZP * (acc - (double)mid[(size_t)i * w + x]);
+        dst[(size_t)i * w + x] = (float)acc;
+    }
+}
+
+// Along x: one workgroup per row, segmented recursion in LDS.
+__global__ void __launch_bounds__(256) k_prefilter_rows2(PlanePair P, int w) {
+    extern __shared__ float smem[];
+    constexpr int SH = 4;
+    float *a = smem, *b = smem + lp_len<SH>(w);
+    float *row = (blockIdx.y ? P.dst[1] : P.dst[0]) + (size_t)blockIdx.x * w;
+    for (int x = threadIdx.x; x < w; x += blockDim.x) a[lp<SH>(x)] = row[x];
+    __syncthreads();
+    prefilter_lines<float, SH>(a, b, a, w, 1, 1.0, threadIdx.x, blockDim.x);
+    for (int x = threadIdx.x; x < w; x += blockDim.x) row[x] = a[lp<SH>(x)];
+}
+
 // Along x: one workgroup per row, segmented recursion in LDS.
 __global__ void __launch_bounds__(256) k_prefilter_rows(float *c, int w) {
     extern __shared__ float smem[];
@@ -1402,6 +1471,23 @@ void prefilter_real(float *c, float *tmp, int w, int h, hipStream_t s) {
     }
     LAUNCHCHECK();
     hipLaunchKernelGGL(k_prefilter_rows, dim3(h), dim3(256), 2 * lp_len<4>(w) * sizeof(float), s, c, w);
+    LAUNCHCHECK();
+}
+
+void prefilter_real2(const float *src0, const float *src1, float *dst0, float *dst1, float *tmp, int w, int h,
+                     hipStream_t s) {
+    const size_t n = (size_t)w * h;
+    const PlanePair P{{src0, src1}, {tmp, tmp + n}, {dst0, dst1}};
+    if (h > 1) {
+        const dim3 g((w + 255) / 256, (h + PF_SEG - 1) / PF_SEG, 2);
+        hipLaunchKernelGGL(k_prefilter_cols_causal2, g, dim3(256), 0, s, P, w, h);
+        hipLaunchKernelGGL(k_prefilter_cols_anti2, g, dim3(256), 0, s, P, w, h);
+    } else {
+        dcopy(dst0, src0, n * sizeof(float), s);
+        dcopy(dst1, src1, n * sizeof(float), s);
+    }
+    LAUNCHCHECK();
+    hipLaunchKernelGGL(k_prefilter_rows2, dim3(h, 2), dim3(256), 2 * lp_len<4>(w) * sizeof(float), s, P, w);
     LAUNCHCHECK();
 }
 

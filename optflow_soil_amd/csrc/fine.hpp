@@ -7,6 +7,10 @@ namespace ofx {
 // fine.hip
 // cubic B-spline coefficients of a real plane in place (tmp: w x h scratch)
 void prefilter_real(float *c, float *tmp, int w, int h, hipStream_t s);
+// The splines of two planes, src0 -> dst0 and src1 -> dst1 (sources kept), in the same launches;
+// tmp: 2 w h floats.
+void prefilter_real2(const float *src0, const float *src1, float *dst0, float *dst1, float *tmp, int w, int h,
+                     hipStream_t s);
 // taps_host: the 2r + 1 weights on the host (passed as a kernel argument)
 void blur(const double *in, double *tmp, float *out, int w, int h, const double *taps_host, int r, hipStream_t s);
 // out = I2((x + shx) + sx, (y + shy) + sy) from the spline c2 (w2 x h2) of image 2.
