@@ -660,6 +660,11 @@ bool second_pass_from_columns(const float2 *in, float2 *out, int n, int rows, in
         return true;                                                                                                \
     }
     OFX_CT_T_LAUNCH(2112, 12, 11)
+    OFX_CT_T_LAUNCH(576, 12, 3)
+    OFX_CT_T_LAUNCH(832, 4, 13)
+    OFX_CT_T_LAUNCH(1600, 4, 5, 5)
+    // 1088 = 16 4 17 and 1216 = 16 4 19 keep the transpose: their radix-17 and -19 butterflies need
+    // 78 VGPRs here, too many for two 1024-thread workgroups per CU
 #undef OFX_CT_T_LAUNCH
     return false;
 }

@@ -22,17 +22,23 @@ dim3 grid2(int w, int h) { return dim3((w + 255) / 256, h); }
 // summaries and sweeps its own rows (a tie between the two keeps the smaller row).
 constexpr int CHUNK = 32;
 
+// Both kernels hold a chunk's feature flags in registers: all CHUNK loads are issued before the
+// first is used, and the sweeps run on registers (one store per output plane).
 __global__ void k_col_summary(const uint8_t *__restrict__ feat, int w, int h, int *__restrict__ first,
                               int *__restrict__ last) {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
     if (x >= w) return;
-    const int y0 = c * CHUNK, y1 = min(y0 + CHUNK, h);
+    const int y0 = c * CHUNK, rows = min(CHUNK, h - y0);
+    bool f8[CHUNK];
+#pragma unroll
+    for (int k = 0; k < CHUNK; k++) f8[k] = k < rows && feat[(size_t)(y0 + k) * w + x] != 0;
     int f = -1, l = -1;
-    for (int y = y0; y < y1; y++)
-        if (feat[(size_t)y * w + x]) {
-            if (f < 0) f = y;
-            l = y;
-        }
+#pragma unroll
+    for (int k = CHUNK - 1; k >= 0; k--)
+        if (f8[k]) f = y0 + k;
+#pragma unroll
+    for (int k = 0; k < CHUNK; k++)
+        if (f8[k]) l = y0 + k;
     first[(size_t)c * w + x] = f;
     last[(size_t)c * w + x] = l;
 }
@@ -43,20 +49,27 @@ __global__ void k_column_nearest(const uint8_t *__restrict__ feat, const int *__
     const int x = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
     if (x >= w) return;
     const int nch = (h + CHUNK - 1) / CHUNK;
+    const int y0 = c * CHUNK, rows = min(CHUNK, h - y0);
+    bool f8[CHUNK];
+#pragma unroll
+    for (int k = 0; k < CHUNK; k++) f8[k] = k < rows && feat[(size_t)(y0 + k) * w + x] != 0;
     int up = -1, down = -1;
     for (int cc = c - 1; cc >= 0 && up < 0; cc--) up = last[(size_t)cc * w + x];
     for (int cc = c + 1; cc < nch && down < 0; cc++) down = first[(size_t)cc * w + x];
-    const int y0 = c * CHUNK, y1 = min(y0 + CHUNK, h);
-    for (int y = y0; y < y1; y++) {
-        const size_t i = (size_t)y * w + x;
-        if (feat[i]) up = y;
-        col_row[i] = up;
+    int ups[CHUNK];
+#pragma unroll
+    for (int k = 0; k < CHUNK; k++) {
+        if (f8[k]) up = y0 + k;
+        ups[k] = up;
     }
-    for (int y = y1 - 1; y >= y0; y--) {
-        const size_t i = (size_t)y * w + x;
-        if (feat[i]) down = y;
-        const int u = col_row[i];
+#pragma unroll
+    for (int k = CHUNK - 1; k >= 0; k--) {
+        if (k >= rows) continue;
+        const int y = y0 + k;
+        if (f8[k]) down = y;
+        const int u = ups[k];
         const int du = u >= 0 ? y - u : 0x7fffffff, dd = down >= 0 ? down - y : 0x7fffffff;
+        const size_t i = (size_t)y * w + x;
         if (u < 0 && down < 0) {
             col_d2[i] = -1;
             col_row[i] = -1;
