@@ -226,6 +226,9 @@ def main():
         "config": {"workload": desc, "pixels_per_step": w * h},
         "roofline": {"kernel": "k_accumulate", "bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "traffic_source": ("stored measurement: profiles/accumulate_traffic_2048.json, two rocprofv3 "
+                                        "--pmc passes (FETCH_SIZE, WRITE_SIZE) scaled by a 1 GiB copy")
+                     if traffic is not None else None,
                      "algorithmic_bytes_per_launch": acc_bytes, "avg_launch_ms": round(acc_avg_s * 1e3, 4)},
         "roofline_moments": moments_roof,
         "stage_ms_per_step": stages,

@@ -377,7 +377,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
             const int e = threadIdx.x + 256 * q;
             if (e < bn) box[0][e] = resp[boff[q]];
         }
-        float4 pc = inside ? refp[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        // the reference pack of filter f + 1 is loaded unconditionally (a pixel outside the frame
+        // reads pixel 0), so that no register copy waits for it before filter f is evaluated
+        float4 pc = refp[i];
         for (int f = 0; f < nf; f++) {
             __syncthreads();
             float2 nx[ACC_BOX / 256 + 1];
@@ -389,7 +391,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
                     const int e = threadIdx.x + 256 * q;
                     if (e < bn) nx[q] = plane[boff[q]];
                 }
-                if (inside) pn = refp[(size_t)(f + 1) * n + i];
+                pn = refp[(size_t)(f + 1) * n + i];
             }
             if (inside) {
                 const float2 *bb = box[f & 1];
