@@ -262,6 +262,54 @@ __device__ __forceinline__ float weight_of(float re, float im, float s2) {
     return sqrtf(fmaf(re, re, fmaf(im, im, 1e-12f * s2)));
 }
 
+// One filter of a pixel (phaseflow.rs:1247-1401): the bicubic sample of the response box bb
+// (tap offsets o), the phase difference against the reference pack p = (re, im, gx, gy), the
+// weight, c = dphi + gx u + gy v and the seven sums s = gxx, gxy, gyy, gxc, gyc, gcc, gw.
+__device__ __forceinline__ void acc_filter(const float2 *bb, const int (&o)[16], const Taps &qx, const Taps &qy,
+                                           const float4 p, double u, double v, double (&s)[7]) {
+    float r2x = 0.f, r2y = 0.f;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        float ax = 0.f, ay = 0.f;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const float2 t = bb[o[4 * j + k]];
+            ax = fmaf(qx.w[k], t.x, ax);
+            ay = fmaf(qx.w[k], t.y, ay);
+        }
+        r2x = fmaf(qy.w[j], ax, r2x);
+        r2y = fmaf(qy.w[j], ay, r2y);
+    }
+    const float re = p.x * r2x + p.y * r2y;
+    const float im = p.y * r2x - p.x * r2y;
+    const double dphi = (double)atan2f(im, re);
+    const double wk = (double)weight_of(re, im, fmaf(r2x, r2x, r2y * r2y));
+    const double gx = p.z, gy = p.w;
+    const double c = dphi + gx * u + gy * v;
+    const double wgx = wk * gx, wgy = wk * gy;
+    s[0] += wgx * gx;
+    s[1] += wgx * gy;
+    s[2] += wgy * gy;
+    s[3] += wgx * c;
+    s[4] += wgy * c;
+    s[5] += wk * c * c;
+    s[6] += wk;
+}
+
+// Stages a box of nodes in 16-byte pairs with the load-to-LDS path of gfx950: pair p = tid + 256 q
+// (plane offset poff[q], p < npairs) lands at dst + 2 p, lane l of a wavefront writing 16 bytes at
+// the wavefront's base (wbase = its first thread) + 16 l.
+template <int PQ>
+__device__ __forceinline__ void stage_pairs(const float2 *plane, float2 *dst, const int (&poff)[PQ], int npairs,
+                                            int wbase) {
+#pragma unroll
+    for (int q = 0; q < PQ; q++)
+        if ((int)threadIdx.x + 256 * q < npairs)
+            __builtin_amdgcn_global_load_lds((const void *)(plane + poff[q]),
+                                             (__attribute__((address_space(3))) void *)(dst + 2 * (256 * q + wbase)), 16,
+                                             0, 0);
+}
+
 // Workgroups are dealt to the chip's 8 XCDs in turn (dispatch index L runs on XCD L mod 8) and
 // each XCD has its own L2. Each row of tiles is split into 8 chunks of c = ceil(gx / 8) adjacent
 // tiles, chunk x going to XCD x: neighbouring tiles, whose staged boxes overlap and share cache
@@ -275,13 +323,13 @@ __device__ __forceinline__ void xcd_tile(int L, int gx, int &tx, int &ty) {
 
 // Six workgroups per CU: the two staged boxes (25.6 KB of LDS) allow six, and 80 VGPRs (81 without
 // the hint) let six wavefronts share each SIMD.
+template <bool DMA>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_accumulate_tiled(const double *__restrict__ ux, const double *__restrict__ uy,
                                                           const float *__restrict__ sx, const float *__restrict__ sy,
                                                           const float *__restrict__ csx, const float *__restrict__ csy,
                                                           const float2 *__restrict__ resp,
                                                           const float4 *__restrict__ refp, double *__restrict__ sums,
                                                           int w, int h, int wp, int hp, int nf) {
-    __shared__ float2 box[2][ACC_BOX];
     __shared__ int red[4][8];
     __shared__ int geom[4];
     const int lx = threadIdx.x & (ACC_T - 1), ly = threadIdx.x >> 4;
@@ -345,15 +393,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
             d = max(d, red[k][3]);
             e = min(e, red[k][4]);
         }
-        const int bw = b - a + 1, bh = d - c + 1;
-        geom[0] = a;
+        // DMA: the box starts on an even column and has an even width, so that it is staged in
+        // 16-byte pairs of nodes; its extra column must stay inside the padded row
+        const int bx = DMA ? (a & ~1) : a, bw = DMA ? ((b - bx + 2) & ~1) : b - a + 1, bh = d - c + 1;
+        geom[0] = bx;
         geom[1] = c;
         geom[2] = bw;
-        geom[3] = (e && bw > 0 && bh > 0 && (long)bw * bh <= ACC_BOX) ? bh : -1;
+        geom[3] = (e && bw > 0 && bh > 0 && (long)bw * bh <= ACC_BOX && (!DMA || bx + bw <= wp)) ? bh : -1;
     }
     __syncthreads();
     const int bx0 = geom[0], by0 = geom[1], bw = geom[2], bh = geom[3];
-    double gxx = 0, gxy = 0, gyy = 0, gxc = 0, gyc = 0, gcc = 0, gw = 0;
+    // the seven sums gxx, gxy, gyy, gxc, gyc, gcc, gw
+    double sm[7] = {0, 0, 0, 0, 0, 0, 0};
     if (bh > 0) {
         const int bn = bw * bh;
         // tap offsets inside the box (pixels outside the frame evaluate nothing)
@@ -363,6 +414,56 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
         for (int j = 0; j < 4; j++)
 #pragma unroll
             for (int k = 0; k < 4; k++) o[4 * j + k] = (oy + j) * bw + ox + k;
+        // the reference pack of filter f + 1 is loaded unconditionally (a pixel outside the frame
+        // reads pixel 0), so that no register copy waits for it before filter f is evaluated
+        float4 pc = refp[i];
+        if constexpr (DMA) {
+            // Staged by the load-to-LDS path of gfx950: pair p = tid + 256 q of the box (nodes
+            // 2 p, 2 p + 1, one row) goes from the plane straight to LDS, lane l of a wavefront
+            // writing 16 bytes at the wavefront's base + 16 l, so no register holds the next box.
+            // The pair offsets in a plane are the same for every filter; the plane and the pairs
+            // are 16-byte aligned (even padded width, even box column and width). The two boxes
+            // are separate LDS arrays and the filter loop runs in pairs, so that the compiler sees
+            // that the staging of one box does not touch the box being read and does not wait for
+            // it there.
+            __shared__ float2 box0[ACC_BOX], box1[ACC_BOX];
+            constexpr int PQ = (ACC_BOX / 2 + 255) / 256;
+            const int hw = bw >> 1, npairs = bn >> 1, wbase = threadIdx.x & ~63;
+            int poff[PQ];
+#pragma unroll
+            for (int q = 0; q < PQ; q++) {
+                const int p = threadIdx.x + 256 * q;
+                const int r = p / hw, c = p - r * hw;
+                poff[q] = p < npairs ? (by0 + r) * wp + bx0 + 2 * c : 0;
+            }
+            stage_pairs<PQ>(resp, box0, poff, npairs, wbase);
+            for (int f = 0; f < nf; f += 2) {
+                // filter f from box0 while filter f + 1 is staged into box1, then filter f + 1
+                // from box1 while filter f + 2 is staged into box0. Before each: every load of
+                // this thread done (the box to read and the pack of its filter), then every
+                // thread's (that box complete, the other one no longer read).
+                __builtin_amdgcn_s_waitcnt(0);
+                __syncthreads();
+                float4 pn = pc;
+                if (f + 1 < nf) {
+                    stage_pairs<PQ>(resp + (size_t)(f + 1) * np, box1, poff, npairs, wbase);
+                    pn = refp[(size_t)(f + 1) * n + i];
+                }
+                if (inside) acc_filter(box0, o, qx, qy, pc, u, v, sm);
+                pc = pn;
+                if (f + 1 < nf) {
+                    __builtin_amdgcn_s_waitcnt(0);
+                    __syncthreads();
+                    if (f + 2 < nf) {
+                        stage_pairs<PQ>(resp + (size_t)(f + 2) * np, box0, poff, npairs, wbase);
+                        pn = refp[(size_t)(f + 2) * n + i];
+                    }
+                    if (inside) acc_filter(box1, o, qx, qy, pc, u, v, sm);
+                    pc = pn;
+                }
+            }
+        } else {
+        __shared__ float2 box[2][ACC_BOX];
         // offsets in a response plane of this thread's box elements e = tid + 256 q, the same for
         // every filter (a plane holds fewer than 2^31 nodes: the sides are at most 7744)
         int boff[ACC_BOX / 256 + 1];
@@ -377,9 +478,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
             const int e = threadIdx.x + 256 * q;
             if (e < bn) box[0][e] = resp[boff[q]];
         }
-        // the reference pack of filter f + 1 is loaded unconditionally (a pixel outside the frame
-        // reads pixel 0), so that no register copy waits for it before filter f is evaluated
-        float4 pc = refp[i];
         for (int f = 0; f < nf; f++) {
             __syncthreads();
             float2 nx[ACC_BOX / 256 + 1];
@@ -393,37 +491,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
                 }
                 pn = refp[(size_t)(f + 1) * n + i];
             }
-            if (inside) {
-                const float2 *bb = box[f & 1];
-                float r2x = 0.f, r2y = 0.f;
-#pragma unroll
-                for (int j = 0; j < 4; j++) {
-                    float ax = 0.f, ay = 0.f;
-#pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        const float2 t = bb[o[4 * j + k]];
-                        ax = fmaf(qx.w[k], t.x, ax);
-                        ay = fmaf(qx.w[k], t.y, ay);
-                    }
-                    r2x = fmaf(qy.w[j], ax, r2x);
-                    r2y = fmaf(qy.w[j], ay, r2y);
-                }
-                const float4 p = pc;
-                const float re = p.x * r2x + p.y * r2y;
-                const float im = p.y * r2x - p.x * r2y;
-                const double dphi = (double)atan2f(im, re);
-                const double wk = (double)weight_of(re, im, fmaf(r2x, r2x, r2y * r2y));
-                const double gx = p.z, gy = p.w;
-                const double c = dphi + gx * u + gy * v;
-                const double wgx = wk * gx, wgy = wk * gy;
-                gxx += wgx * gx;
-                gxy += wgx * gy;
-                gyy += wgy * gy;
-                gxc += wgx * c;
-                gyc += wgy * c;
-                gcc += wk * c * c;
-                gw += wk;
-            }
+            if (inside) acc_filter(box[f & 1], o, qx, qy, pc, u, v, sm);
             if (f + 1 < nf) {
                 float2 *dst = box[(f + 1) & 1];
 #pragma unroll
@@ -433,6 +501,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
                 }
             }
             pc = pn;
+        }
         }
     } else if (inside) {
         for (int f = 0; f < nf; f++) {
@@ -445,23 +514,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
             const double gx = p.z, gy = p.w;
             const double c = dphi + gx * u + gy * v;
             const double wgx = wk * gx, wgy = wk * gy;
-            gxx += wgx * gx;
-            gxy += wgx * gy;
-            gyy += wgy * gy;
-            gxc += wgx * c;
-            gyc += wgy * c;
-            gcc += wk * c * c;
-            gw += wk;
+            sm[0] += wgx * gx;
+            sm[1] += wgx * gy;
+            sm[2] += wgy * gy;
+            sm[3] += wgx * c;
+            sm[4] += wgy * c;
+            sm[5] += wk * c * c;
+            sm[6] += wk;
         }
     }
     if (!inside) return;
-    sums[0 * n + i] = gxx;
-    sums[1 * n + i] = gxy;
-    sums[2 * n + i] = gyy;
-    sums[3 * n + i] = gxc;
-    sums[4 * n + i] = gyc;
-    sums[5 * n + i] = gcc;
-    sums[6 * n + i] = gw;
+#pragma unroll
+    for (int k = 0; k < 7; k++) sums[k * n + i] = sm[k];
 }
 
 // ------------------------------------------------------------------ window moments (order 1)
@@ -1523,8 +1587,13 @@ void accumulate(const double *ux, const double *uy, const float *sx, const float
                 hipStream_t s) {
     const int wp = w + 2 * PAD, hp = h + 2 * PAD;
     const int gx = (w + ACC_T - 1) / ACC_T, gy = (h + ACC_T - 1) / ACC_T;
-    hipLaunchKernelGGL(k_accumulate_tiled, dim3(8 * ((gx + 7) / 8) * gy), dim3(256), 0, s,
-                       ux, uy, sx, sy, csx, csy, resp, refp, sums, w, h, wp, hp, F);
+    // an even padded width keeps every 16-byte pair of nodes aligned for the load-to-LDS staging
+    if (wp % 2 == 0)
+        hipLaunchKernelGGL(k_accumulate_tiled<true>, dim3(8 * ((gx + 7) / 8) * gy), dim3(256), 0, s,
+                           ux, uy, sx, sy, csx, csy, resp, refp, sums, w, h, wp, hp, F);
+    else
+        hipLaunchKernelGGL(k_accumulate_tiled<false>, dim3(8 * ((gx + 7) / 8) * gy), dim3(256), 0, s,
+                           ux, uy, sx, sy, csx, csy, resp, refp, sums, w, h, wp, hp, F);
     LAUNCHCHECK();
 }
 
