@@ -31,7 +31,12 @@ __global__ void k_col_summary(const uint8_t *__restrict__ feat, int w, int h, in
     const int y0 = c * CHUNK, rows = min(CHUNK, h - y0);
     bool f8[CHUNK];
 #pragma unroll
-    for (int k = 0; k < CHUNK; k++) f8[k] = k < rows && feat[(size_t)(y0 + k) * w + x] != 0;
+    for (int k = 0; k < CHUNK; k++) {
+        // every row loaded (the chunk's last row again past the frame), so that the loads are not
+        // split by a branch and are all in flight together
+        const bool v = feat[(size_t)(y0 + min(k, rows - 1)) * w + x] != 0;
+        f8[k] = k < rows && v;
+    }
     int f = -1, l = -1;
 #pragma unroll
     for (int k = CHUNK - 1; k >= 0; k--)
@@ -52,7 +57,12 @@ __global__ void k_column_nearest(const uint8_t *__restrict__ feat, const int *__
     const int y0 = c * CHUNK, rows = min(CHUNK, h - y0);
     bool f8[CHUNK];
 #pragma unroll
-    for (int k = 0; k < CHUNK; k++) f8[k] = k < rows && feat[(size_t)(y0 + k) * w + x] != 0;
+    for (int k = 0; k < CHUNK; k++) {
+        // every row loaded (the chunk's last row again past the frame), so that the loads are not
+        // split by a branch and are all in flight together
+        const bool v = feat[(size_t)(y0 + min(k, rows - 1)) * w + x] != 0;
+        f8[k] = k < rows && v;
+    }
     int up = -1, down = -1;
     for (int cc = c - 1; cc >= 0 && up < 0; cc--) up = last[(size_t)cc * w + x];
     for (int cc = c + 1; cc < nch && down < 0; cc++) down = first[(size_t)cc * w + x];
