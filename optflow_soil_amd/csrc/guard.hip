@@ -80,16 +80,11 @@ __global__ void k_column_nearest(const uint8_t *__restrict__ feat, const int *__
         const int u = ups[k];
         const int du = u >= 0 ? y - u : 0x7fffffff, dd = down >= 0 ? down - y : 0x7fffffff;
         const size_t i = (size_t)y * w + x;
-        if (u < 0 && down < 0) {
-            col_d2[i] = -1;
-            col_row[i] = -1;
-        } else if (du <= dd) {  // a tie keeps the smaller row
-            col_d2[i] = du * du;
-            col_row[i] = u;
-        } else {
-            col_d2[i] = dd * dd;
-            col_row[i] = down;
-        }
+        // by selects, so that the stores of the chunk are not split by branches; a tie keeps the
+        // smaller row, no feature in the column gives -1
+        const bool none = u < 0 && down < 0, above = du <= dd;
+        col_d2[i] = none ? -1 : above ? du * du : dd * dd;
+        col_row[i] = none ? -1 : above ? u : down;
     }
 }
 
