@@ -668,9 +668,11 @@ __device__ __forceinline__ void solve_pixel(const double (&mp)[3][3][3], const d
         M[a][a] += RIDGE * tr / 6.0;
         b0[a] = bv[a];
     }
-    // Cholesky (lower triangle in place), then L y = b, L^T x = y; one division per column (the
-    // reciprocal of its diagonal) instead of one per element. A diagonal that is not positive gives a
-    // NaN reciprocal, so the result is not finite and the pixel falls back, as with the divisions.
+    // Cholesky (lower triangle in place), then L y = b, L^T x = y; one reciprocal square root per
+    // column (the reciprocal of its diagonal, within an ulp or two of 1 / sqrt) instead of a
+    // division per element; the diagonal itself is never used again. A diagonal that is not
+    // positive gives a NaN or infinite reciprocal, so the result is not finite and the pixel falls
+    // back, as with the divisions.
     double il[6];
 #pragma unroll
     for (int j = 0; j < 6; j++) {
@@ -678,9 +680,7 @@ __device__ __forceinline__ void solve_pixel(const double (&mp)[3][3][3], const d
 #pragma unroll
         for (int k = 0; k < j; k++) d -= M[j][k] * M[j][k];
         ok = ok && isfinite(d) && d > 0;
-        const double l = sqrt(d);
-        il[j] = 1.0 / l;
-        M[j][j] = l;
+        il[j] = rsqrt(d);
 #pragma unroll
         for (int a = j + 1; a < 6; a++) {
             double s = M[a][j];
