@@ -369,27 +369,30 @@ __global__ void k_strain_y(const double *__restrict__ mx, const double *__restri
     for (int q = 0; q < 11; q++) out[q * n + i] = a[q];
 }
 
-// strain.rs:64-108: Cholesky with a relative pivot test, two right-hand sides
+// strain.rs:64-108: Cholesky with a relative pivot test, two right-hand sides. Each column's
+// diagonal is kept as its reciprocal (one reciprocal square root, within an ulp or two of
+// 1 / sqrt), so every division of the factorisation and of the two substitutions is a multiply.
 __device__ bool solve6(double m[6][6], double rhs[6][2], double c[6][2]) {
-    double l[6][6];
+    double l[6][6], il[6];
 #pragma unroll
     for (int i = 0; i < 6; i++)
 #pragma unroll
         for (int j = 0; j < 6; j++) l[i][j] = 0;
 #pragma unroll
-    for (int i = 0; i < 6; i++)
+    for (int j = 0; j < 6; j++) {
+        double d = m[j][j];
 #pragma unroll
-        for (int j = 0; j <= i; j++) {
+        for (int k = 0; k < j; k++) d -= l[j][k] * l[j][k];
+        if (isnan(d) || d <= 1e-12 * m[j][j]) return false;
+        il[j] = rsqrt(d);
+#pragma unroll
+        for (int i = j + 1; i < 6; i++) {
             double s = m[i][j];
 #pragma unroll
             for (int k = 0; k < j; k++) s -= l[i][k] * l[j][k];
-            if (i == j) {
-                if (isnan(s) || s <= 1e-12 * m[i][i]) return false;
-                l[i][i] = sqrt(s);
-            } else {
-                l[i][j] = s / l[j][j];
-            }
+            l[i][j] = s * il[j];
         }
+    }
 #pragma unroll
     for (int col = 0; col < 2; col++) {
         double yv[6];
@@ -398,14 +401,14 @@ __device__ bool solve6(double m[6][6], double rhs[6][2], double c[6][2]) {
             double s = rhs[i][col];
 #pragma unroll
             for (int k = 0; k < i; k++) s -= l[i][k] * yv[k];
-            yv[i] = s / l[i][i];
+            yv[i] = s * il[i];
         }
 #pragma unroll
         for (int i = 5; i >= 0; i--) {
             double s = yv[i];
 #pragma unroll
             for (int k = i + 1; k < 6; k++) s -= l[k][i] * c[k][col];
-            c[i][col] = s / l[i][i];
+            c[i][col] = s * il[i];
         }
     }
     return true;
