@@ -22,10 +22,29 @@ constexpr double ZP = -0.26794919243112270647;  // sqrt(3) - 2
 // Along y: one thread per (column, segment of PF_SEG rows), coalesced across the wavefront; each
 // segment starts its recursion PF_WARM rows early (error z^PF_WARM, as the LDS form) in double.
 // Causal pass src -> mid, anti-causal pass mid -> dst (dst may be src).
-__global__ void k_prefilter_cols_causal(const float *__restrict__ src, float *__restrict__ mid, int w, int h) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (x >= w) return;
+// One (column, segment) of the causal pass, src -> mid. An interior segment (a full PF_SEG rows,
+// its warm-up inside the frame) issues all PF_WARM + PF_SEG loads before the recursion uses them;
+// the loads of a walk in a loop would each wait for the one before.
+__device__ __forceinline__ void causal_segment(const float *__restrict__ src, float *__restrict__ mid, int w, int h,
+                                               int x, int j) {
     const int a = j * PF_SEG, b = min(a + PF_SEG, h);
+    if (a > PF_WARM && b - a == PF_SEG) {
+        constexpr int NV = PF_WARM + PF_SEG;
+        float v[NV];
+        const float *p = src + (size_t)(a - PF_WARM) * w + x;
+#pragma unroll
+        for (int k = 0; k < NV; k++) v[k] = p[(size_t)k * w];
+        double acc = 0;
+#pragma unroll
+        for (int k = 0; k < PF_WARM; k++) acc = 6.0 * (double)v[k] + ZP * acc;
+        float *q = mid + (size_t)a * w + x;
+#pragma unroll
+        for (int k = 0; k < PF_SEG; k++) {
+            acc = 6.0 * (double)v[PF_WARM + k] + ZP * acc;
+            q[(size_t)k * w] = (float)acc;
+        }
+        return;
+    }
     double acc = 0;
     int i;
     if (a <= PF_WARM) {
@@ -47,10 +66,28 @@ __global__ void k_prefilter_cols_causal(const float *__restrict__ src, float *__
     }
 }
 
-__global__ void k_prefilter_cols_anti(const float *__restrict__ mid, float *__restrict__ dst, int w, int h) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (x >= w) return;
+// One (column, segment) of the anti-causal pass, mid -> dst, interior segments with their loads
+// issued first as above.
+__device__ __forceinline__ void anti_segment(const float *__restrict__ mid, float *__restrict__ dst, int w, int h,
+                                             int x, int j) {
     const int a = j * PF_SEG, b = min(a + PF_SEG, h);
+    if (b + PF_WARM < h && b - a == PF_SEG) {
+        constexpr int NV = PF_SEG + PF_WARM;
+        float v[NV];  // v[k] is row a + k
+        const float *p = mid + (size_t)a * w + x;
+#pragma unroll
+        for (int k = 0; k < NV; k++) v[k] = p[(size_t)k * w];
+        double acc = 0;
+#pragma unroll
+        for (int k = NV - 1; k >= PF_SEG; k--) acc = ZP * (acc - (double)v[k]);
+        float *q = dst + (size_t)a * w + x;
+#pragma unroll
+        for (int k = PF_SEG - 1; k >= 0; k--) {
+            acc = ZP * (acc - (double)v[k]);
+            q[(size_t)k * w] = (float)acc;
+        }
+        return;
+    }
     double acc = 0;
     int i;
     if (b + PF_WARM >= h) {
@@ -65,6 +102,16 @@ __global__ void k_prefilter_cols_anti(const float *__restrict__ mid, float *__re
         acc = ZP * (acc - (double)mid[(size_t)i * w + x]);
         dst[(size_t)i * w + x] = (float)acc;
     }
+}
+
+__global__ void k_prefilter_cols_causal(const float *__restrict__ src, float *__restrict__ mid, int w, int h) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < w) causal_segment(src, mid, w, h, x, blockIdx.y);
+}
+
+__global__ void k_prefilter_cols_anti(const float *__restrict__ mid, float *__restrict__ dst, int w, int h) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < w) anti_segment(mid, dst, w, h, x, blockIdx.y);
 }
 
 // Two planes per launch (blockIdx.z): the spline of each pre-warp component from the component
@@ -75,53 +122,14 @@ struct PlanePair {
 };
 __global__ void k_prefilter_cols_causal2(PlanePair P, int w, int h) {
     const int z = blockIdx.z;
-    const float *__restrict__ src = z ? P.src[1] : P.src[0];
-    float *__restrict__ mid = z ? P.mid[1] : P.mid[0];
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (x >= w) return;
-    const int a = j * PF_SEG, b = min(a + PF_SEG, h);
-    double acc = 0;
-    int i;
-    if (a <= PF_WARM) {
-        const int hz = min(h, PF_HORIZON);
-        double zk = 1;
-        for (int k = 0; k < hz; k++) {
-            acc += zk * 6.0 * (double)src[(size_t)k * w + x];
-            zk *= ZP;
-        }
-        if (a == 0) mid[x] = (float)acc;
-        i = 1;
-    } else {
-        i = a - PF_WARM;
-    }
-    for (; i < a; i++) acc = 6.0 * (double)src[(size_t)i * w + x] + ZP * acc;
-    for (i = max(a, 1); i < b; i++) {
-        acc = 6.0 * (double)src[(size_t)i * w + x] + ZP * acc;
-        mid[(size_t)i * w + x] = (float)acc;
-    }
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < w) causal_segment(z ? P.src[1] : P.src[0], z ? P.mid[1] : P.mid[0], w, h, x, blockIdx.y);
 }
 
 __global__ void k_prefilter_cols_anti2(PlanePair P, int w, int h) {
     const int z = blockIdx.z;
-    const float *__restrict__ mid = z ? P.mid[1] : P.mid[0];
-    float *__restrict__ dst = z ? P.dst[1] : P.dst[0];
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
-    if (x >= w) return;
-    const int a = j * PF_SEG, b = min(a + PF_SEG, h);
-    double acc = 0;
-    int i;
-    if (b + PF_WARM >= h) {
-        acc = (ZP / (ZP * ZP - 1.0)) * ((double)mid[(size_t)(h - 1) * w + x] + ZP * (double)mid[(size_t)(h - 2) * w + x]);
-        if (b == h) dst[(size_t)(h - 1) * w + x] = (float)acc;
-        i = h - 2;
-    } else {
-        i = b + PF_WARM - 1;
-    }
-    for (; i >= b; i--) acc = ZP * (acc - (double)mid[(size_t)i * w + x]);
-    for (i = min(b - 1, h - 2); i >= a; i--) {
-        acc = ZP * (acc - (double)mid[(size_t)i * w + x]);
-        dst[(size_t)i * w + x] = (float)acc;
-    }
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    if (x < w) anti_segment(z ? P.mid[1] : P.mid[0], z ? P.dst[1] : P.dst[0], w, h, x, blockIdx.y);
 }
 
 // Along x: one workgroup per row, segmented recursion in LDS.

@@ -107,8 +107,12 @@ __global__ void k_within(const int *__restrict__ col_d2, int w, int h, double th
 // The same test with the row segment a workgroup reaches (its 256 pixels and T on each side)
 // staged in LDS once, instead of 2T + 1 global loads per pixel; positions outside the row read as
 // -1 (no column point), which the loop skips as k_within skips them. For T <= WITHIN_MAXT.
+// The distance test on integers: sqrt(d2) <= thr holds for exactly the squared distances
+// d2 <= B, B the largest integer whose correctly rounded square root is at most thr (B < 0 when
+// thr < 0), found on the host. A squared distance here is at most 256^2 plus a squared column
+// distance (below 7744^2), so 32 bits hold it; a column without a feature (-1) counts as none.
 constexpr int WITHIN_MAXT = 256;
-__global__ void __launch_bounds__(256) k_within_lds(const int *__restrict__ col_d2, int w, double thr, int T,
+__global__ void __launch_bounds__(256) k_within_lds(const int *__restrict__ col_d2, int w, int B, int T,
                                                     uint8_t *__restrict__ out) {
     __shared__ int seg[256 + 2 * WITHIN_MAXT];
     const int x0 = blockIdx.x * 256, y = blockIdx.y;
@@ -121,14 +125,12 @@ __global__ void __launch_bounds__(256) k_within_lds(const int *__restrict__ col_
     const int x = x0 + threadIdx.x;
     if (x >= w) return;
     const int *g0 = seg + threadIdx.x + T;  // g0[dx] is column x + dx
-    long long best = -1;
+    int best = 0x7fffffff;
     for (int dx = -T; dx <= T; dx++) {
         const int g = g0[dx];
-        if (g < 0) continue;
-        const long long d2 = (long long)dx * dx + g;
-        if (best < 0 || d2 < best) best = d2;
+        best = min(best, g < 0 ? 0x7fffffff : dx * dx + g);
     }
-    out[(size_t)y * w + x] = best >= 0 && sqrt((double)best) <= thr;
+    out[(size_t)y * w + x] = best <= B;
 }
 
 __global__ void k_zero_counters(Counters *cnt) {
@@ -522,9 +524,17 @@ size_t column_summary_ints(int w, int h) { return 2 * (size_t)((h + CHUNK - 1) /
 
 void within_distance(const int *col_d2, int w, int h, double thr, uint8_t *within, hipStream_t s) {
     const int T = thr >= 0 ? (int)std::floor(thr) : -1;
-    if (T <= WITHIN_MAXT)
-        hipLaunchKernelGGL(k_within_lds, dim3((w + 255) / 256, h), dim3(256), 0, s, col_d2, w, thr, T, within);
-    else
+    if (T <= WITHIN_MAXT) {
+        // the largest B with sqrt(B) <= thr, from floor(thr^2) corrected by the same correctly
+        // rounded square root the device takes
+        long long B = -1;
+        if (thr >= 0) {
+            B = (long long)std::floor(thr * thr);
+            while (std::sqrt((double)(B + 1)) <= thr) B++;
+            while (B >= 0 && std::sqrt((double)B) > thr) B--;
+        }
+        hipLaunchKernelGGL(k_within_lds, dim3((w + 255) / 256, h), dim3(256), 0, s, col_d2, w, (int)B, T, within);
+    } else
         hipLaunchKernelGGL(k_within, grid2(w, h), dim3(256), 0, s, col_d2, w, h, thr, T, within);
     LAUNCHCHECK();
 }
