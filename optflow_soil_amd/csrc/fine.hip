@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <mutex>
 
 #include "fine.hpp"
 #include "lines.hpp"
@@ -132,16 +133,23 @@ __global__ void k_prefilter_cols_anti2(PlanePair P, int w, int h) {
     if (x < w) anti_segment(z ? P.mid[1] : P.mid[0], z ? P.dst[1] : P.dst[0], w, h, x, blockIdx.y);
 }
 
-// Along x: one workgroup per row, segmented recursion in LDS.
+// Along x: one workgroup per row y, the rows y of both planes side by side in LDS as two lines of
+// one segmented recursion (twice the segments of one row for the 256 threads to walk).
 __global__ void __launch_bounds__(256) k_prefilter_rows2(PlanePair P, int w) {
     extern __shared__ float smem[];
     constexpr int SH = 4;
-    float *a = smem, *b = smem + lp_len<SH>(w);
-    float *row = (blockIdx.y ? P.dst[1] : P.dst[0]) + (size_t)blockIdx.x * w;
-    for (int x = threadIdx.x; x < w; x += blockDim.x) a[lp<SH>(x)] = row[x];
+    float *a = smem, *b = smem + lp_len<SH>(2 * w);
+    float *row0 = P.dst[0] + (size_t)blockIdx.x * w, *row1 = P.dst[1] + (size_t)blockIdx.x * w;
+    for (int x = threadIdx.x; x < w; x += blockDim.x) {
+        a[lp<SH>(x)] = row0[x];
+        a[lp<SH>(w + x)] = row1[x];
+    }
     __syncthreads();
-    prefilter_lines<float, SH>(a, b, a, w, 1, 1.0, threadIdx.x, blockDim.x);
-    for (int x = threadIdx.x; x < w; x += blockDim.x) row[x] = a[lp<SH>(x)];
+    prefilter_lines<float, SH>(a, b, a, w, 2, 1.0, threadIdx.x, blockDim.x);
+    for (int x = threadIdx.x; x < w; x += blockDim.x) {
+        row0[x] = a[lp<SH>(x)];
+        row1[x] = a[lp<SH>(w + x)];
+    }
 }
 
 // Along x: one workgroup per row, segmented recursion in LDS.
@@ -1561,7 +1569,13 @@ void prefilter_real2(const float *src0, const float *src1, float *dst0, float *d
         dcopy(dst1, src1, n * sizeof(float), s);
     }
     LAUNCHCHECK();
-    hipLaunchKernelGGL(k_prefilter_rows2, dim3(h, 2), dim3(256), 2 * lp_len<4>(w) * sizeof(float), s, P, w);
+    // two rows of up to 4032 samples in two buffers: up to 68.5 KB, above the 64 KB default
+    static std::once_flag once;
+    std::call_once(once, [] {
+        (void)hipFuncSetAttribute((const void *)k_prefilter_rows2, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  160 * 1024);
+    });
+    hipLaunchKernelGGL(k_prefilter_rows2, dim3(h), dim3(256), 2 * lp_len<4>(2 * w) * sizeof(float), s, P, w);
     LAUNCHCHECK();
 }
 
