@@ -1420,18 +1420,17 @@ __global__ void __launch_bounds__(256) k_select_hist(const double *__restrict__ 
             atomicAdd(&lh[bin], 1u);
         }
     };
-    // four loads in flight per thread, then the remainder one at a time; every lane of a wavefront
+    // eight loads in flight per thread, then the remainder one at a time; every lane of a wavefront
     // runs the same trip counts, so the ballots see the whole wavefront
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     const size_t first = (size_t)blockIdx.x * blockDim.x + (threadIdx.x & ~63u), lane = threadIdx.x & 63;
     size_t i = first;
-    for (; i + 3 * stride + 63 < n; i += 4 * stride) {
-        const double x0 = v[i + lane], x1 = v[i + stride + lane], x2 = v[i + 2 * stride + lane],
-                     x3 = v[i + 3 * stride + lane];
-        count(x0, true);
-        count(x1, true);
-        count(x2, true);
-        count(x3, true);
+    for (; i + 7 * stride + 63 < n; i += 8 * stride) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) x[u] = v[i + u * stride + lane];
+#pragma unroll
+        for (int u = 0; u < 8; u++) count(x[u], true);
     }
     for (; i < n; i += stride) {
         const bool live = i + lane < n;
@@ -1673,6 +1672,18 @@ void energy_reset(const double *energy, const double *med, const double *ux_in, 
 
 size_t median_scratch_bytes() { return sizeof(SelectState); }
 
+// Workgroups of a histogram pass (512, two per CU): each ends with up to 256 atomic adds into the one global
+// histogram, so fewer and longer workgroups (eight loads in flight per thread keep HBM busy)
+// (OFX_SELECT_BLOCKS overrides, for timing).
+int hist_blocks(size_t n) {
+    static const int cap = [] {
+        const char *v = std::getenv("OFX_SELECT_BLOCKS");
+        const int b = v ? std::atoi(v) : 0;
+        return b > 0 ? b : 512;
+    }();
+    return (int)std::max<size_t>(1, std::min<size_t>(cap, (n + 255) / 256));
+}
+
 
 // numpy.median of n values (NaN if any is NaN), for an even n the mean of the n/2 - 1 and n/2 order
 // statistics (core/select.rs:19-42).
@@ -1681,8 +1692,8 @@ void median(const double *v, size_t n, void *scratch, double *out, hipStream_t s
     const int even = (n % 2 == 0) ? 1 : 0;
     SelectState *st = (SelectState *)scratch;
     hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, k);
-    const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
-    for (int p = 0; p < 8; p++) hipLaunchKernelGGL(k_select_hist, dim3(blocks), dim3(256), 0, s, v, n, p, st, 0);
+    const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256), hblocks = hist_blocks(n);
+    for (int p = 0; p < 8; p++) hipLaunchKernelGGL(k_select_hist, dim3(hblocks), dim3(256), 0, s, v, n, p, st, 0);
     if (even) hipLaunchKernelGGL(k_select_below, dim3(blocks), dim3(256), 0, s, v, n, st, 0);
     hipLaunchKernelGGL(k_select_finish, dim3(1), dim3(64), 0, s, st, k, even, out, 0);
     LAUNCHCHECK();
@@ -1742,11 +1753,11 @@ void median_energy_reset(const double *energy, size_t n, void *scratch, double *
     const int even = (n % 2 == 0) ? 1 : 0;
     SelectState *st = (SelectState *)scratch;
     hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, k);
-    const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256);
-    for (int p = 0; p < 2; p++) hipLaunchKernelGGL(k_select_hist, dim3(blocks), dim3(256), 0, s, energy, n, p, st, 0);
+    const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256), hblocks = hist_blocks(n);
+    for (int p = 0; p < 2; p++) hipLaunchKernelGGL(k_select_hist, dim3(hblocks), dim3(256), 0, s, energy, n, p, st, 0);
     hipLaunchKernelGGL(k_energy_reset_bounded, dim3(blocks), dim3(256), 0, s, energy, st, even, ux_in, uy_in, ux, uy,
                        resid, slope, n);
-    for (int p = 2; p < 8; p++) hipLaunchKernelGGL(k_select_hist, dim3(blocks), dim3(256), 0, s, energy, n, p, st, 1);
+    for (int p = 2; p < 8; p++) hipLaunchKernelGGL(k_select_hist, dim3(hblocks), dim3(256), 0, s, energy, n, p, st, 1);
     if (even) hipLaunchKernelGGL(k_select_below, dim3(blocks), dim3(256), 0, s, energy, n, st, 1);
     hipLaunchKernelGGL(k_select_finish, dim3(1), dim3(64), 0, s, st, k, even, med, 1);
     hipLaunchKernelGGL(k_energy_reset, dim3((n + 255) / 256), dim3(256), 0, s, energy, (const double *)med, ux_in,

@@ -142,8 +142,21 @@ __global__ void __launch_bounds__(256) k_trusted(const uint8_t *__restrict__ low
                                                  size_t n, uint8_t *__restrict__ ok, Counters *cnt) {
     __shared__ unsigned int red[2][4];
     unsigned int nl = 0, no = 0;
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
-        const uint8_t o = !within_r[i];  // farther than R from every low pixel
+    // four pixels per step as 32-bit words (the byte planes are allocations of their own, so their
+    // starts are aligned), the last n mod 4 pixels one at a time. nz(v) has bit 7 of each byte set
+    // where that byte of v is not zero (no carry crosses a byte).
+    auto nz = [](unsigned int v) { return (((v & 0x7f7f7f7fu) + 0x7f7f7f7fu) | v) & 0x80808080u; };
+    const size_t stride = (size_t)gridDim.x * blockDim.x, n4 = n / 4;
+    const unsigned int *l4 = (const unsigned int *)low, *w4 = (const unsigned int *)within_r;
+    unsigned int *o4 = (unsigned int *)ok;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n4; q += stride) {
+        const unsigned int o = (~nz(w4[q]) & 0x80808080u) >> 7;  // farther than R from every low pixel
+        o4[q] = o;
+        nl += __popc(nz(l4[q]));
+        no += __popc(o);
+    }
+    for (size_t i = 4 * n4 + (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        const uint8_t o = !within_r[i];
         ok[i] = o;
         nl += low[i] != 0;
         no += o;
