@@ -1672,9 +1672,9 @@ void energy_reset(const double *energy, const double *med, const double *ux_in, 
 
 size_t median_scratch_bytes() { return sizeof(SelectState); }
 
-// Workgroups of a histogram pass (512, two per CU): each ends with up to 256 atomic adds into the one global
-// histogram, so fewer and longer workgroups (eight loads in flight per thread keep HBM busy)
-// (OFX_SELECT_BLOCKS overrides, for timing).
+// Workgroups of a histogram pass: each ends with up to 256 atomic adds into the one global
+// histogram, so 512 longer workgroups (two per CU, eight loads in flight per thread) rather than
+// 1024. OFX_SELECT_BLOCKS overrides the count, for timing.
 int hist_blocks(size_t n) {
     static const int cap = [] {
         const char *v = std::getenv("OFX_SELECT_BLOCKS");
