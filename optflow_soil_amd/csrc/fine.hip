@@ -225,6 +225,39 @@ __global__ void __launch_bounds__(256) k_blur_x_lds(const double *__restrict__ i
     out[(size_t)y * w + x] = (float)s;
 }
 
+// The x pass for a compile-time radius R, four adjacent outputs per thread (1024 per workgroup):
+// each of a thread's 4 + 2R inputs is read from LDS once and added into the outputs it reaches,
+// taps in the order of the sequential sum. Column c of the segment sits at (c mod 4) Q + c / 4, so
+// the 64 lanes of a wavefront read 64 consecutive words.
+template <int R>
+__global__ void __launch_bounds__(256) k_blur_x_rb(const double *__restrict__ in, float *__restrict__ out, int w,
+                                                   const BT bt) {
+    constexpr int P = 1024, NB = 4, Q = (P + 2 * R + 3) / 4;
+    __shared__ double seg[4 * Q];
+    const int x0 = blockIdx.x * P, y = blockIdx.y;
+    const double *row = in + (size_t)y * w;
+    for (int c = threadIdx.x; c < P + 2 * R; c += 256) seg[(c & 3) * Q + (c >> 2)] = row[min(max(x0 - R + c, 0), w - 1)];
+    __syncthreads();
+    double acc[NB];
+#pragma unroll
+    for (int q = 0; q < NB; q++) acc[q] = 0;
+#pragma unroll
+    for (int j = 0; j < NB + 2 * R; j++) {
+        const double v = seg[(j & 3) * Q + threadIdx.x + (j >> 2)];
+#pragma unroll
+        for (int q = 0; q < NB; q++) {
+            const int t = j - q;
+            if (t >= 0 && t <= 2 * R) acc[q] += bt.k[t] * v;
+        }
+    }
+    float *o = out + (size_t)y * w;
+#pragma unroll
+    for (int q = 0; q < NB; q++) {
+        const int x = x0 + NB * threadIdx.x + q;
+        if (x < w) o[x] = (float)acc[q];
+    }
+}
+
 // ------------------------------------------------------------------ warp of image 2
 // warped(X) = I2((x + shx) + s_x, (y + shy) + s_y) from the spline c2 of image 2 (w2 x h2), the
 // window of the second frame displaced by the shift (phaseflow.rs:1416-1439; a zero shift is
@@ -1587,7 +1620,10 @@ void blur(const double *in, double *tmp, float *out, int w, int h, const double 
                            in, tmp, w, h, bt);
     else
         hipLaunchKernelGGL(k_blur_y, grid2(w, h), dim3(256), 0, s, in, tmp, w, h, bt, r);
-    hipLaunchKernelGGL(k_blur_x_lds, grid2(w, h), dim3(256), 0, s, tmp, out, w, bt, r);
+    if (r == 16)
+        hipLaunchKernelGGL(k_blur_x_rb<16>, dim3((w + 1023) / 1024, h), dim3(256), 0, s, tmp, out, w, bt);
+    else
+        hipLaunchKernelGGL(k_blur_x_lds, grid2(w, h), dim3(256), 0, s, tmp, out, w, bt, r);
     LAUNCHCHECK();
 }
 
