@@ -136,56 +136,39 @@ __device__ float2 *fft2_lds(float2 *X, float2 *Y, int wy, int wx, const PocArgs 
 // (row pass) and columns handed to consecutive lanes (column pass) both spread over the banks.
 constexpr int T64 = 65;
 
-// Radix-8 Stockham pass (p = P) along the rows: butterfly item -> row item % 64, index item / 64.
-template <int P>
-__device__ __forceinline__ void pass64_rows(const float2 *src, float2 *dst, const float2 *tw, int tid, int nt) {
-    for (int item = tid; item < 512; item += nt) {
-        const int row = item & 63, i = item >> 6;
-        const int k = i % P;
-        float2 x[8];
+// Radix-8 Stockham pass (p = P) of the 64 x 64 tile along its rows (COLS false) or columns, in
+// place, for a workgroup of exactly 512 threads: butterfly item -> line item % 64, index item / 64.
+// Every thread reads its butterfly's 8 inputs, the workgroup waits, every thread writes its 8
+// outputs over the same tile, and the workgroup waits again.
+template <int P, bool COLS>
+__device__ __forceinline__ void pass64_inplace(float2 *buf, const float2 *tw) {
+    const int item = threadIdx.x;
+    const int line = item & 63, i = item >> 6;
+    const int k = i % P;
+    float2 x[8];
 #pragma unroll
-        for (int q = 0; q < 8; q++) x[q] = src[row * T64 + i + 8 * q];
-        if constexpr (P > 1) {
+    for (int q = 0; q < 8; q++) x[q] = COLS ? buf[(i + 8 * q) * T64 + line] : buf[line * T64 + i + 8 * q];
+    if constexpr (P > 1) {
 #pragma unroll
-            for (int q = 1; q < 8; q++) x[q] = cmul(x[q], tw[q * k]);
-        }
-        dft8(x);
-        const int j = (i / P) * P * 8 + k;
-#pragma unroll
-        for (int q = 0; q < 8; q++) dst[row * T64 + j + q * P] = x[q];
+        for (int q = 1; q < 8; q++) x[q] = cmul(x[q], tw[q * k]);
     }
+    dft8(x);
+    const int j = (i / P) * P * 8 + k;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        if (COLS) buf[(j + q * P) * T64 + line] = x[q];
+        else buf[line * T64 + j + q * P] = x[q];
+    }
+    __syncthreads();
 }
 
-// The same along the columns: butterfly item -> column item % 64, index item / 64.
-template <int P>
-__device__ __forceinline__ void pass64_cols(const float2 *src, float2 *dst, const float2 *tw, int tid, int nt) {
-    for (int item = tid; item < 512; item += nt) {
-        const int col = item & 63, i = item >> 6;
-        const int k = i % P;
-        float2 x[8];
-#pragma unroll
-        for (int q = 0; q < 8; q++) x[q] = src[(i + 8 * q) * T64 + col];
-        if constexpr (P > 1) {
-#pragma unroll
-            for (int q = 1; q < 8; q++) x[q] = cmul(x[q], tw[q * k]);
-        }
-        dft8(x);
-        const int j = (i / P) * P * 8 + k;
-#pragma unroll
-        for (int q = 0; q < 8; q++) dst[(j + q * P) * T64 + col] = x[q];
-    }
-}
-
-// Forward 2-D DFT of the 64 x 64 tile X (natural layout in and out, result in X; Y scratch).
-__device__ __forceinline__ void fft64x64(float2 *X, float2 *Y, const float2 *tw) {
-    pass64_rows<1>(X, Y, tw, threadIdx.x, blockDim.x);
-    __syncthreads();
-    pass64_rows<8>(Y, X, tw, threadIdx.x, blockDim.x);
-    __syncthreads();
-    pass64_cols<1>(X, Y, tw, threadIdx.x, blockDim.x);
-    __syncthreads();
-    pass64_cols<8>(Y, X, tw, threadIdx.x, blockDim.x);
-    __syncthreads();
+// Forward 2-D DFT of the 64 x 64 tile X in place (natural layout in and out), 512 threads.
+__device__ __forceinline__ void fft64x64_inplace(float2 *X, const float2 *tw) {
+    pass64_inplace<1, false>(X, tw);
+    pass64_inplace<8, false>(X, tw);
+    pass64_inplace<1, true>(X, tw);
+    pass64_inplace<8, true>(X, tw);
 }
 
 __device__ double block_sum(double v, double *red) {
@@ -254,9 +237,12 @@ __global__ void __launch_bounds__(POC_NT) k_poc_grid(const float *__restrict__ I
     extern __shared__ float2 lds[];
     const int wy = WIN ? WIN : P.wy, wx = WIN ? WIN : P.wx;
     const int n = wy * wx;
-    const int tile = WIN ? WIN * T64 : n;  // LDS elements of one buffer
+    // WIN = 64: one tile, transformed in place; Y is a 64-element area for the block sums and the
+    // argmax partials. Otherwise two buffers of n elements (the second the transforms' scratch).
+    const int tile = WIN ? WIN * T64 : n;  // LDS elements of X
+    const int ytile = WIN ? 64 : n;
     float2 *X = lds, *Y = lds + tile;
-    float2 *twx = Y + tile, *twy = WIN ? twx : twx + wx;
+    float2 *twx = Y + ytile, *twy = WIN ? twx : twx + wx;
     float *hy = (float *)(twy + wy), *hx = hy + wy;
     for (int i = threadIdx.x; i < wx; i += blockDim.x) twx[i] = twx_g[i];
     if (!WIN)
@@ -275,7 +261,7 @@ __global__ void __launch_bounds__(POC_NT) k_poc_grid(const float *__restrict__ I
     {
         float2 *r;
         if constexpr (WIN == 64) {
-            fft64x64(X, Y, twx);
+            fft64x64_inplace(X, twx);
             r = X;
         } else {
             r = fft2_lds(X, Y, wy, wx, P, twx, twy);
@@ -307,7 +293,7 @@ __global__ void __launch_bounds__(POC_NT) k_poc_grid(const float *__restrict__ I
         load_window(I2, P.W, wy0, wx0, wy, wx, WIN ? T64 : wx, hy, hx, X, red);
         float2 *B;
         if constexpr (WIN == 64) {
-            fft64x64(X, Y, twx);
+            fft64x64_inplace(X, twx);
             B = X;
         } else {
             B = fft2_lds(X, Y, wy, wx, P, twx, twy);
@@ -338,7 +324,7 @@ __global__ void __launch_bounds__(POC_NT) k_poc_grid(const float *__restrict__ I
         float2 *r2;
         int pitch;
         if constexpr (WIN == 64) {
-            fft64x64(X, Y, twx);  // forward transform of the conjugate = conjugate of the inverse
+            fft64x64_inplace(X, twx);  // forward transform of the conjugate = conjugate of the inverse
             r2 = X;
             pitch = T64;
         } else {
@@ -775,8 +761,9 @@ void poc_level(const float *a, const float *b, int lw, int lh, int wy, int wx, c
     hipLaunchKernelGGL(k_hann, dim3((wy + wx + 255) / 256), dim3(256), 0, s, ws->hann, wy, wx);
     LAUNCHCHECK();
     if (wy == 64 && wx == 64) {
-        // 67.6 KiB: above the 64 KiB default, within the 160 KiB of a CDNA4 workgroup
-        const size_t lds = (2 * (size_t)64 * T64 + 64) * sizeof(float2) + 128 * sizeof(float);
+        // one 64 x 64 tile (transformed in place), the 64-element scratch area, twiddles and Hann
+        // tables: 34 KiB, so three or four workgroups share a CU
+        const size_t lds = ((size_t)64 * T64 + 64 + 64) * sizeof(float2) + 128 * sizeof(float);
         static std::once_flag once;
         std::call_once(once, [] {
             (void)hipFuncSetAttribute((const void *)k_poc_grid<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
