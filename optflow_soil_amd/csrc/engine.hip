@@ -529,8 +529,7 @@ void refine_device(Workspace &ws, const float *i1, const float *i2, int w, int h
         fft_inverse_filtered(spec, hp, wp, bank, 0, F, MUL_G, false, tmp, tmp2, resp, s);
         fft_inverse_filtered(spec, hp, wp, bank, 0, F, MUL_G_IFX, false, tmp, tmp2, resp + F * np, s);
         fft_inverse_filtered(spec, hp, wp, bank, 0, F, MUL_G_IFY, false, tmp, tmp2, resp + 2 * F * np, s);
-        for (int k = 0; k < F; k++)
-            refpack(resp + k * np, resp + (F + k) * np, resp + (2 * F + k) * np, refp + k * n, w, h, wp, s);
+        refpack(resp, F, refp, w, h, wp, s);
         if (cache_i1) {
             ws.pack_valid = true;
             ws.pack_w = w;

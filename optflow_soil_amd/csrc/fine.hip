@@ -276,17 +276,18 @@ __global__ void k_warp(const float *__restrict__ c2, int w2, int h2, double shx,
 // Per pixel and filter: r, and the phase gradient Im(conj(r) r_x) / (|r|^2 + 1e-12) (likewise y)
 // (phaseflow.py:100-111; gabor.rs:230-269). Stored as float4 (re, im, gx, gy); the amplitude
 // sqrt(|r|^2 + 1e-12) is recomputed where it is used.
-__global__ void k_refpack(const float2 *__restrict__ r, const float2 *__restrict__ rx, const float2 *__restrict__ ry,
-                          float4 *__restrict__ out, int w, int h, int wp) {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+__global__ void k_refpack(const float2 *__restrict__ resp, float4 *__restrict__ out, int w, int h, int wp, int F,
+                          size_t np) {
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, f = blockIdx.z;
     if (x >= w) return;
     const size_t j = (size_t)(y + PAD) * wp + (x + PAD);
-    const float2 a = r[j], bx = rx[j], by = ry[j];
+    // filter f's planes r, rx, ry: planes f, F + f, 2F + f of the 3F inverses
+    const float2 a = resp[f * np + j], bx = resp[(F + f) * np + j], by = resp[(2 * F + f) * np + j];
     const double re = a.x, im = a.y;
     const double a2 = re * re + im * im + 1e-12;
     const double gx = (re * (double)bx.y - im * (double)bx.x) / a2;
     const double gy = (re * (double)by.y - im * (double)by.x) / a2;
-    out[(size_t)y * w + x] = make_float4(a.x, a.y, (float)gx, (float)gy);
+    out[(size_t)f * w * h + (size_t)y * w + x] = make_float4(a.x, a.y, (float)gx, (float)gy);
 }
 
 // ------------------------------------------------------------------ accumulation
@@ -1634,8 +1635,9 @@ void warp(const float *c2, int w2, int h2, long shx, long shy, const float *sx, 
     LAUNCHCHECK();
 }
 
-void refpack(const float2 *r, const float2 *rx, const float2 *ry, float4 *out, int w, int h, int wp, hipStream_t s) {
-    hipLaunchKernelGGL(k_refpack, grid2(w, h), dim3(256), 0, s, r, rx, ry, out, w, h, wp);
+void refpack(const float2 *resp, int F, float4 *out, int w, int h, int wp, hipStream_t s) {
+    const size_t np = (size_t)wp * (h + 2 * PAD);
+    hipLaunchKernelGGL(k_refpack, dim3((w + 255) / 256, h, F), dim3(256), 0, s, resp, out, w, h, wp, F, np);
     LAUNCHCHECK();
 }
 

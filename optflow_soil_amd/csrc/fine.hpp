@@ -16,7 +16,9 @@ void blur(const double *in, double *tmp, float *out, int w, int h, const double 
 // out = I2((x + shx) + sx, (y + shy) + sy) from the spline c2 (w2 x h2) of image 2.
 void warp(const float *c2, int w2, int h2, long shx, long shy, const float *sx, const float *sy, float *out, int w,
           int h, hipStream_t s);
-void refpack(const float2 *r, const float2 *rx, const float2 *ry, float4 *out, int w, int h, int wp, hipStream_t s);
+// the reference pack of all F filters from the 3F inverses (r, rx, ry planes of filter f at f,
+// F + f, 2F + f), one launch
+void refpack(const float2 *resp, int F, float4 *out, int w, int h, int wp, hipStream_t s);
 void accumulate(const double *ux, const double *uy, const float *sx, const float *sy, const float *csx,
                 const float *csy, const float2 *resp, const float4 *refp, double *sums, int w, int h, int F,
                 hipStream_t s);
