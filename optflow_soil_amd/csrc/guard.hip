@@ -354,20 +354,23 @@ __global__ void k_validity(const uint8_t *__restrict__ low, const uint8_t *__res
 
 // ---------------------------------------------------------------- strain (strain.py:61-112)
 // y pass with zero padding: 11 planes, m t^0..4 | u t^0..2 | v t^0..2 (m = valid, u, v zero-filled)
-__global__ void k_strain_y(const double *__restrict__ mx, const double *__restrict__ my, int w, int h, int r,
+// R > 0: the radius at compile time (the loop unrolled, every load issued before the sums use
+// them); R = 0: the radius r at run time.
+template <int R>
+__global__ void k_strain_y(const double *__restrict__ mx, const double *__restrict__ my, int w, int h, int r_rt,
                            double *__restrict__ out) {
+    const int r = R > 0 ? R : r_rt;
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     if (x >= w) return;
     const size_t n = (size_t)w * h;
     double a[11];
 #pragma unroll
     for (int q = 0; q < 11; q++) a[q] = 0;
-    for (int j = -r; j <= r; j++) {
+    // row y + j of the window; rows outside the frame count as invalid pixels (zero terms, which
+    // leave every sum as it is)
+    auto add = [&](int j, double u, double v) {
         const int yy = y + j;
-        if (yy < 0 || yy >= h) continue;
-        const size_t i = (size_t)yy * w + x;
-        const double u = mx[i], v = my[i];
-        const bool ok = isfinite(u) && isfinite(v);
+        const bool ok = yy >= 0 && yy < h && isfinite(u) && isfinite(v);
         const double m = ok ? 1.0 : 0.0, u0 = ok ? u : 0.0, v0 = ok ? v : 0.0;
         const double t = j, t2 = t * t;
         const double tp[5] = {1.0, t, t2, t2 * t, t2 * t2};
@@ -377,6 +380,23 @@ __global__ void k_strain_y(const double *__restrict__ mx, const double *__restri
         for (int q = 0; q < 3; q++) {
             a[5 + q] += tp[q] * u0;
             a[8 + q] += tp[q] * v0;
+        }
+    };
+    if constexpr (R > 0) {
+        // every row of the window loaded (clamped to the frame) before the sums use them
+        double us[2 * R + 1], vs[2 * R + 1];
+#pragma unroll
+        for (int k = 0; k <= 2 * R; k++) {
+            const size_t i = (size_t)min(max(y - R + k, 0), h - 1) * w + x;
+            us[k] = mx[i];
+            vs[k] = my[i];
+        }
+#pragma unroll
+        for (int k = 0; k <= 2 * R; k++) add(k - R, us[k], vs[k]);
+    } else {
+        for (int j = -r; j <= r; j++) {
+            const size_t i = (size_t)min(max(y + j, 0), h - 1) * w + x;
+            add(j, mx[i], my[i]);
         }
     }
     const size_t i = (size_t)y * w + x;
@@ -600,7 +620,10 @@ void validity(const uint8_t *low, const uint8_t *mask, const uint8_t *near_src, 
 void strain(const double *mx, const double *my, int w, int h, int win, double *ytmp, float *ux_out, float *uy_out,
             uint8_t *valid, float *exx, float *eyy, float *exy, float *gmax, float *vol, hipStream_t s) {
     const int r = win / 2;
-    hipLaunchKernelGGL(k_strain_y, grid2(w, h), dim3(256), 0, s, mx, my, w, h, r, ytmp);
+    if (r == 7)  // the default 15 px window
+        hipLaunchKernelGGL(k_strain_y<7>, grid2(w, h), dim3(256), 0, s, mx, my, w, h, r, ytmp);
+    else
+        hipLaunchKernelGGL(k_strain_y<0>, grid2(w, h), dim3(256), 0, s, mx, my, w, h, r, ytmp);
     const size_t lds = (size_t)11 * (256 + 2 * r) * sizeof(double);
     if (lds <= 65536)
         hipLaunchKernelGGL(k_strain_x<true>, grid2(w, h), dim3(256), lds, s, ytmp, mx, my, w, h, r, win, ux_out, uy_out,
