@@ -1494,15 +1494,15 @@ __global__ void __launch_bounds__(256) k_select_below(const double *__restrict__
             mx = k > mx ? k : mx;
         }
     };
-    // four loads in flight per thread, then the remainder one at a time
+    // eight loads in flight per thread, then the remainder one at a time
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    for (; i + 3 * stride < n; i += 4 * stride) {
-        const double x0 = v[i], x1 = v[i + stride], x2 = v[i + 2 * stride], x3 = v[i + 3 * stride];
-        below(x0);
-        below(x1);
-        below(x2);
-        below(x3);
+    for (; i + 7 * stride < n; i += 8 * stride) {
+        double x[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) x[u] = v[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < 8; u++) below(x[u]);
     }
     for (; i < n; i += stride) below(v[i]);
     // wavefront reduction, then one atomic per workgroup
@@ -1732,7 +1732,7 @@ void median(const double *v, size_t n, void *scratch, double *out, hipStream_t s
     hipLaunchKernelGGL(k_select_init, dim3(1), dim3(256), 0, s, st, k);
     const int blocks = (int)std::min<size_t>(1024, (n + 255) / 256), hblocks = hist_blocks(n);
     for (int p = 0; p < 8; p++) hipLaunchKernelGGL(k_select_hist, dim3(hblocks), dim3(256), 0, s, v, n, p, st, 0);
-    if (even) hipLaunchKernelGGL(k_select_below, dim3(blocks), dim3(256), 0, s, v, n, st, 0);
+    if (even) hipLaunchKernelGGL(k_select_below, dim3(hblocks), dim3(256), 0, s, v, n, st, 0);
     hipLaunchKernelGGL(k_select_finish, dim3(1), dim3(64), 0, s, st, k, even, out, 0);
     LAUNCHCHECK();
 }
@@ -1796,7 +1796,7 @@ void median_energy_reset(const double *energy, size_t n, void *scratch, double *
     hipLaunchKernelGGL(k_energy_reset_bounded, dim3(blocks), dim3(256), 0, s, energy, st, even, ux_in, uy_in, ux, uy,
                        resid, slope, n);
     for (int p = 2; p < 8; p++) hipLaunchKernelGGL(k_select_hist, dim3(hblocks), dim3(256), 0, s, energy, n, p, st, 1);
-    if (even) hipLaunchKernelGGL(k_select_below, dim3(blocks), dim3(256), 0, s, energy, n, st, 1);
+    if (even) hipLaunchKernelGGL(k_select_below, dim3(hblocks), dim3(256), 0, s, energy, n, st, 1);
     hipLaunchKernelGGL(k_select_finish, dim3(1), dim3(64), 0, s, st, k, even, med, 1);
     hipLaunchKernelGGL(k_energy_reset, dim3((n + 255) / 256), dim3(256), 0, s, energy, (const double *)med, ux_in,
                        uy_in, ux, uy, resid, slope, n, (const unsigned int *)&st->undecided);
