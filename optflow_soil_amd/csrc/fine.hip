@@ -236,7 +236,8 @@ __global__ void __launch_bounds__(256) k_blur_x_rb(const double *__restrict__ in
     __shared__ double seg[4 * Q];
     const int x0 = blockIdx.x * P, y = blockIdx.y;
     const double *row = in + (size_t)y * w;
-    for (int c = threadIdx.x; c < P + 2 * R; c += 256) seg[(c & 3) * Q + (c >> 2)] = row[min(max(x0 - R + c, 0), w - 1)];
+    for (int c = threadIdx.x; c < P + 2 * R; c += 256)
+        seg[(c & 3) * Q + (c >> 2)] = row[min(max(x0 - R + c, 0), w - 1)];
     __syncthreads();
     double acc[NB];
 #pragma unroll
@@ -355,9 +356,9 @@ __device__ __forceinline__ void stage_pairs(const float2 *plane, float2 *dst, co
 #pragma unroll
     for (int q = 0; q < PQ; q++)
         if ((int)threadIdx.x + 256 * q < npairs)
-            __builtin_amdgcn_global_load_lds((const void *)(plane + poff[q]),
-                                             (__attribute__((address_space(3))) void *)(dst + 2 * (256 * q + wbase)), 16,
-                                             0, 0);
+            __builtin_amdgcn_global_load_lds(
+                (const void *)(plane + poff[q]),
+                (__attribute__((address_space(3))) void *)(dst + 2 * (256 * q + wbase)), 16, 0, 0);
 }
 
 // Workgroups are dealt to the chip's 8 XCDs in turn (dispatch index L runs on XCD L mod 8) and
@@ -374,7 +375,8 @@ __device__ __forceinline__ void xcd_tile(int L, int gx, int &tx, int &ty) {
 // Six workgroups per CU: the two staged boxes (25.6 KB of LDS) allow six, and 80 VGPRs (81 without
 // the hint) let six wavefronts share each SIMD.
 template <bool DMA>
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k_accumulate_tiled(const double *__restrict__ ux, const double *__restrict__ uy,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6)))
+k_accumulate_tiled(const double *__restrict__ ux, const double *__restrict__ uy,
                                                           const float *__restrict__ sx, const float *__restrict__ sy,
                                                           const float *__restrict__ csx, const float *__restrict__ csy,
                                                           const float2 *__restrict__ resp,
@@ -513,45 +515,45 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) k
                 }
             }
         } else {
-        __shared__ float2 box[2][ACC_BOX];
-        // offsets in a response plane of this thread's box elements e = tid + 256 q, the same for
-        // every filter (a plane holds fewer than 2^31 nodes: the sides are at most 7744)
-        int boff[ACC_BOX / 256 + 1];
+            __shared__ float2 box[2][ACC_BOX];
+            // offsets in a response plane of this thread's box elements e = tid + 256 q, the same
+            // for every filter (a plane holds fewer than 2^31 nodes: the sides are at most 7744)
+            int boff[ACC_BOX / 256 + 1];
 #pragma unroll
-        for (int q = 0; q < ACC_BOX / 256 + 1; q++) {
-            const int e = threadIdx.x + 256 * q;
-            const int r = e / bw, c = e - r * bw;
-            boff[q] = e < bn ? (by0 + r) * wp + bx0 + c : 0;
-        }
-#pragma unroll
-        for (int q = 0; q < ACC_BOX / 256 + 1; q++) {
-            const int e = threadIdx.x + 256 * q;
-            if (e < bn) box[0][e] = resp[boff[q]];
-        }
-        for (int f = 0; f < nf; f++) {
-            __syncthreads();
-            float2 nx[ACC_BOX / 256 + 1];
-            float4 pn = pc;
-            if (f + 1 < nf) {
-                const float2 *plane = resp + (size_t)(f + 1) * np;
-#pragma unroll
-                for (int q = 0; q < ACC_BOX / 256 + 1; q++) {
-                    const int e = threadIdx.x + 256 * q;
-                    if (e < bn) nx[q] = plane[boff[q]];
-                }
-                pn = refp[(size_t)(f + 1) * n + i];
+            for (int q = 0; q < ACC_BOX / 256 + 1; q++) {
+                const int e = threadIdx.x + 256 * q;
+                const int r = e / bw, c = e - r * bw;
+                boff[q] = e < bn ? (by0 + r) * wp + bx0 + c : 0;
             }
-            if (inside) acc_filter(box[f & 1], o, qx, qy, pc, u, v, sm);
-            if (f + 1 < nf) {
-                float2 *dst = box[(f + 1) & 1];
 #pragma unroll
-                for (int q = 0; q < ACC_BOX / 256 + 1; q++) {
-                    const int e = threadIdx.x + 256 * q;
-                    if (e < bn) dst[e] = nx[q];
-                }
+            for (int q = 0; q < ACC_BOX / 256 + 1; q++) {
+                const int e = threadIdx.x + 256 * q;
+                if (e < bn) box[0][e] = resp[boff[q]];
             }
-            pc = pn;
-        }
+            for (int f = 0; f < nf; f++) {
+                __syncthreads();
+                float2 nx[ACC_BOX / 256 + 1];
+                float4 pn = pc;
+                if (f + 1 < nf) {
+                    const float2 *plane = resp + (size_t)(f + 1) * np;
+#pragma unroll
+                    for (int q = 0; q < ACC_BOX / 256 + 1; q++) {
+                        const int e = threadIdx.x + 256 * q;
+                        if (e < bn) nx[q] = plane[boff[q]];
+                    }
+                    pn = refp[(size_t)(f + 1) * n + i];
+                }
+                if (inside) acc_filter(box[f & 1], o, qx, qy, pc, u, v, sm);
+                if (f + 1 < nf) {
+                    float2 *dst = box[(f + 1) & 1];
+#pragma unroll
+                    for (int q = 0; q < ACC_BOX / 256 + 1; q++) {
+                        const int e = threadIdx.x + 256 * q;
+                        if (e < bn) dst[e] = nx[q];
+                    }
+                }
+                pc = pn;
+            }
         }
     } else if (inside) {
         for (int f = 0; f < nf; f++) {

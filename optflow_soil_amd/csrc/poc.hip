@@ -229,7 +229,8 @@ constexpr int POC_PER = POC_MAX / POC_NT;  // spectrum elements a thread keeps
 // transposed ([kx][ky]). Eight wavefronts per SIMD (64 VGPRs, a few spilled): with the 34 KB tile
 // of WIN = 64, four workgroups per CU (445 -> 426 us per 2048^2 grid against three at 78 VGPRs).
 template <int WIN>
-__global__ void __launch_bounds__(POC_NT) __attribute__((amdgpu_waves_per_eu(8))) k_poc_grid(const float *__restrict__ I1, const float *__restrict__ I2,
+__global__ void __launch_bounds__(POC_NT) __attribute__((amdgpu_waves_per_eu(8)))
+k_poc_grid(const float *__restrict__ I1, const float *__restrict__ I2,
                                                      PocArgs P, const float2 *__restrict__ twx_g,
                                                      const float2 *__restrict__ twy_g, const float *__restrict__ hann_g,
                                                      const double *__restrict__ ix, const double *__restrict__ iy,
