@@ -8,8 +8,9 @@ function cites the reference file and line it follows.
 Parity pinning: the `not gpu` tests check every function here against the reference's own golden
 fixtures (tests/golden, from rust/crates/optflow-core/tests/fixtures), against scipy/numpy where a
 function reduces to a library routine (FFT, EDT, labels, correlate1d), and against closed forms
-(plane waves, translations, quadratic fields). `track_sequence` in hybrid mode is not implemented
-here yet (parity unpinned for seq_mode="hybrid").
+(plane waves, translations, quadratic fields). `track_sequence` in hybrid mode is pinned by the
+reference's two hybrid fixtures (fb maps and timing shares identical, ux within 1e-7), its local
+affine gate by a brute-force window sum and by planted outliers. No function here is unpinned.
 """
 import ctypes
 import os
